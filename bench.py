@@ -1,0 +1,192 @@
+"""Benchmark of one SCF iteration's grid build on one MI355X.
+
+A step is one grid build on synthetic input already resident in HBM: from the
+density matrix P to rho, its renormalisation, Slater+VWN, the Becke-Poisson
+Hartree potential, J = Phi^T diag(w V/2) Phi and XC = Phi^T diag(w v_xc) Phi
+(the reference's per-iteration grid work, src/dft.cpp:330-433).  The metric is
+N_pts * N_bf^2 per second.  Default workload: benzene, 6-31G (66 CGFs),
+12 atoms x 75 radial x 302 Lebedev points, lmax 8; P is the core-Hamiltonian
+guess.
+
+Besides the throughput line it reports, for the kernel with the largest share
+of the build, its achieved rate against the MI355X peak (timed with HIP events
+on the stream the kernel runs on), and the reference-faithful CPU restatement
+timed on this host on one full build of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+HARTREE_FLOPS_PER_TERM = 10.0
+
+
+def kernel_work(name, mol_info):
+    """Algorithmic flops of one launch of a build kernel (None if not a compute kernel)."""
+    npts, nbf, nat, nlm = mol_info["npts"], mol_info["nbf"], mol_info["natoms"], mol_info["nlm"]
+    if name == "hartree_cross":
+        return HARTREE_FLOPS_PER_TERM * npts * (nat - 1) * nlm / mol_info["cross_launches"]
+    if name == "contract":
+        return 2 * 2.0 * npts * nbf * nbf  # J and XC, 2 flops per pt*bf^2 each
+    if name == "density":
+        return 2.0 * npts * nbf * nbf + 2.0 * npts * nbf
+    return None
+
+
+def cpu_baseline(mol, nrad, nang, lmax, P):
+    """Time the oracle (reference-faithful FP64 CPU restatement) on one full build."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    g = O.Grid(mol, nrad, nang, lmax)
+    t0 = time.perf_counter()
+    g.set_density(P)
+    g.correct_densities()
+    g.hartree()
+    g.xc()
+    dt = time.perf_counter() - t0
+    return dt, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="benzene_631g")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is designed, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from dftcxx_amd import gridlib as G
+    from dftcxx_amd import molecule as M
+
+    make, nrad, nang = M.WORKLOADS[args.workload]
+    lmax = 8
+    mol = make()
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+
+    # synthetic input: core-Hamiltonian guess density matrix
+    S, T, V = G.one_electron(mol)
+    w, U = np.linalg.eigh(S)
+    X = U / np.sqrt(w)
+    _, Cc = np.linalg.eigh(X.T @ (T + V) @ X)
+    C = X @ Cc
+    P = C[:, :mol.nelec // 2] @ C[:, :mol.nelec // 2].T
+
+    t0 = time.perf_counter()
+    grid = G.MolecularGrid(mol, nrad, nang, lmax)
+    setup_s = time.perf_counter() - t0
+    nbf, npts = grid.nbf, grid.npts
+    stream = torch.cuda.Stream(device=dev)
+    grid.set_stream(stream.cuda_stream)
+    dP = torch.tensor(P.T.copy(), dtype=torch.float64, device=dev)
+    dJ = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
+    dX = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
+    dS = torch.empty(4, dtype=torch.float64, device=dev)
+    ptrs = (dP.data_ptr(), dJ.data_ptr(), dX.data_ptr(), dS.data_ptr())
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        grid.build_device(*ptrs)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    for _ in range(args.steps):
+        grid.build_device(*ptrs)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = e0.elapsed_time(e1)
+    ms_per_step = 1e3 * wall / args.steps
+    units = float(npts) * nbf * nbf
+    value = units * args.steps / wall
+    scal = dS.cpu().numpy()
+
+    out = {
+        "metric": "grid V_xc+J build throughput (N_pts*N_bf^2 per second)",
+        "value": value,
+        "unit": "pts*bf^2/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (core-Hamiltonian guess P)",
+        "config": {"workload": args.workload, "molecule": mol.name, "basis": mol.basis,
+                   "n_bf": nbf, "n_atoms": mol.natoms, "n_radial": nrad, "n_angular": nang,
+                   "lmax": lmax, "n_pts": npts, "precision": "fp64"},
+        "device_ms_per_step": dev_ms / args.steps,
+        "grid_setup_s": setup_s,
+        "e_xc": float(scal[0]),
+        "n_elec": float(scal[2]),
+    }
+
+    if not args.no_profile:
+        grid.profile_enable(True)
+        grid.profile_reset()
+        nprof = max(3, min(args.steps, 10))
+        for _ in range(nprof):
+            grid.build_device(*ptrs)
+        torch.cuda.synchronize()
+        prof = grid.profile_read()
+        grid.profile_enable(False)
+        total = sum(ms for ms, _ in prof.values())
+        out["kernels_ms_per_step"] = {k: ms / nprof for k, (ms, _) in sorted(prof.items())}
+        info = {"npts": npts, "nbf": nbf, "natoms": mol.natoms, "nlm": (lmax + 1) ** 2,
+                "cross_launches": max(1, prof.get("hartree_cross", (0, nprof))[1] // nprof)}
+        dom = max(prof.items(), key=lambda kv: kv[1][0])[0]
+        ms, cnt = prof[dom]
+        avg_s = ms / cnt / 1e3
+        work = kernel_work(dom, info)
+        roof = {"kernel": dom, "share_of_build": prof[dom][0] / total if total else None}
+        if work is not None:
+            ach = work / avg_s / 1e12
+            roof.update({"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS})
+        else:
+            roof.update({"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": None})
+        roof["avg_launch_ms"] = avg_s * 1e3
+        roof["traffic"] = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                t = json.load(f)
+            roof["traffic"] = t.get(dom)
+        out["roofline"] = roof
+
+    if not args.no_cpu_baseline:
+        dt, threads = cpu_baseline(mol, nrad, nang, lmax, P)
+        out["cpu_baseline"] = {"value": units / dt, "unit": "pts*bf^2/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"one full grid build of {args.workload} ({dt:.1f} s)"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

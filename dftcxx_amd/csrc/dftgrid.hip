@@ -1,0 +1,1526 @@
+// MI355X (gfx950) grid-integration library: device kernels and the C ABI.
+//
+// Device data layout (HBM), per handle:
+//   points      SoA px/py/pz/w/wb, npts_pad = npts rounded up to 256, padding
+//               points carry zero weight and zero amplitudes;
+//   amplitudes  phi[npts_pad][ldphi], ldphi = nbf rounded up to 32, zero padded;
+//   Hartree     rho_lm, U [natoms][nrad][nlm]; spline table [natoms][nrad][nlm]{y,D};
+//               partial potentials vpart[natoms][npts_pad];
+//   matrices    nbf x nbf column-major.
+// Kernels (one SCF build, in launch order):
+//   pfrag -> density (FP64 MFMA) -> cell_sum -> scale -> cell_sum -> lda ->
+//   rho_lm -> poisson -> spline -> vcell -> hartree_cross -> vreduce ->
+//   contract (FP64 MFMA) -> contract_reduce -> scalars.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dftgrid.h"
+#include "grid_tables.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace
+
+void dg::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return fail(DG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));         \
+  } while (0)
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kMaxL = 12;  // largest lmax compiled
+constexpr double kPiD = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Setup kernels (once per geometry)
+// ---------------------------------------------------------------------------
+
+// Grid points: atom a, radial node i, Lebedev point j of p = (a*nrad + i)*nang + j.
+// Position R_a + r_i*Omega_j, weight w_i * w_j * |r_i Omega_j|^2 * 4 pi
+// (src/atomicgrid.cpp:50-87).
+__global__ void k_grid(const double* __restrict__ atoms, const double* __restrict__ rad,
+                       const double* __restrict__ wrad, const double* __restrict__ leb, long npts,
+                       long npts_pad, int nrad, int nang, double* px, double* py, double* pz,
+                       double* w) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts_pad) return;
+  if (p >= npts) {
+    px[p] = py[p] = pz[p] = 0.0;
+    w[p] = 0.0;
+    return;
+  }
+  const int j = (int)(p % nang);
+  const long ai = p / nang;
+  const int i = (int)(ai % nrad);
+  const int a = (int)(ai / nrad);
+  const double r = rad[i];
+  const double ox = leb[4 * j] * r, oy = leb[4 * j + 1] * r, oz = leb[4 * j + 2] * r;
+  px[p] = atoms[4 * a] + ox;
+  py[p] = atoms[4 * a + 1] + oy;
+  pz[p] = atoms[4 * a + 2] + oz;
+  double wp = wrad[i] * leb[4 * j + 3];
+  wp *= (ox * ox + oy * oy + oz * oz) * 4.0 * kPiD;
+  w[p] = wp;
+}
+
+// Becke fuzzy-cell weight of every point (src/moleculargrid.cpp:229-329):
+// w_a = P_a / sum_n P_n, P_n = prod_{m != n} s(mu_nm), s = (1 - f(f(f(mu))))/2,
+// f(x) = 1.5x - 0.5x^3, mu_nm = (|r-R_n| - |r-R_m|) / |R_n-R_m|.
+// One wave per block; the point's atom distances live in LDS [natoms][64].
+__global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms,
+                                              const double* __restrict__ invR, int natoms, long npts,
+                                              long cell, const double* __restrict__ px,
+                                              const double* __restrict__ py,
+                                              const double* __restrict__ pz, double* w, double* wb) {
+  extern __shared__ double dist[];
+  const int lane = threadIdx.x;
+  const long p = (long)blockIdx.x * 64 + lane;
+  const bool valid = p < npts;
+  const double x = valid ? px[p] : 0.0, y = valid ? py[p] : 0.0, z = valid ? pz[p] : 0.0;
+  for (int m = 0; m < natoms; ++m) {
+    const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
+    dist[m * 64 + lane] = sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  if (!valid) {
+    if (p < (long)gridDim.x * 64) wb[p] = 0.0;
+    return;
+  }
+  const int own = (int)(p / cell);
+  double denom = 0.0, nom = 1.0;
+  for (int n = 0; n < natoms; ++n) {
+    const double dn = dist[n * 64 + lane];
+    double prod = 1.0;
+    for (int m = 0; m < natoms; ++m) {
+      if (m == n) continue;
+      double mu = (dn - dist[m * 64 + lane]) * invR[n * natoms + m];
+      for (int it = 0; it < 3; ++it) mu = 1.5 * mu - 0.5 * (mu * mu * mu);
+      prod *= 0.5 * (1.0 - mu);
+    }
+    denom += prod;
+    if (n == own) nom = prod;
+  }
+  const double v = nom / denom;
+  wb[p] = v;
+  w[p] *= v;
+}
+
+// Basis amplitudes phi[p][cgf], one thread per (point, shell).  A shell's
+// primitives share one exponential per point; the CGF value is
+// sum_k c_k N_k x^l y^m z^n exp(-alpha_k r^2) (src/cgf.cpp:49-57, 146-154).
+// cn0 = c N for (L,0,0)-type components, cn1 = c N for mixed d components.
+__global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* __restrict__ sh_atom,
+                      const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
+                      const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
+                      const double* __restrict__ alpha, const double* __restrict__ cn0,
+                      const double* __restrict__ cn1, long npts, const double* __restrict__ px,
+                      const double* __restrict__ py, const double* __restrict__ pz, int ldphi,
+                      double* phi) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts * nshell) return;
+  const long p = g / nshell;
+  const int s = (int)(g % nshell);
+  const int a = sh_atom[s];
+  const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double r0 = 0.0, r1 = 0.0;
+  const int off = sh_off[s], np = sh_nprim[s];
+  for (int k = 0; k < np; ++k) {
+    const double e = exp(-alpha[off + k] * r2);
+    r0 += cn0[off + k] * e;
+    r1 += cn1[off + k] * e;
+  }
+  double* out = phi + p * ldphi + sh_cgf[s];
+  switch (sh_L[s]) {
+    case 0:
+      out[0] = r0;
+      break;
+    case 1:
+      out[0] = r0 * dx;
+      out[1] = r0 * dy;
+      out[2] = r0 * dz;
+      break;
+    default:
+      out[0] = r0 * dx * dx;
+      out[1] = r1 * dx * dy;
+      out[2] = r1 * dx * dz;
+      out[3] = r0 * dy * dy;
+      out[4] = r1 * dy * dz;
+      out[5] = r0 * dz * dz;
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Density: rho_p = 2 phi_p^T P phi_p  (src/gridpoint.cpp:82-84)
+// ---------------------------------------------------------------------------
+
+// P rearranged into the A-operand order of the 16x16x4 FP64 matrix-core op:
+// frag[b][s][lane] = P(16b + lane%16, kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4.
+// The same permutation of the reduction index is used for phi in k_density, so
+// the result tile lands on the lanes that hold the matching phi values.
+__global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* frag) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)4 * nt * 64;
+  if (e >= per * nt) return;
+  const int b = (int)(e / per);
+  const int rem = (int)(e % per);
+  const int s = rem / 64, l = rem % 64;
+  const int j = 16 * b + (l & 15);
+  const int k = 16 * (s / 4) + 4 * (l >> 4) + (s & 3);
+  frag[e] = (j < nbf && k < nbf) ? P[j + (long)k * nbf] : 0.0;
+}
+
+// One wave handles 16 points; lane (c = lane%16, g = lane/16) keeps
+// phi[p0+c][kk(s,g)] for every reduction step s in registers (B operand).
+// For output tile b the wave forms T[p][16b+4g+v] = (P phi_p)_j with 4*NT
+// MFMAs, then dots it with phi[p][16b+4g+v].  The P fragments of one tile are
+// staged in LDS and shared by the block's 4 waves.
+template <int NT>
+__global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
+                                                 const double* __restrict__ frag, int nt_used,
+                                                 double* rho) {
+  __shared__ double pl[4 * NT * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const double* row = phi + (p0 + c) * ldphi;
+  double f[4 * NT];
+#pragma unroll
+  for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  double acc = 0.0;
+  for (int b = 0; b < nt_used; ++b) {
+    __syncthreads();
+    const double* src = frag + (long)b * 4 * NT * 64;
+    for (int e = threadIdx.x; e < 4 * NT * 64; e += 256) pl[e] = src[e];
+    __syncthreads();
+    dbl4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4 * NT; ++s) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
+    const double* rb = row + 16 * b + 4 * g;
+    acc += d[0] * rb[0] + d[1] * rb[1] + d[2] * rb[2] + d[3] * rb[3];
+  }
+  acc += __shfl_xor(acc, 16);
+  acc += __shfl_xor(acc, 32);
+  if (g == 0) rho[p0 + c] = 2.0 * acc;
+}
+
+// out[a] = sum over the points of cell a of x*y, fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_cell_sum(const double* __restrict__ x,
+                                                  const double* __restrict__ y, long cell,
+                                                  double* out) {
+  __shared__ double red[256];
+  const long base = (long)blockIdx.x * cell;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < cell; i += 256) s += x[base + i] * y[base + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// rho *= sum(Z) / sum_a q_a  (src/moleculargrid.cpp:132-146).  scalars[1] = sum q.
+__global__ void k_scale(double* rho, long n, const double* __restrict__ q, int natoms, double zsum,
+                        double* scalars) {
+  double qs = 0.0;
+  for (int a = 0; a < natoms; ++a) qs += q[a];
+  const double f = zsum / qs;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) rho[p] *= f;
+  if (p == 0 && scalars) scalars[1] = qs;
+}
+
+// ---------------------------------------------------------------------------
+// LDA: Slater exchange (alpha = 2/3) + VWN correlation (VWN5 parameters,
+// paramagnetic branch: rho_a = rho_b = rho/2 gives zeta = 0), per point
+// (src/functionals.cpp:24-158, src/dft.cpp:407-424).
+// uxc = w (v_xa + v_xb + v_ca + v_cb)/2, exc_part[block] = sum w (e_x + e_c).
+// ---------------------------------------------------------------------------
+struct VwnConst {
+  double A, x0, b, c, Q, X0, k1, k2;
+};
+
+__device__ __forceinline__ void vwn_para(double x, const VwnConst& k, double& eps, double& deps) {
+  // eps(x) = A [ ln(x^2/X) - k2 ln((x-x0)^2/X) + k1 atan(Q/(2x+b)) ],  X = x^2+bx+c,
+  // k1 = (2b/Q)(1 - x0(2x0+b)/X0), k2 = b x0/X0; deps = d eps/dx.
+  const double X = x * x + k.b * x + k.c;
+  const double tx = 2.0 * x + k.b;
+  const double at = atan(k.Q / tx);
+  eps = k.A * (log(x * x / X) - k.k2 * log((x - k.x0) * (x - k.x0) / X) + k.k1 * at);
+  const double qq = tx * tx + k.Q * k.Q;
+  deps = k.A * (2.0 / x - tx / X - 4.0 * k.b / qq -
+                k.k2 * (2.0 / (x - k.x0) - tx / X - 4.0 * (2.0 * k.x0 + k.b) / qq));
+}
+
+__global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho,
+                                             const double* __restrict__ w, long n, VwnConst vk,
+                                             double xfac, double* uxc, double* exd,
+                                             double* exc_part) {
+  __shared__ double red[256];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  double e = 0.0;
+  if (p < n) {
+    const double r = rho[p];
+    const double ra = r * 0.5;
+    double ex = 0.0, vx = 0.0, ec = 0.0, vc = 0.0;
+    if (ra >= 1e-10) {
+      const double r3 = cbrt(ra);
+      ex = 2.0 * (xfac * ra * r3);
+      vx = 4.0 / 3.0 * xfac * r3;
+    }
+    if (r >= 1e-10) {
+      const double x = sqrt(cbrt(3.0 / 4.0 / kPiD / r));  // (3/(4 pi rho))^{1/6}
+      double eps, deps;
+      vwn_para(x, vk, eps, deps);
+      ec = eps * r;
+      vc = eps - (x / 6.0) * deps;
+    }
+    const double wp = w[p];
+    uxc[p] = wp * ((vx + vx + vc + vc) * 0.5);
+    exd[p] = ex + ec;
+    e = wp * (ex + ec);
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) exc_part[blockIdx.x] = red[0];
+}
+
+// scalars[0] = sum of exc_part (fixed tree order); scalars[2] = sum q.
+__global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_part, long nparts,
+                                                 const double* __restrict__ q, int natoms,
+                                                 double* scalars) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long i = threadIdx.x; i < nparts; i += 256) s += exc_part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    scalars[0] = red[0];
+    if (q) {
+      double qs = 0.0;
+      for (int a = 0; a < natoms; ++a) qs += q[a];
+      scalars[2] = qs;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hartree potential (src/atomicgrid.cpp:241-466, src/moleculargrid.cpp:336-389)
+// ---------------------------------------------------------------------------
+
+// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p,  one block per (a, i).
+__global__ void k_rho_lm(const double* __restrict__ rho, const double* __restrict__ wb,
+                         const double* __restrict__ leb, const double* __restrict__ ylm, int nang,
+                         int nlm, double* rho_lm) {
+  const long ai = blockIdx.x;  // a*nrad + i
+  const int lm = threadIdx.x;
+  const long base = ai * nang;
+  if (lm >= nlm) return;
+  double acc = 0.0;
+  for (int j = 0; j < nang; ++j) {
+    const double f = rho[base + j] * leb[4 * j + 3] * wb[base + j];
+    acc += f * ylm[(long)j * nlm + lm];
+  }
+  rho_lm[ai * nlm + lm] = acc * (4.0 * kPiD);
+}
+
+// U[a][i][lm] = sum_k Minv_l[i+1][k] g_k with g_0 = sqrt(4 pi) q_a [lm = 0],
+// g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm], g_{N+1} = 0 (src/atomicgrid.cpp:395-432).
+// minvT[l][k][i] holds rows 1..N of the inverse FD matrix, transposed.
+__global__ void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
+                          const double* __restrict__ rad, const double* __restrict__ minvT, int nrad,
+                          int nlm, double* U) {
+  extern __shared__ double gv[];
+  const int a = blockIdx.x / nlm, lm = blockIdx.x % nlm;
+  int l = 0;
+  while ((l + 1) * (l + 1) <= lm) ++l;
+  const int n2 = nrad + 2;
+  for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+    double v = 0.0;
+    if (k == 0)
+      v = (lm == 0) ? sqrt(4.0 * kPiD) * q[a] : 0.0;
+    else if (k <= nrad)
+      v = -4.0 * kPiD * rad[k - 1] * rho_lm[((long)a * nrad + (k - 1)) * nlm + lm];
+    gv[k] = v;
+  }
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i >= nrad) return;
+  const double* M = minvT + (long)l * n2 * nrad;
+  double acc = 0.0;
+  for (int k = 0; k < n2; ++k) acc += M[(long)k * nrad + i] * gv[k];
+  U[((long)a * nrad + i) * nlm + lm] = acc;
+}
+
+// Not-a-knot spline slopes of U_lm over the ascending knots (reversed radial
+// nodes), one thread per (a, lm) (src/cspline.cpp:66-127,
+// src/atomicgrid.cpp:535-556).  tab[a][t][lm] = {y_t, D_t}.
+__global__ void k_spline(const double* __restrict__ U, const double* __restrict__ kn,
+                         const double* __restrict__ sa, const double* __restrict__ sden,
+                         const double* __restrict__ scmod, int nrad, int nlm, int natoms,
+                         double* tab) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= natoms * nlm) return;
+  const int a = g / nlm, lm = g % nlm;
+  const int n = nrad;
+  auto yv = [&](int t) { return U[((long)a * nrad + (n - 1 - t)) * nlm + lm]; };
+  double* T = tab + (long)a * n * nlm * 2 + lm * 2;
+  const long st = (long)nlm * 2;
+  // right-hand side and forward sweep; y'_t is parked in the D slot
+  double h0 = kn[1] - kn[0], h1 = kn[2] - kn[1];
+  double y0 = yv(0), y1 = yv(1), y2 = yv(2);
+  double r0 = (y1 - y0) / h0, r1 = (y2 - y1) / h1;
+  double Y = r0 * (3 * h0 * h1 + 2 * h1 * h1) + r1 * h0 * h0;
+  double yp = Y / sden[0];
+  T[0] = y0;
+  T[1] = yp;
+  for (int t = 1; t < n - 1; ++t) {
+    const double ym = yv(t - 1), yc = yv(t), yn = yv(t + 1);
+    h0 = kn[t] - kn[t - 1];
+    h1 = kn[t + 1] - kn[t];
+    r0 = (yc - ym) / h0;
+    r1 = (yn - yc) / h1;
+    Y = 3 * (r0 * h1 + r1 * h0);
+    yp = (Y - sa[t] * yp) / sden[t];
+    T[t * st] = yc;
+    T[t * st + 1] = yp;
+  }
+  Y = r0 * h1 * h1 + r1 * (3 * h0 * h1 + 2 * h0 * h0);
+  yp = (Y - sa[n - 1] * yp) / sden[n - 1];
+  T[(n - 1) * st] = yv(n - 1);
+  T[(n - 1) * st + 1] = yp;
+  double D = yp;
+  for (int t = n - 2; t >= 0; --t) {
+    D = T[t * st + 1] - scmod[t] * D;
+    T[t * st + 1] = D;
+  }
+}
+
+// Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
+// (src/atomicgrid.cpp:434-462).  ylmT is [lm][j].
+__global__ void k_vcell(const double* __restrict__ U, const double* __restrict__ ylmT,
+                        const double* __restrict__ atoms, const double* __restrict__ px,
+                        const double* __restrict__ py, const double* __restrict__ pz, long npts,
+                        int nrad, int nang, int nlm, double* vcell) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const int j = (int)(p % nang);
+  const long ai = p / nang;
+  const int a = (int)(ai / nrad);
+  const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
+  const double rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  const double* u = U + ai * nlm;
+  double acc = 0.0;
+  for (int lm = 0; lm < nlm; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
+  vcell[p] = rinv * acc;
+}
+
+struct CrossArgs {
+  const double* px;
+  const double* py;
+  const double* pz;
+  const double* atoms;
+  const double* knots;  // ascending, nrad
+  const double* invh;   // 1/(x_t - x_{t-1})
+  const double* tab;    // [natoms][nrad][nlm]{y, D}
+  const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
+  double* vpart;        // [natoms][npts_pad]
+  long npts, npts_pad, cell, chunk;
+  int nrad, nlm, l0, l1, accumulate;
+};
+
+// Potential of atom k's spherical-harmonic expansion at every point outside
+// cell k: V_k(p) = sum_lm Y_lm(r_p - R_k) S_{k,lm}(|r_p - R_k|) / |r_p - R_k|
+// (src/moleculargrid.cpp:349-376).  Grid: (point chunk, k).  Atom k's spline
+// table for l in [l0, l1) sits in LDS; each thread walks its points, runs the
+// associated-Legendre / Chebyshev recurrences once per (p, k) and forms the
+// four sums sum Y y_{s-1}, sum Y D_{s-1}, sum Y y_s, sum Y D_s, from which the
+// cubic on interval s is evaluated once (linear in the per-lm coefficients).
+// The interval is the reference's first knot with r <= x_s, found by
+// bisection; r outside [x_0, x_max) clamps to the end values (src/cspline.cpp:156-168).
+template <int LMAX>
+__global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
+  extern __shared__ double lds[];
+  const int k = blockIdx.y;
+  const int n = A.nrad;
+  const int lmb = A.l0 * A.l0, nlmc = A.l1 * A.l1 - lmb;
+  double* kn = lds;
+  double* ih = lds + n;
+  double* tab = lds + 2 * n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    kn[i] = A.knots[i];
+    ih[i] = A.invh[i];
+  }
+  const double* src = A.tab + (long)k * n * A.nlm * 2;
+  for (int i = threadIdx.x; i < n * nlmc * 2; i += blockDim.x) {
+    const int t = i / (nlmc * 2), rem = i % (nlmc * 2);
+    tab[i] = src[(long)t * A.nlm * 2 + lmb * 2 + rem];
+  }
+  __syncthreads();
+  const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
+  const long pb = (long)blockIdx.x * A.chunk;
+  const long pe = min(pb + A.chunk, A.npts);
+  const long clo = (long)k * A.cell, chi = clo + A.cell;
+  const double xlo = kn[0], xhi = kn[n - 1];
+  for (long p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+    if (p >= clo && p < chi) continue;
+    const double dx = A.px[p] - Ax, dy = A.py[p] - Ay, dz = A.pz[p] - Az;
+    const double rxy2 = dx * dx + dy * dy;
+    const double r = sqrt(rxy2 + dz * dz);
+    const double rinv = 1.0 / r;
+    const double rxy = sqrt(rxy2);
+    const double ct = dz * rinv, st = rxy * rinv;
+    double cp = 1.0, sp = 0.0;
+    if (rxy > 0.0) {
+      const double ir = 1.0 / rxy;
+      cp = dx * ir;
+      sp = dy * ir;
+    }
+    int s0, s1;
+    double t = 0.0;
+    bool clamp = true;
+    if (r < xlo) {
+      s0 = s1 = 0;
+    } else if (r >= xhi) {
+      s0 = s1 = n - 1;
+    } else {
+      int lo = 1, hi = n - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (r <= kn[mid])
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      s1 = lo;
+      s0 = lo - 1;
+      t = r - kn[s0];
+      clamp = false;
+    }
+    const double* T0 = tab + s0 * nlmc * 2;
+    const double* T1 = tab + s1 * nlmc * 2;
+    double sy0 = 0.0, sd0 = 0.0, sy1 = 0.0, sd1 = 0.0;
+    double pmm = 1.0, cm = 1.0, sm = 0.0;
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) {
+      if (m > 0) {
+        pmm *= -(double)(2 * m - 1) * st;
+        const double c2 = cm * cp - sm * sp;
+        sm = sm * cp + cm * sp;
+        cm = c2;
+      }
+      double p1 = 0.0, p2 = 0.0;
+#pragma unroll
+      for (int l = m; l <= LMAX; ++l) {
+        const int ci = 3 * (l * (l + 1) / 2 + m);
+        double pl;
+        if (l == m)
+          pl = pmm;
+        else if (l == m + 1)
+          pl = ct * (double)(2 * m + 1) * pmm;
+        else
+          pl = A.shc[ci + 1] * ct * p1 - A.shc[ci + 2] * p2;
+        p2 = p1;
+        p1 = pl;
+        if (l >= A.l0 && l < A.l1) {
+          const double base = A.shc[ci] * pl;
+          const int ip = (l * l + l + m - lmb) * 2;
+          const double yp = (m == 0) ? base : base * cm;
+          sy0 += yp * T0[ip];
+          sd0 += yp * T0[ip + 1];
+          sy1 += yp * T1[ip];
+          sd1 += yp * T1[ip + 1];
+          if (m > 0) {
+            const int in = (l * l + l - m - lmb) * 2;
+            const double yn = base * sm;
+            sy0 += yn * T0[in];
+            sd0 += yn * T0[in + 1];
+            sy1 += yn * T1[in];
+            sd1 += yn * T1[in + 1];
+          }
+        }
+      }
+    }
+    double val;
+    if (clamp) {
+      val = sy0;
+    } else {
+      const double hi_ = ih[s1];
+      const double dy = (sy1 - sy0) * hi_;
+      const double c = hi_ * (3.0 * dy - 2.0 * sd0 - sd1);
+      const double d = hi_ * hi_ * (-2.0 * dy + sd0 + sd1);
+      val = sy0 + sd0 * t + c * t * t + d * t * t * t;
+    }
+    double* out = A.vpart + (long)k * A.npts_pad + p;
+    if (A.accumulate)
+      *out += val * rinv;
+    else
+      *out = val * rinv;
+  }
+}
+
+// V[p] = sum_k (k == own ? vcell[p] : vpart[k][p]), k ascending
+// (src/moleculargrid.cpp:347-378); uJ[p] = w[p] V[p] / 2 (src/atomicgrid.cpp:475-485).
+__global__ void k_vreduce(const double* __restrict__ vpart, const double* __restrict__ vcell,
+                          const double* __restrict__ w, long npts, long npts_pad, long cell,
+                          int natoms, double* V, double* uJ) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts_pad) return;
+  if (p >= npts) {
+    V[p] = 0.0;
+    uJ[p] = 0.0;
+    return;
+  }
+  const int own = (int)(p / cell);
+  double v = 0.0;
+  for (int k = 0; k < natoms; ++k) v += (k == own) ? vcell[p] : vpart[(long)k * npts_pad + p];
+  V[p] = v;
+  uJ[p] = (w[p] * v) * 0.5;
+}
+
+// ---------------------------------------------------------------------------
+// Contractions  C_o = Phi^T diag(u_o) Phi  (src/dft.cpp:424-432, src/atomicgrid.cpp:471-488)
+// ---------------------------------------------------------------------------
+// Block (bi <= bj) of 32x32 outputs per workgroup, split-K over points.  Each
+// wave accumulates 2x2 tiles of 16x16 with the FP64 16x16x4 MFMA: lane
+// (c, g) feeds A = phi[p_g][i0 + c] and B = u[p_g] phi[p_g][j0 + c]; waves
+// interleave 4-point steps; the 4 waves are summed in LDS in a fixed order.
+// part[split][blk][o][32][32].
+template <int NOUT>
+__global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi, int ldphi,
+                                                  const double* __restrict__ u0,
+                                                  const double* __restrict__ u1,
+                                                  const int* __restrict__ blocks, long chunk,
+                                                  long npts_pad, int nblk, double* part) {
+  __shared__ double red[4 * 16 * 64];
+  const int blk = blockIdx.x, split = blockIdx.y;
+  const int bi = blocks[2 * blk], bj = blocks[2 * blk + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long pb = (long)split * chunk;
+  const long pe = min(pb + chunk, npts_pad);  // chunk and npts_pad are multiples of 16
+  const int i0 = 32 * bi + c, j0 = 32 * bj + c;
+  dbl4 acc[NOUT][2][2];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[o][a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (long p = pb + 4 * wave + g; p < pe; p += 16) {
+    const double* row = phi + p * ldphi;
+    const double a0 = row[i0], a1 = row[i0 + 16];
+    const double b0 = row[j0], b1 = row[j0 + 16];
+    const double w0 = u0[p];
+    acc[0][0][0] = mfma16x16x4(a0, w0 * b0, acc[0][0][0]);
+    acc[0][0][1] = mfma16x16x4(a0, w0 * b1, acc[0][0][1]);
+    acc[0][1][0] = mfma16x16x4(a1, w0 * b0, acc[0][1][0]);
+    acc[0][1][1] = mfma16x16x4(a1, w0 * b1, acc[0][1][1]);
+    if (NOUT == 2) {
+      const double w1 = u1[p];
+      acc[NOUT - 1][0][0] = mfma16x16x4(a0, w1 * b0, acc[NOUT - 1][0][0]);
+      acc[NOUT - 1][0][1] = mfma16x16x4(a0, w1 * b1, acc[NOUT - 1][0][1]);
+      acc[NOUT - 1][1][0] = mfma16x16x4(a1, w1 * b0, acc[NOUT - 1][1][0]);
+      acc[NOUT - 1][1][1] = mfma16x16x4(a1, w1 * b1, acc[NOUT - 1][1][1]);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[(wave * 16 + (a * 2 + b) * 4 + v) * 64 + lane] = acc[o][a][b][v];
+    __syncthreads();
+    double* dst = part + (((long)split * nblk + blk) * NOUT + o) * 1024;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      const double s = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
+      const int ln = e & 63, v = (e >> 6) & 3, ab = e >> 8;
+      const int ta = ab >> 1, tb = ab & 1;
+      const int il = 16 * ta + 4 * (ln >> 4) + v, jl = 16 * tb + (ln & 15);
+      dst[il * 32 + jl] = s;
+    }
+  }
+}
+
+// out_o[i + j nbf] = sum over splits of the block holding (i, j) (or (j, i)).
+__global__ void k_contract_reduce(const double* __restrict__ part, int nsplit, int nblk, int nout,
+                                  int nb32, int nbf, double* out0, double* out1) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)nbf * nbf) return;
+  int i = (int)(e % nbf), j = (int)(e / nbf);
+  if (i / 32 > j / 32) {
+    const int t = i;
+    i = j;
+    j = t;
+  }
+  const int bi = i / 32, bj = j / 32;
+  const int blk = bi * nb32 - bi * (bi - 1) / 2 + (bj - bi);
+  const int idx = (i % 32) * 32 + (j % 32);
+  for (int o = 0; o < nout; ++o) {
+    double s = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) s += part[(((long)sp * nblk + blk) * nout + o) * 1024 + idx];
+    double* out = (o == 0) ? out0 : out1;
+    if (out) out[e] = s;
+  }
+}
+
+// Probe of the 16x16x4 FP64 MFMA lane layout assumed above:
+// A[i][k] on lane i + 16k, B[k][j] on lane j + 16k, D[4g+v][c] in register v of lane c + 16g.
+__global__ void k_mfma_probe(double* out) {
+  const int lane = threadIdx.x;
+  const int c = lane & 15, g = lane >> 4;
+  const double a = 1.0 + c + 100.0 * g;          // A[c][g]
+  const double b = 1.0 + 0.5 * c - 3.0 * g;       // B[g][c]
+  dbl4 d = mfma16x16x4(a, b, dbl4{0.0, 0.0, 0.0, 0.0});
+  for (int v = 0; v < 4; ++v) out[v * 64 + lane] = d[v];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(DG_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                  " bytes failed: " + hipGetErrorString(e));
+  }
+  return DG_OK;
+}
+
+long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct ProfEntry {
+  std::string name;
+  hipEvent_t e0, e1;
+};
+struct ProfStat {
+  double ms = 0.0;
+  int n = 0;
+};
+
+struct dg_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int natoms = 0, nbf = 0, nshell = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0, precision = 0;
+  long npts = 0, npts_pad = 0, cell = 0;
+  int ldphi = 0, nt16 = 0, nb32 = 0, nblk = 0, nsplit = 0;
+  long chunk_k = 0, cross_chunk = 0, nlda_blocks = 0;
+  double zsum = 0.0, xfac = 0.0;
+  VwnConst vwn{};
+  std::vector<std::pair<int, int>> lchunks;  // l ranges of the cross kernel
+  size_t cross_lds = 0;
+  // device buffers
+  double *atoms = nullptr, *invR = nullptr, *rad = nullptr, *wrad = nullptr, *leb = nullptr;
+  double *px = nullptr, *py = nullptr, *pz = nullptr, *w = nullptr, *wb = nullptr;
+  double* phi = nullptr;
+  int *sh_atom = nullptr, *sh_L = nullptr, *sh_nprim = nullptr, *sh_off = nullptr, *sh_cgf = nullptr;
+  double *pr_alpha = nullptr, *pr_cn0 = nullptr, *pr_cn1 = nullptr;
+  double *P = nullptr, *pfrag = nullptr, *rho = nullptr, *uxc = nullptr, *exd = nullptr;
+  double *uJ = nullptr, *V = nullptr, *vcell = nullptr, *vpart = nullptr;
+  double *q = nullptr, *exc_part = nullptr, *scalars = nullptr;
+  double *ylm = nullptr, *ylmT = nullptr, *minvT = nullptr, *shc = nullptr;
+  double *knots = nullptr, *invh = nullptr, *sa = nullptr, *sden = nullptr, *scmod = nullptr;
+  double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
+  int* blocks = nullptr;
+  double *cpart = nullptr, *J = nullptr, *XC = nullptr;
+  // state
+  bool has_density = false, has_hartree = false, has_xc = false;
+  // profiling
+  bool prof = false;
+  std::vector<ProfEntry> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, ProfStat> stats;
+
+  ~dg_handle() {
+    double* bufs[] = {atoms, invR,  rad,   wrad,   leb,   px,     py,    pz,   w,     wb,
+                      phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
+                      V,     vcell, vpart, q,      exc_part, scalars, ylm, ylmT, minvT, shc,
+                      knots, invh,  sa,    sden,   scmod, rho_lm, U,     tab,  cpart, J,
+                      XC};
+    for (double* b : bufs)
+      if (b) (void)hipFree(b);
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks};
+    for (int* b : ibufs)
+      if (b) (void)hipFree(b);
+    for (auto& e : pending) {
+      (void)hipEventDestroy(e.e0);
+      (void)hipEventDestroy(e.e1);
+    }
+    for (auto& e : event_pool) (void)hipEventDestroy(e);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+hipEvent_t take_event(dg_handle* h) {
+  hipEvent_t e;
+  if (!h->event_pool.empty()) {
+    e = h->event_pool.back();
+    h->event_pool.pop_back();
+  } else {
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  }
+  return e;
+}
+
+// Runs launch() on the handle's stream, bracketed by events when profiling.
+template <typename F>
+int run(dg_handle* h, const char* name, F launch) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof) {
+    e0 = take_event(h);
+    e1 = take_event(h);
+    if (e0) (void)hipEventRecord(e0, h->stream);
+  }
+  launch();
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(DG_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(err));
+  if (h->prof && e0 && e1) {
+    (void)hipEventRecord(e1, h->stream);
+    h->pending.push_back({name, e0, e1});
+  }
+  return DG_OK;
+}
+
+#define DG_TRY(expr)              \
+  do {                            \
+    int _rc = (expr);             \
+    if (_rc != DG_OK) return _rc; \
+  } while (0)
+
+unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int NT>
+void launch_density_nt(dg_handle* h) {
+  hipLaunchKernelGGL(k_density<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
+                     h->ldphi, h->pfrag, std::min(NT, (h->nbf + 15) / 16), h->rho);
+}
+
+int launch_density(dg_handle* h) {
+  return run(h, "density", [&] {
+    switch (h->nt16) {
+#define DG_NT(N) \
+  case N:        \
+    launch_density_nt<N>(h); \
+    break;
+      DG_NT(2) DG_NT(4) DG_NT(6) DG_NT(8) DG_NT(10) DG_NT(12) DG_NT(14) DG_NT(16) DG_NT(18)
+      DG_NT(20) DG_NT(22) DG_NT(24) DG_NT(26) DG_NT(28) DG_NT(30) DG_NT(32)
+#undef DG_NT
+      default:
+        break;
+    }
+  });
+}
+
+int stage_density(dg_handle* h, const double* dP) {
+  const long nfrag = (long)h->nt16 * 4 * h->nt16 * 64;
+  DG_TRY(run(h, "pfrag", [&] {
+    hipLaunchKernelGGL(k_pfrag, dim3(nblocks(nfrag, 256)), dim3(256), 0, h->stream, dP, h->nbf, h->nt16,
+                       h->pfrag);
+  }));
+  DG_TRY(launch_density(h));
+  h->has_density = true;
+  h->has_hartree = h->has_xc = false;
+  return DG_OK;
+}
+
+int stage_correct(dg_handle* h) {
+  DG_TRY(run(h, "cell_sum", [&] {
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+  }));
+  DG_TRY(run(h, "scale", [&] {
+    hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
+                       h->npts_pad, h->q, h->natoms, h->zsum, h->scalars);
+  }));
+  DG_TRY(run(h, "cell_sum", [&] {
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+  }));
+  return DG_OK;
+}
+
+template <int L>
+void launch_cross_l(dg_handle* h, CrossArgs& a, dim3 grid) {
+  hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
+}
+
+template <int L>
+hipError_t allow_cross_lds(size_t bytes) {
+  return hipFuncSetAttribute((const void*)k_hartree_cross<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+hipError_t allow_cross_lds_l(int lmax, size_t bytes) {
+  switch (lmax) {
+#define DG_L(N) \
+  case N:       \
+    return allow_cross_lds<N>(bytes);
+    DG_L(0) DG_L(1) DG_L(2) DG_L(3) DG_L(4) DG_L(5) DG_L(6) DG_L(7) DG_L(8) DG_L(9) DG_L(10) DG_L(11) DG_L(12)
+#undef DG_L
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+void launch_cross(dg_handle* h, CrossArgs& a, dim3 grid) {
+  switch (h->lmax) {
+#define DG_L(N)                    \
+  case N:                          \
+    launch_cross_l<N>(h, a, grid); \
+    break;
+    DG_L(0) DG_L(1) DG_L(2) DG_L(3) DG_L(4) DG_L(5) DG_L(6) DG_L(7) DG_L(8) DG_L(9) DG_L(10) DG_L(11) DG_L(12)
+#undef DG_L
+    default:
+      break;
+  }
+}
+
+int stage_hartree(dg_handle* h) {
+  DG_TRY(run(h, "rho_lm", [&] {
+    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * h->nrad), dim3(round_up(h->nlm, 64)), 0, h->stream,
+                       h->rho, h->wb, h->leb, h->ylm, h->nang, h->nlm, h->rho_lm);
+  }));
+  DG_TRY(run(h, "poisson", [&] {
+    hipLaunchKernelGGL(k_poisson, dim3(h->natoms * h->nlm), dim3(round_up(h->nrad + 2, 64)),
+                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->q, h->rad, h->minvT,
+                       h->nrad, h->nlm, h->U);
+  }));
+  DG_TRY(run(h, "spline", [&] {
+    hipLaunchKernelGGL(k_spline, dim3(nblocks((long)h->natoms * h->nlm, 64)), dim3(64), 0, h->stream,
+                       h->U, h->knots, h->sa, h->sden, h->scmod, h->nrad, h->nlm, h->natoms, h->tab);
+  }));
+  DG_TRY(run(h, "vcell", [&] {
+    hipLaunchKernelGGL(k_vcell, dim3(nblocks(h->npts, 256)), dim3(256), 0, h->stream, h->U, h->ylmT,
+                       h->atoms, h->px, h->py, h->pz, h->npts, h->nrad, h->nang, h->nlm, h->vcell);
+  }));
+  if (h->natoms > 1) {
+    CrossArgs a;
+    a.px = h->px;
+    a.py = h->py;
+    a.pz = h->pz;
+    a.atoms = h->atoms;
+    a.knots = h->knots;
+    a.invh = h->invh;
+    a.tab = h->tab;
+    a.shc = h->shc;
+    a.vpart = h->vpart;
+    a.npts = h->npts;
+    a.npts_pad = h->npts_pad;
+    a.cell = h->cell;
+    a.chunk = h->cross_chunk;
+    a.nrad = h->nrad;
+    a.nlm = h->nlm;
+    dim3 grid(nblocks(h->npts, (int)h->cross_chunk), h->natoms);
+    for (size_t ci = 0; ci < h->lchunks.size(); ++ci) {
+      a.l0 = h->lchunks[ci].first;
+      a.l1 = h->lchunks[ci].second;
+      a.accumulate = ci > 0;
+      DG_TRY(run(h, "hartree_cross", [&] { launch_cross(h, a, grid); }));
+    }
+  }
+  DG_TRY(run(h, "vreduce", [&] {
+    hipLaunchKernelGGL(k_vreduce, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->vpart,
+                       h->vcell, h->w, h->npts, h->npts_pad, h->cell, h->natoms, h->V, h->uJ);
+  }));
+  return DG_OK;
+}
+
+int stage_lda(dg_handle* h) {
+  return run(h, "lda", [&] {
+    hipLaunchKernelGGL(k_lda, dim3((unsigned)h->nlda_blocks), dim3(256), 0, h->stream, h->rho, h->w,
+                       h->npts_pad, h->vwn, h->xfac, h->uxc, h->exd, h->exc_part);
+  });
+}
+
+int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
+  const int nout = u1 ? 2 : 1;
+  DG_TRY(run(h, "contract", [&] {
+    dim3 grid(h->nblk, h->nsplit);
+    if (nout == 2)
+      hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
+                         h->chunk_k, h->npts_pad, h->nblk, h->cpart);
+    else
+      hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
+                         h->chunk_k, h->npts_pad, h->nblk, h->cpart);
+  }));
+  return run(h, "contract_reduce", [&] {
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->nbf * h->nbf, 256)), dim3(256), 0, h->stream,
+                       h->cpart, h->nsplit, h->nblk, nout, h->nb32, h->nbf, out0, out1);
+  });
+}
+
+int stage_scalars(dg_handle* h) {
+  return run(h, "scalars", [&] {
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks, h->q,
+                       h->natoms, h->scalars);
+  });
+}
+
+int sync(dg_handle* h) {
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+int copy_out(dg_handle* h, double* dst, const double* src, size_t n) {
+  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* dg_last_error(void) { return g_last_error.c_str(); }
+int dg_version(void) { return 100; }
+int dg_lebedev_available(int npoints) { return dg::lebedev_rule(npoints) != nullptr; }
+
+int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* prm, dg_handle** out) {
+  g_last_error.clear();
+  if (!mol || !basis || !prm || !out) return fail(DG_ERR_INVALID, "dg_create: null argument");
+  *out = nullptr;
+  if (mol->natoms < 1 || !mol->Z || !mol->xyz) return fail(DG_ERR_INVALID, "dg_create: empty molecule");
+  if (basis->nshell < 1 || !basis->atom || !basis->L || !basis->nprim || !basis->offset || !basis->alpha ||
+      !basis->coef)
+    return fail(DG_ERR_INVALID, "dg_create: empty basis");
+  if (prm->n_radial < 5) return fail(DG_ERR_INVALID, "dg_create: n_radial must be >= 5");
+  if (prm->lmax < 0 || prm->lmax > kMaxL)
+    return fail(DG_ERR_INVALID, "dg_create: lmax must be in 0.." + std::to_string(kMaxL));
+  const double* lebt = dg::lebedev_rule(prm->n_angular);
+  if (!lebt) return fail(DG_ERR_INVALID, "dg_create: no Lebedev rule with " + std::to_string(prm->n_angular) + " points");
+  if (prm->precision != DG_PREC_FP64 && prm->precision != DG_PREC_MIXED)
+    return fail(DG_ERR_INVALID, "dg_create: unknown precision mode");
+  for (int a = 0; a < mol->natoms; ++a)
+    if (mol->Z[a] < 1 || mol->Z[a] > 18) return fail(DG_ERR_INVALID, "dg_create: Z outside 1..18");
+  for (int i = 0; i < mol->natoms; ++i)
+    for (int j = i + 1; j < mol->natoms; ++j) {
+      const double dx = mol->xyz[3 * i] - mol->xyz[3 * j], dy = mol->xyz[3 * i + 1] - mol->xyz[3 * j + 1],
+                   dz = mol->xyz[3 * i + 2] - mol->xyz[3 * j + 2];
+      if (dx * dx + dy * dy + dz * dz == 0.0) return fail(DG_ERR_INVALID, "dg_create: two atoms coincide");
+    }
+  if (mol->natoms > 1024) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 1024 atoms");
+
+  auto h = new dg_handle();
+  std::unique_ptr<dg_handle> guard(h);
+  HIP_TRY(hipGetDevice(&h->device));
+  HIP_TRY(hipStreamCreate(&h->stream));
+  h->natoms = mol->natoms;
+  h->nrad = prm->n_radial;
+  h->nang = prm->n_angular;
+  h->lmax = prm->lmax;
+  h->nlm = (prm->lmax + 1) * (prm->lmax + 1);
+  h->precision = prm->precision;
+  h->cell = (long)h->nrad * h->nang;
+  h->npts = h->cell * h->natoms;
+  h->npts_pad = round_up(h->npts, 256);
+
+  // shells -> CGF offsets, primitive tables
+  h->nshell = basis->nshell;
+  std::vector<int> cgf_off(h->nshell);
+  int nbf = 0, nprim_tot = 0;
+  for (int s = 0; s < h->nshell; ++s) {
+    const int L = basis->L[s];
+    if (L < 0 || L > 2) return fail(DG_ERR_INVALID, "dg_create: shell L must be 0, 1 or 2");
+    if (basis->atom[s] < 0 || basis->atom[s] >= mol->natoms) return fail(DG_ERR_INVALID, "dg_create: shell atom out of range");
+    if (basis->nprim[s] < 1 || basis->offset[s] < 0) return fail(DG_ERR_INVALID, "dg_create: bad primitive range");
+    cgf_off[s] = nbf;
+    nbf += (L + 1) * (L + 2) / 2;
+    nprim_tot = std::max(nprim_tot, basis->offset[s] + basis->nprim[s]);
+  }
+  h->nbf = nbf;
+  h->ldphi = (int)round_up(nbf, 32);
+  h->nt16 = h->ldphi / 16;
+  h->nb32 = h->ldphi / 32;
+  if (h->nt16 > 32) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 512 basis functions");
+  std::vector<double> alpha(nprim_tot), cn0(nprim_tot), cn1(nprim_tot);
+  for (int s = 0; s < h->nshell; ++s) {
+    const int L = basis->L[s];
+    for (int k = 0; k < basis->nprim[s]; ++k) {
+      const int i = basis->offset[s] + k;
+      const double a = basis->alpha[i], c = basis->coef[i];
+      if (!(a > 0.0)) return fail(DG_ERR_INVALID, "dg_create: primitive exponent must be positive");
+      alpha[i] = a;
+      cn0[i] = c * dg::gto_norm(a, L, 0, 0);
+      cn1[i] = c * dg::gto_norm(a, L == 2 ? 1 : L, L == 2 ? 1 : 0, 0);
+    }
+  }
+  std::vector<double> atoms(4 * h->natoms), invR((size_t)h->natoms * h->natoms, 0.0);
+  for (int a = 0; a < h->natoms; ++a) {
+    for (int d = 0; d < 3; ++d) atoms[4 * a + d] = mol->xyz[3 * a + d];
+    atoms[4 * a + 3] = mol->Z[a];
+    h->zsum += mol->Z[a];
+  }
+  for (int n = 0; n < h->natoms; ++n)
+    for (int m = 0; m < h->natoms; ++m)
+      if (n != m) {
+        const double dx = atoms[4 * m] - atoms[4 * n], dy = atoms[4 * m + 1] - atoms[4 * n + 1],
+                     dz = atoms[4 * m + 2] - atoms[4 * n + 2];
+        invR[(size_t)n * h->natoms + m] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+
+  // radial grid, angular tables
+  std::vector<double> rad, wrad;
+  dg::radial_grid(h->nrad, rad, wrad);
+  std::vector<double> ylm, ylmT;
+  dg::ylm_table(h->nang, lebt, h->lmax, ylm);
+  ylmT.resize(ylm.size());
+  for (int j = 0; j < h->nang; ++j)
+    for (int lm = 0; lm < h->nlm; ++lm) ylmT[(size_t)lm * h->nang + j] = ylm[(size_t)j * h->nlm + lm];
+  // Poisson inverses, rows 1..N, stored [l][k][i]
+  const int n2 = h->nrad + 2;
+  std::vector<double> minvT((size_t)(h->lmax + 1) * n2 * h->nrad);
+  for (int l = 0; l <= h->lmax; ++l) {
+    std::vector<double> M;
+    dg::poisson_matrix(h->nrad, rad.data(), l, M);
+    if (!dg::invert_matrix(n2, M)) return fail(DG_ERR_INVALID, "dg_create: singular Poisson matrix");
+    for (int k = 0; k < n2; ++k)
+      for (int i = 0; i < h->nrad; ++i) minvT[((size_t)l * n2 + k) * h->nrad + i] = M[(size_t)(i + 1) * n2 + k];
+  }
+  // spline system over ascending knots
+  std::vector<double> knots(h->nrad), invh(h->nrad, 0.0);
+  for (int t = 0; t < h->nrad; ++t) knots[t] = rad[h->nrad - 1 - t];
+  for (int t = 1; t < h->nrad; ++t) invh[t] = 1.0 / (knots[t] - knots[t - 1]);
+  dg::SplineSystem sys = dg::spline_system(knots);
+  // spherical-harmonic recurrence constants per (l, m >= 0): {N_lm, (2l-1)/(l-m), (l+m-1)/(l-m)}
+  std::vector<double> shc(3 * (kMaxL + 1) * (kMaxL + 2) / 2, 0.0);
+  for (int l = 0; l <= kMaxL; ++l)
+    for (int m = 0; m <= l; ++m) {
+      const int ci = 3 * (l * (l + 1) / 2 + m);
+      shc[ci] = dg::sh_prefactor(l, m);
+      if (l >= m + 2) {
+        shc[ci + 1] = (double)(2 * l - 1) / (double)(l - m);
+        shc[ci + 2] = (double)(l + m - 1) / (double)(l - m);
+      }
+    }
+  // LDA constants
+  h->xfac = -2.25 * (2.0 / 3.0) * std::pow(3.0 / 4.0 / dg::kPi, 1.0 / 3.0);
+  {
+    const double A = 0.0310907, x0 = -0.10498, b = 3.72744, c = 12.9352;
+    const double Q = std::sqrt(4.0 * c - b * b), X0 = x0 * x0 + b * x0 + c;
+    h->vwn = VwnConst{A, x0, b, c, Q, X0, (2.0 * b / Q) * (1.0 - x0 * (2.0 * x0 + b) / X0), b * x0 / X0};
+  }
+  // contraction blocks and split-K
+  std::vector<int> blocks;
+  for (int bi = 0; bi < h->nb32; ++bi)
+    for (int bj = bi; bj < h->nb32; ++bj) {
+      blocks.push_back(bi);
+      blocks.push_back(bj);
+    }
+  h->nblk = (int)blocks.size() / 2;
+  {
+    const long target_wg = 2048;
+    long ns = std::max(1L, target_wg / h->nblk);
+    ns = std::min(ns, h->npts_pad / 64);
+    h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
+    h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
+  }
+  h->nlda_blocks = h->npts_pad / 256;
+  // cross kernel: l ranges whose spline table fits in LDS
+  {
+    const size_t limit = 150 * 1024;
+    auto bytes = [&](int l0, int l1) {
+      return (size_t)(2 * h->nrad + (size_t)h->nrad * (l1 * l1 - l0 * l0) * 2) * sizeof(double);
+    };
+    int l0 = 0;
+    size_t maxb = 0;
+    while (l0 <= h->lmax) {
+      int l1 = l0 + 1;
+      if (bytes(l0, l1) > limit)
+        return fail(DG_ERR_UNSUPPORTED, "dg_create: radial grid too large for the Hartree kernel");
+      while (l1 <= h->lmax && bytes(l0, l1 + 1) <= limit) ++l1;
+      h->lchunks.push_back({l0, l1});
+      maxb = std::max(maxb, bytes(l0, l1));
+      l0 = l1;
+    }
+    h->cross_lds = maxb;
+    // about two work-groups per CU over all (chunk, atom) pairs, >= 4 points per thread
+    const long ch = round_up(std::max(4096L, h->npts * h->natoms / 512), 1024);
+    h->cross_chunk = std::min(ch, round_up(h->npts, 1024));
+  }
+
+  // allocations
+  const long np = h->npts_pad;
+  DG_TRY(dalloc(&h->atoms, atoms.size()));
+  DG_TRY(dalloc(&h->invR, invR.size()));
+  DG_TRY(dalloc(&h->rad, h->nrad));
+  DG_TRY(dalloc(&h->wrad, h->nrad));
+  DG_TRY(dalloc(&h->leb, 4 * (size_t)h->nang));
+  for (double** b : {&h->px, &h->py, &h->pz, &h->w, &h->wb, &h->rho, &h->uxc, &h->exd, &h->uJ, &h->V, &h->vcell})
+    DG_TRY(dalloc(b, np));
+  DG_TRY(dalloc(&h->phi, (size_t)np * h->ldphi));
+  DG_TRY(dalloc(&h->sh_atom, h->nshell));
+  DG_TRY(dalloc(&h->sh_L, h->nshell));
+  DG_TRY(dalloc(&h->sh_nprim, h->nshell));
+  DG_TRY(dalloc(&h->sh_off, h->nshell));
+  DG_TRY(dalloc(&h->sh_cgf, h->nshell));
+  DG_TRY(dalloc(&h->pr_alpha, nprim_tot));
+  DG_TRY(dalloc(&h->pr_cn0, nprim_tot));
+  DG_TRY(dalloc(&h->pr_cn1, nprim_tot));
+  DG_TRY(dalloc(&h->P, (size_t)h->nbf * h->nbf));
+  DG_TRY(dalloc(&h->pfrag, (size_t)h->nt16 * 4 * h->nt16 * 64));
+  DG_TRY(dalloc(&h->vpart, (size_t)h->natoms * np));
+  DG_TRY(dalloc(&h->q, h->natoms));
+  DG_TRY(dalloc(&h->exc_part, h->nlda_blocks));
+  DG_TRY(dalloc(&h->scalars, 4));
+  DG_TRY(dalloc(&h->ylm, ylm.size()));
+  DG_TRY(dalloc(&h->ylmT, ylmT.size()));
+  DG_TRY(dalloc(&h->minvT, minvT.size()));
+  DG_TRY(dalloc(&h->shc, shc.size()));
+  DG_TRY(dalloc(&h->knots, h->nrad));
+  DG_TRY(dalloc(&h->invh, h->nrad));
+  DG_TRY(dalloc(&h->sa, h->nrad));
+  DG_TRY(dalloc(&h->sden, h->nrad));
+  DG_TRY(dalloc(&h->scmod, h->nrad));
+  DG_TRY(dalloc(&h->rho_lm, (size_t)h->natoms * h->nrad * h->nlm));
+  DG_TRY(dalloc(&h->U, (size_t)h->natoms * h->nrad * h->nlm));
+  DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
+  DG_TRY(dalloc(&h->blocks, blocks.size()));
+  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->nblk * 2 * 1024));
+  DG_TRY(dalloc(&h->J, (size_t)h->nbf * h->nbf));
+  DG_TRY(dalloc(&h->XC, (size_t)h->nbf * h->nbf));
+
+  std::vector<int> shv(h->nshell);
+  auto up = [&](void* d, const void* s, size_t bytes) {
+    return hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, h->stream);
+  };
+  HIP_TRY(up(h->atoms, atoms.data(), atoms.size() * 8));
+  HIP_TRY(up(h->invR, invR.data(), invR.size() * 8));
+  HIP_TRY(up(h->rad, rad.data(), rad.size() * 8));
+  HIP_TRY(up(h->wrad, wrad.data(), wrad.size() * 8));
+  HIP_TRY(up(h->leb, lebt, 4 * (size_t)h->nang * 8));
+  HIP_TRY(up(h->sh_atom, basis->atom, h->nshell * 4));
+  HIP_TRY(up(h->sh_L, basis->L, h->nshell * 4));
+  HIP_TRY(up(h->sh_nprim, basis->nprim, h->nshell * 4));
+  HIP_TRY(up(h->sh_off, basis->offset, h->nshell * 4));
+  HIP_TRY(up(h->sh_cgf, cgf_off.data(), h->nshell * 4));
+  HIP_TRY(up(h->pr_alpha, alpha.data(), nprim_tot * 8));
+  HIP_TRY(up(h->pr_cn0, cn0.data(), nprim_tot * 8));
+  HIP_TRY(up(h->pr_cn1, cn1.data(), nprim_tot * 8));
+  HIP_TRY(up(h->ylm, ylm.data(), ylm.size() * 8));
+  HIP_TRY(up(h->ylmT, ylmT.data(), ylmT.size() * 8));
+  HIP_TRY(up(h->minvT, minvT.data(), minvT.size() * 8));
+  HIP_TRY(up(h->shc, shc.data(), shc.size() * 8));
+  HIP_TRY(up(h->knots, knots.data(), knots.size() * 8));
+  HIP_TRY(up(h->invh, invh.data(), invh.size() * 8));
+  HIP_TRY(up(h->sa, sys.a.data(), sys.a.size() * 8));
+  HIP_TRY(up(h->sden, sys.den.data(), sys.den.size() * 8));
+  HIP_TRY(up(h->scmod, sys.cmod.data(), sys.cmod.size() * 8));
+  HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
+  HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)h->natoms * np * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->rho, 0, (size_t)np * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->vcell, 0, (size_t)np * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->scalars, 0, 4 * 8, h->stream));
+
+  HIP_TRY(allow_cross_lds_l(h->lmax, h->cross_lds));
+  if ((size_t)h->natoms * 64 * 8 > 160 * 1024)
+    return fail(DG_ERR_UNSUPPORTED, "dg_create: too many atoms for the Becke kernel's LDS");
+  HIP_TRY(hipFuncSetAttribute((const void*)k_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)std::max<size_t>((size_t)h->natoms * 64 * 8, 1024)));
+  DG_TRY(run(h, "grid", [&] {
+    hipLaunchKernelGGL(k_grid, dim3(nblocks(np, 256)), dim3(256), 0, h->stream, h->atoms, h->rad, h->wrad,
+                       h->leb, h->npts, np, h->nrad, h->nang, h->px, h->py, h->pz, h->w);
+  }));
+  DG_TRY(run(h, "becke", [&] {
+    hipLaunchKernelGGL(k_becke, dim3(nblocks(np, 64)), dim3(64), (size_t)h->natoms * 64 * 8, h->stream,
+                       h->atoms, h->invR, h->natoms, h->npts, h->cell, h->px, h->py, h->pz, h->w, h->wb);
+  }));
+  DG_TRY(run(h, "phi", [&] {
+    hipLaunchKernelGGL(k_phi, dim3(nblocks(h->npts * h->nshell, 256)), dim3(256), 0, h->stream, h->atoms,
+                       h->nshell, h->sh_atom, h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha,
+                       h->pr_cn0, h->pr_cn1, h->npts, h->px, h->py, h->pz, h->ldphi, h->phi);
+  }));
+  DG_TRY(sync(h));
+  guard.release();
+  *out = h;
+  return DG_OK;
+}
+
+int dg_destroy(dg_handle* h) {
+  if (!h) return DG_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+  return DG_OK;
+}
+
+int dg_info(const dg_handle* h, int* nbf, long* npts, int* natoms) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_info: null handle");
+  if (nbf) *nbf = h->nbf;
+  if (npts) *npts = h->npts;
+  if (natoms) *natoms = h->natoms;
+  return DG_OK;
+}
+
+int dg_set_stream(dg_handle* h, void* stream) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_set_stream: null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (stream) {
+    if (h->own_stream) HIP_TRY(hipStreamDestroy(h->stream));
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+  } else if (!h->own_stream) {
+    HIP_TRY(hipStreamCreate(&h->stream));
+    h->own_stream = true;
+  }
+  return DG_OK;
+}
+
+int dg_get_stream(const dg_handle* h, void** stream) {
+  if (!h || !stream) return fail(DG_ERR_INVALID, "dg_get_stream: null argument");
+  *stream = (void*)h->stream;
+  return DG_OK;
+}
+
+int dg_set_density(dg_handle* h, const double* P, int nbf) {
+  if (!h || !P) return fail(DG_ERR_INVALID, "dg_set_density: null argument");
+  if (nbf != h->nbf) return fail(DG_ERR_INVALID, "dg_set_density: nbf mismatch");
+  for (long i = 0; i < (long)nbf * nbf; ++i)
+    if (!std::isfinite(P[i])) return fail(DG_ERR_INVALID, "dg_set_density: P contains a non-finite value");
+  HIP_TRY(hipMemcpyAsync(h->P, P, (size_t)nbf * nbf * 8, hipMemcpyHostToDevice, h->stream));
+  DG_TRY(stage_density(h, h->P));
+  return sync(h);
+}
+
+int dg_correct_densities(dg_handle* h, double* nelec_before) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_correct_densities: null handle");
+  if (!h->has_density) return fail(DG_ERR_STATE, "dg_correct_densities: no density set");
+  DG_TRY(stage_correct(h));
+  h->has_hartree = h->has_xc = false;
+  if (nelec_before) {
+    double s[4];
+    HIP_TRY(hipMemcpyAsync(s, h->scalars, 4 * 8, hipMemcpyDeviceToHost, h->stream));
+    DG_TRY(sync(h));
+    *nelec_before = s[1];
+  }
+  return sync(h);
+}
+
+int dg_calculate_density(dg_handle* h, double* nelec) {
+  if (!h || !nelec) return fail(DG_ERR_INVALID, "dg_calculate_density: null argument");
+  if (!h->has_density) return fail(DG_ERR_STATE, "dg_calculate_density: no density set");
+  DG_TRY(run(h, "cell_sum", [&] {
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+  }));
+  std::vector<double> q(h->natoms);
+  DG_TRY(copy_out(h, q.data(), h->q, h->natoms));
+  double s = 0.0;
+  for (double v : q) s += v;
+  *nelec = s;
+  return DG_OK;
+}
+
+int dg_hartree(dg_handle* h, double* J) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_hartree: null handle");
+  if (!h->has_density) return fail(DG_ERR_STATE, "dg_hartree: no density set");
+  DG_TRY(run(h, "cell_sum", [&] {
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+  }));
+  DG_TRY(stage_hartree(h));
+  DG_TRY(stage_contract(h, h->uJ, nullptr, h->J, nullptr));
+  h->has_hartree = true;
+  if (J) return copy_out(h, J, h->J, (size_t)h->nbf * h->nbf);
+  return sync(h);
+}
+
+int dg_xc(dg_handle* h, double* XC, double* exc) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_xc: null handle");
+  if (!h->has_density) return fail(DG_ERR_STATE, "dg_xc: no density set");
+  DG_TRY(stage_lda(h));
+  DG_TRY(stage_contract(h, h->uxc, nullptr, h->XC, nullptr));
+  DG_TRY(stage_scalars(h));
+  h->has_xc = true;
+  if (XC) DG_TRY(copy_out(h, XC, h->XC, (size_t)h->nbf * h->nbf));
+  if (exc) {
+    double s[4];
+    DG_TRY(copy_out(h, s, h->scalars, 4));
+    *exc = s[0];
+  }
+  return sync(h);
+}
+
+int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, double* dscalars) {
+  if (!h || !dP) return fail(DG_ERR_INVALID, "dg_build_device: null argument");
+  DG_TRY(stage_density(h, dP));
+  DG_TRY(stage_correct(h));
+  DG_TRY(stage_lda(h));
+  DG_TRY(stage_hartree(h));
+  DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J));
+  DG_TRY(stage_scalars(h));
+  if (dscalars)
+    HIP_TRY(hipMemcpyAsync(dscalars, h->scalars, 4 * 8, hipMemcpyDeviceToDevice, h->stream));
+  h->has_hartree = h->has_xc = true;
+  return DG_OK;
+}
+
+int dg_get_weights(dg_handle* h, double* w) {
+  if (!h || !w) return fail(DG_ERR_INVALID, "dg_get_weights: null argument");
+  return copy_out(h, w, h->w, h->npts);
+}
+int dg_get_becke_weights(dg_handle* h, double* wb) {
+  if (!h || !wb) return fail(DG_ERR_INVALID, "dg_get_becke_weights: null argument");
+  return copy_out(h, wb, h->wb, h->npts);
+}
+int dg_get_positions(dg_handle* h, double* xyz) {
+  if (!h || !xyz) return fail(DG_ERR_INVALID, "dg_get_positions: null argument");
+  std::vector<double> x(h->npts), y(h->npts), z(h->npts);
+  DG_TRY(copy_out(h, x.data(), h->px, h->npts));
+  DG_TRY(copy_out(h, y.data(), h->py, h->npts));
+  DG_TRY(copy_out(h, z.data(), h->pz, h->npts));
+  for (long i = 0; i < h->npts; ++i) {
+    xyz[3 * i] = x[i];
+    xyz[3 * i + 1] = y[i];
+    xyz[3 * i + 2] = z[i];
+  }
+  return DG_OK;
+}
+int dg_get_densities(dg_handle* h, double* rho) {
+  if (!h || !rho) return fail(DG_ERR_INVALID, "dg_get_densities: null argument");
+  if (!h->has_density) return fail(DG_ERR_STATE, "dg_get_densities: no density set");
+  return copy_out(h, rho, h->rho, h->npts);
+}
+int dg_get_amplitudes(dg_handle* h, double* phi) {
+  if (!h || !phi) return fail(DG_ERR_INVALID, "dg_get_amplitudes: null argument");
+  HIP_TRY(hipMemcpy2DAsync(phi, (size_t)h->nbf * 8, h->phi, (size_t)h->ldphi * 8, (size_t)h->nbf * 8,
+                           h->npts, hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+int dg_get_hartree_potential(dg_handle* h, double* V) {
+  if (!h || !V) return fail(DG_ERR_INVALID, "dg_get_hartree_potential: null argument");
+  if (!h->has_hartree) return fail(DG_ERR_STATE, "dg_get_hartree_potential: no Hartree potential computed");
+  return copy_out(h, V, h->V, h->npts);
+}
+int dg_get_hartree_parts(dg_handle* h, double* rho_lm, double* U, double* vcell) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_get_hartree_parts: null handle");
+  if (!h->has_hartree) return fail(DG_ERR_STATE, "dg_get_hartree_parts: no Hartree potential computed");
+  const size_t n = (size_t)h->natoms * h->nrad * h->nlm;
+  if (rho_lm) DG_TRY(copy_out(h, rho_lm, h->rho_lm, n));
+  if (U) DG_TRY(copy_out(h, U, h->U, n));
+  if (vcell) DG_TRY(copy_out(h, vcell, h->vcell, h->npts));
+  return DG_OK;
+}
+int dg_get_xc_parts(dg_handle* h, double* exd, double* wv) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_get_xc_parts: null handle");
+  if (!h->has_xc) return fail(DG_ERR_STATE, "dg_get_xc_parts: no XC computed");
+  if (exd) DG_TRY(copy_out(h, exd, h->exd, h->npts));
+  if (wv) DG_TRY(copy_out(h, wv, h->uxc, h->npts));
+  return DG_OK;
+}
+
+int dg_profile_enable(dg_handle* h, int enable) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_profile_enable: null handle");
+  h->prof = enable != 0;
+  return DG_OK;
+}
+
+static int collect(dg_handle* h) {
+  if (h->pending.empty()) return DG_OK;
+  DG_TRY(sync(h));
+  for (auto& e : h->pending) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e.e0, e.e1));
+    ProfStat& s = h->stats[e.name];
+    s.ms += ms;
+    s.n += 1;
+    h->event_pool.push_back(e.e0);
+    h->event_pool.push_back(e.e1);
+  }
+  h->pending.clear();
+  return DG_OK;
+}
+
+int dg_profile_read(dg_handle* h, int max, char* names, double* total_ms, int* launches, int* n) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_profile_read: null handle");
+  DG_TRY(collect(h));
+  int i = 0;
+  for (auto& kv : h->stats) {
+    if (i < max) {
+      if (names) {
+        std::strncpy(names + 48 * i, kv.first.c_str(), 47);
+        names[48 * i + 47] = 0;
+      }
+      if (total_ms) total_ms[i] = kv.second.ms;
+      if (launches) launches[i] = kv.second.n;
+    }
+    ++i;
+  }
+  if (n) *n = i;
+  return DG_OK;
+}
+
+int dg_profile_reset(dg_handle* h) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_profile_reset: null handle");
+  DG_TRY(collect(h));
+  h->stats.clear();
+  return DG_OK;
+}
+
+int dg_selftest_mfma(int* ok) {
+  if (!ok) return fail(DG_ERR_INVALID, "dg_selftest_mfma: null argument");
+  *ok = 0;
+  double* d = nullptr;
+  DG_TRY(dalloc(&d, 256));
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, d);
+  std::vector<double> out(256);
+  hipError_t e = hipMemcpy(out.data(), d, 256 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_selftest_mfma: ") + hipGetErrorString(e));
+  int good = 1;
+  for (int lane = 0; lane < 64; ++lane)
+    for (int v = 0; v < 4; ++v) {
+      const int i = 4 * (lane >> 4) + v, j = lane & 15;
+      double ref = 0.0;
+      for (int k = 0; k < 4; ++k) ref += (1.0 + i + 100.0 * k) * (1.0 + 0.5 * j - 3.0 * k);
+      if (std::fabs(out[v * 64 + lane] - ref) > 1e-9 * std::fabs(ref) + 1e-12) good = 0;
+    }
+  *ok = good;
+  return DG_OK;
+}
+
+}  // extern "C"

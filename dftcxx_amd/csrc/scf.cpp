@@ -1,0 +1,539 @@
+// Host SCF driver around the GPU grid library: one-electron integrals, the
+// symmetric eigen-solver and the SCF loop of the reference's DFT class
+// (src/dft.cpp:84-142, 160-227, 235-246, 300-366, 441-447), in FP64 on the host.
+// The grid work of every iteration runs on the GPU through dg_build_device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dftcxx/molecular_grid.hpp"
+#include "../../include/dftgrid.h"
+#include "grid_tables.h"
+
+namespace dftcxx {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Boys function F_n(T) = int_0^1 t^{2n} exp(-T t^2) dt.  The argument is
+// clamped to >= 1e-8 as the reference does (src/gamma.cpp:41-46).  Series
+// e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) for T < 40 with downward
+// recursion, F_0 = sqrt(pi/T)/2 and upward recursion above.
+// ---------------------------------------------------------------------------
+void boys(int nmax, double T, double* F) {
+  T = std::max(std::fabs(T), 1e-8);
+  const double et = std::exp(-T);
+  if (T < 40.0) {
+    double term = 1.0 / (2 * nmax + 1), sum = term;
+    for (int k = 1; k < 400; ++k) {
+      term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+      sum += term;
+      if (term < 1e-17 * sum) break;
+    }
+    F[nmax] = et * sum;
+    for (int n = nmax; n > 0; --n) F[n - 1] = (2.0 * T * F[n] + et) / (2 * n - 1);
+  } else {
+    F[0] = 0.5 * std::sqrt(dg::kPi / T);
+    for (int n = 0; n < nmax; ++n) F[n + 1] = ((2 * n + 1) * F[n] - et) / (2.0 * T);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// McMurchie-Davidson Hermite coefficients E^{ij}_t for one Cartesian direction
+// ---------------------------------------------------------------------------
+constexpr int kMaxE = 5;  // i, j up to 2 (+2 for the kinetic energy)
+struct Herm {
+  double e[kMaxE][kMaxE][2 * kMaxE];  // [i][j][t]
+};
+void hermite(int imax, int jmax, double a, double b, double xa, double xb, Herm& H) {
+  const double p = a + b, mu = a * b / p, xab = xa - xb;
+  const double xp = (a * xa + b * xb) / p, xpa = xp - xa, xpb = xp - xb, o2p = 0.5 / p;
+  std::memset(&H, 0, sizeof(Herm));
+  H.e[0][0][0] = std::exp(-mu * xab * xab);
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j) {
+      if (i == 0 && j == 0) continue;
+      for (int t = 0; t <= i + j; ++t) {
+        double v;
+        if (i > 0) {
+          const auto& s = H.e[i - 1][j];
+          v = xpa * s[t] + (t + 1 < 2 * kMaxE ? (t + 1) * s[t + 1] : 0.0) + (t > 0 ? o2p * s[t - 1] : 0.0);
+        } else {
+          const auto& s = H.e[i][j - 1];
+          v = xpb * s[t] + (t + 1 < 2 * kMaxE ? (t + 1) * s[t + 1] : 0.0) + (t > 0 ? o2p * s[t - 1] : 0.0);
+        }
+        H.e[i][j][t] = v;
+      }
+    }
+}
+
+const int kCart[3][6][3] = {{{0, 0, 0}},
+                            {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+                            {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+
+// S, T, V (nuclear attraction of all nuclei) over contracted Cartesian
+// Gaussians with primitive norms of src/cgf.cpp:102-114.  The reference's
+// nuclear-attraction prefactor uses pi = 3.14159265359 (src/integrals.cpp:342,
+// 386), reproduced by the factor kPiTruncated / kPi.
+void one_electron(const MoleculeData& m, std::vector<double>& S, std::vector<double>& T,
+                  std::vector<double>& V) {
+  const int n = m.nbf(), ns = (int)m.shell_L.size();
+  S.assign((size_t)n * n, 0.0);
+  T.assign((size_t)n * n, 0.0);
+  V.assign((size_t)n * n, 0.0);
+  std::vector<int> off(ns);
+  for (int s = 0, o = 0; s < ns; ++s) {
+    off[s] = o;
+    o += (m.shell_L[s] + 1) * (m.shell_L[s] + 2) / 2;
+  }
+  const double vfac = dg::kPiTruncated / dg::kPi;
+  for (int sa = 0; sa < ns; ++sa)
+    for (int sb = sa; sb < ns; ++sb) {
+      const int La = m.shell_L[sa], Lb = m.shell_L[sb];
+      const int na = (La + 1) * (La + 2) / 2, nb = (Lb + 1) * (Lb + 2) / 2;
+      const double* A = &m.xyz[3 * m.shell_atom[sa]];
+      const double* B = &m.xyz[3 * m.shell_atom[sb]];
+      std::vector<double> s_blk(na * nb, 0.0), t_blk(na * nb, 0.0), v_blk(na * nb, 0.0);
+      for (int ka = 0; ka < m.shell_nprim[sa]; ++ka)
+        for (int kb = 0; kb < m.shell_nprim[sb]; ++kb) {
+          const double a = m.alpha[m.shell_offset[sa] + ka], b = m.alpha[m.shell_offset[sb] + kb];
+          const double ca = m.coef[m.shell_offset[sa] + ka], cb = m.coef[m.shell_offset[sb] + kb];
+          const double p = a + b;
+          Herm H[3];
+          for (int d = 0; d < 3; ++d) hermite(La, Lb + 2, a, b, A[d], B[d], H[d]);
+          const double sq = std::sqrt(dg::kPi / p);
+          double P[3];
+          for (int d = 0; d < 3; ++d) P[d] = (a * A[d] + b * B[d]) / p;
+          // R_tuv for every nucleus, t+u+v <= La+Lb
+          const int Lt = La + Lb;
+          std::vector<double> Rsum((Lt + 1) * (Lt + 1) * (Lt + 1), 0.0);
+          for (int c = 0; c < m.natoms(); ++c) {
+            const double* C = &m.xyz[3 * c];
+            const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
+            double F[16];
+            boys(Lt, p * (X * X + Y * Y + Z * Z), F);
+            // R^n_{tuv}, built by the MD recursion from R^n_{000} = (-2p)^n F_n
+            const int D = Lt + 1;
+            std::vector<double> R((size_t)(Lt + 1) * D * D * D, 0.0);
+            auto r = [&](int nn, int t, int u, int v) -> double& {
+              return R[(((size_t)nn * D + t) * D + u) * D + v];
+            };
+            double f = 1.0;
+            for (int nn = 0; nn <= Lt; ++nn) {
+              r(nn, 0, 0, 0) = f * F[nn];
+              f *= -2.0 * p;
+            }
+            for (int nn = Lt - 1; nn >= 0; --nn)
+              for (int t = 0; t <= Lt - nn; ++t)
+                for (int u = 0; u <= Lt - nn - t; ++u)
+                  for (int v = 0; v <= Lt - nn - t - u; ++v) {
+                    if (t + u + v == 0) continue;
+                    double val;
+                    if (t > 0)
+                      val = (t > 1 ? (t - 1) * r(nn + 1, t - 2, u, v) : 0.0) + X * r(nn + 1, t - 1, u, v);
+                    else if (u > 0)
+                      val = (u > 1 ? (u - 1) * r(nn + 1, t, u - 2, v) : 0.0) + Y * r(nn + 1, t, u - 1, v);
+                    else
+                      val = (v > 1 ? (v - 1) * r(nn + 1, t, u, v - 2) : 0.0) + Z * r(nn + 1, t, u, v - 1);
+                    r(nn, t, u, v) = val;
+                  }
+            for (int t = 0; t <= Lt; ++t)
+              for (int u = 0; u <= Lt - t; ++u)
+                for (int v = 0; v <= Lt - t - u; ++v)
+                  Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v] += -(double)m.Z[c] * r(0, t, u, v);
+          }
+          for (int ia = 0; ia < na; ++ia)
+            for (int ib = 0; ib < nb; ++ib) {
+              const int* la = kCart[La][ia];
+              const int* lb = kCart[Lb][ib];
+              const double Na = dg::gto_norm(a, la[0], la[1], la[2]);
+              const double Nb = dg::gto_norm(b, lb[0], lb[1], lb[2]);
+              const double pre = ca * cb * Na * Nb;
+              double s1[3], t1[3];
+              for (int d = 0; d < 3; ++d) {
+                const int i = la[d], j = lb[d];
+                s1[d] = H[d].e[i][j][0] * sq;
+                const double sp2 = H[d].e[i][j + 2][0] * sq;
+                const double sm2 = j >= 2 ? H[d].e[i][j - 2][0] * sq : 0.0;
+                t1[d] = -2.0 * b * b * sp2 + b * (2 * j + 1) * s1[d] - 0.5 * j * (j - 1) * sm2;
+              }
+              const double ov = s1[0] * s1[1] * s1[2];
+              const double kin = t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2];
+              double vn = 0.0;
+              for (int t = 0; t <= la[0] + lb[0]; ++t)
+                for (int u = 0; u <= la[1] + lb[1]; ++u)
+                  for (int v = 0; v <= la[2] + lb[2]; ++v)
+                    vn += H[0].e[la[0]][lb[0]][t] * H[1].e[la[1]][lb[1]][u] * H[2].e[la[2]][lb[2]][v] *
+                          Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v];
+              vn *= 2.0 * dg::kPi / p * vfac;
+              s_blk[ia * nb + ib] += pre * ov;
+              t_blk[ia * nb + ib] += pre * kin;
+              v_blk[ia * nb + ib] += pre * vn;
+            }
+        }
+      for (int ia = 0; ia < na; ++ia)
+        for (int ib = 0; ib < nb; ++ib) {
+          const size_t i = off[sa] + ia, j = off[sb] + ib;
+          S[i + j * n] = S[j + i * n] = s_blk[ia * nb + ib];
+          T[i + j * n] = T[j + i * n] = t_blk[ia * nb + ib];
+          V[i + j * n] = V[j + i * n] = v_blk[ia * nb + ib];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric eigen-solver: Householder reduction to tridiagonal form, then the
+// implicit QL iteration with Wilkinson-type shifts (EISPACK tred2/tql2
+// algorithms).  A column-major n x n; eigenvalues ascending, vectors in Z's columns.
+// ---------------------------------------------------------------------------
+void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::vector<double>& Z) {
+  std::vector<double> A(Ain);
+  auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
+  Z.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
+  auto z = [&](int i, int j) -> double& { return Z[i + (size_t)j * n]; };
+  std::vector<double> v(n), pv(n), qv(n), d(n), e(n, 0.0);
+  for (int k = 0; k + 2 < n; ++k) {
+    double norm2 = 0.0;
+    for (int i = k + 1; i < n; ++i) norm2 += a(i, k) * a(i, k);
+    const double xnorm = std::sqrt(norm2);
+    if (xnorm == 0.0) continue;
+    const double alpha = a(k + 1, k) > 0 ? -xnorm : xnorm;
+    for (int i = 0; i < n; ++i) v[i] = 0.0;
+    for (int i = k + 1; i < n; ++i) v[i] = a(i, k);
+    v[k + 1] -= alpha;
+    double vn2 = 0.0;
+    for (int i = k + 1; i < n; ++i) vn2 += v[i] * v[i];
+    if (vn2 == 0.0) continue;
+    const double inv = 1.0 / std::sqrt(vn2);
+    for (int i = k + 1; i < n; ++i) v[i] *= inv;
+    // A <- H A H on the trailing block, H = I - 2 v v^T
+    for (int i = k + 1; i < n; ++i) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s += a(i, j) * v[j];
+      pv[i] = s;
+    }
+    double K = 0.0;
+    for (int i = k + 1; i < n; ++i) K += v[i] * pv[i];
+    for (int i = k + 1; i < n; ++i) qv[i] = pv[i] - K * v[i];
+    for (int j = k + 1; j < n; ++j)
+      for (int i = k + 1; i < n; ++i) a(i, j) -= 2.0 * (v[i] * qv[j] + qv[i] * v[j]);
+    a(k + 1, k) = a(k, k + 1) = alpha;
+    for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+    // Z <- Z H
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s += z(i, j) * v[j];
+      for (int j = k + 1; j < n; ++j) z(i, j) -= 2.0 * s * v[j];
+    }
+  }
+  for (int i = 0; i < n; ++i) d[i] = a(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+  // implicit QL; e[i] couples d[i] and d[i+1]
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0; iter < 200; ++iter) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-16 * dd) break;
+      }
+      if (m == l) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = m - 1;
+      bool underflow = false;
+      for (; i >= l; --i) {
+        const double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int k = 0; k < n; ++k) {
+          const double fz = z(k, i + 1);
+          z(k, i + 1) = s * z(k, i) + c * fz;
+          z(k, i) = c * z(k, i) - s * fz;
+        }
+      }
+      if (underflow) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  w.resize(n);
+  std::vector<double> Zs((size_t)n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = d[idx[j]];
+    std::memcpy(&Zs[(size_t)j * n], &Z[(size_t)idx[j] * n], sizeof(double) * n);
+  }
+  Z.swap(Zs);
+}
+
+void gemm(int n, const double* A, bool ta, const double* B, double* C) {  // C = op(A) B, column-major
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += (ta ? A[k + (size_t)i * n] : A[i + (size_t)k * n]) * B[k + (size_t)j * n];
+      C[i + (size_t)j * n] = s;
+    }
+}
+
+// The reference's DFT class (src/dft.h:41-216) with the grid work on the GPU.
+class DFT {
+ public:
+  DFT(const MoleculeData& mol, const dg_grid_params& grid) : mol_(mol), grid_(mol) {
+    grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision);
+    grid_.create_grid();
+    n_ = grid_.nbf();
+    nelec_ = mol.nelec();
+  }
+  ~DFT() {
+    if (dP_) (void)hipFree(dP_);
+  }
+
+  // src/dft.cpp:160-227
+  void construct_matrices() {
+    std::vector<double> Tm, Vm;
+    one_electron(mol_, S_, Tm, Vm);
+    H_.resize(S_.size());
+    for (size_t i = 0; i < S_.size(); ++i) H_[i] = Tm[i] + Vm[i];
+    enuc_ = 0.0;
+    for (int i = 0; i < mol_.natoms(); ++i)
+      for (int j = i + 1; j < mol_.natoms(); ++j) {
+        const double dx = mol_.xyz[3 * i] - mol_.xyz[3 * j], dy = mol_.xyz[3 * i + 1] - mol_.xyz[3 * j + 1],
+                     dz = mol_.xyz[3 * i + 2] - mol_.xyz[3 * j + 2];
+        enuc_ += mol_.Z[i] * mol_.Z[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    // canonical orthogonalisation X = U s^{-1/2}, src/dft.cpp:300-316
+    std::vector<double> s, U;
+    eigh(n_, S_, s, U);
+    X_.assign((size_t)n_ * n_, 0.0);
+    for (int j = 0; j < n_; ++j) {
+      if (!(s[j] > 0.0)) throw std::runtime_error("overlap matrix is not positive definite");
+      const double f = 1.0 / std::sqrt(s[j]);
+      for (int i = 0; i < n_; ++i) X_[i + (size_t)j * n_] = U[i + (size_t)j * n_] * f;
+    }
+    J_.assign((size_t)n_ * n_, 0.0);
+    XC_.assign((size_t)n_ * n_, 0.0);
+    P_.assign((size_t)n_ * n_, 0.0);
+    exc_ = 0.0;  // the reference reads it before the first XC evaluation
+    if (hipMalloc((void**)&dP_, sizeof(double) * (3 * (size_t)n_ * n_ + 4)) != hipSuccess)
+      throw std::runtime_error("hipMalloc failed");
+    density_matrix();
+    grid_build(false);
+    energy();
+  }
+
+  // src/dft.cpp:84-142
+  int scf(double threshold, int min_iter, int max_iter, double* energies, double* sec_iter) {
+    double old_energy = et_, difference = 1.0;
+    int iteration = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (difference > threshold || iteration < min_iter) {
+      iteration++;
+      density_matrix();
+      grid_build(true);
+      energy();
+      if (energies) energies[iteration - 1] = et_;
+      difference = std::fabs(et_ - old_energy);
+      old_energy = et_;
+      if (iteration >= max_iter) break;
+    }
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (sec_iter) *sec_iter = iteration ? dt / iteration : 0.0;
+    return iteration;
+  }
+
+  double mixing = 0.5;
+  double et_ = 0.0, e_one_ = 0.0, e_j_ = 0.0, exc_ = 0.0, enuc_ = 0.0, nelec_grid_ = 0.0;
+  std::vector<double> P_;
+
+ private:
+  // src/dft.cpp:330-366
+  void density_matrix() {
+    const size_t nn = (size_t)n_ * n_;
+    std::vector<double> F(nn), tmp(nn), Fp(nn), Cc, C(nn), w;
+    for (size_t i = 0; i < nn; ++i) F[i] = H_[i] + 2.0 * J_[i] + XC_[i];
+    gemm(n_, X_.data(), true, F.data(), tmp.data());
+    std::vector<double> Xt(X_);
+    gemm(n_, tmp.data(), false, X_.data(), Fp.data());
+    eigh(n_, Fp, w, Cc);
+    // C = X Cc
+    for (int j = 0; j < n_; ++j)
+      for (int i = 0; i < n_; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < n_; ++k) s += X_[i + (size_t)k * n_] * Cc[k + (size_t)j * n_];
+        C[i + (size_t)j * n_] = s;
+      }
+    std::vector<double> Pn(nn, 0.0);
+    const int nocc = nelec_ / 2;
+    for (int j = 0; j < n_; ++j)
+      for (int i = 0; i < n_; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < nocc; ++k) s += C[i + (size_t)k * n_] * C[j + (size_t)k * n_];
+        Pn[i + (size_t)j * n_] = s;
+      }
+    if (first_) {
+      P_ = Pn;
+      first_ = false;
+    } else {
+      for (size_t i = 0; i < nn; ++i) P_[i] = (1.0 - mixing) * Pn[i] + mixing * P_[i];
+    }
+  }
+
+  // set_density + correct_densities + Hartree J (+ XC) on the GPU
+  void grid_build(bool with_xc) {
+    const size_t nn = (size_t)n_ * n_;
+    double* dJ = dP_ + nn;
+    double* dXC = dJ + nn;
+    double* dS = dXC + nn;
+    hipStream_t st;
+    void* sp = nullptr;
+    dg_throw(dg_get_stream(grid_.handle(), &sp));
+    st = (hipStream_t)sp;
+    if (hipMemcpyAsync(dP_, P_.data(), nn * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+      throw std::runtime_error("copy of P failed");
+    dg_throw(dg_build_device(grid_.handle(), dP_, dJ, dXC, dS));
+    double sc[4];
+    std::vector<double> XC(nn);
+    if (hipMemcpyAsync(J_.data(), dJ, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(XC.data(), dXC, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(sc, dS, 4 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      throw std::runtime_error("copy of grid results failed");
+    nelec_grid_ = sc[2];
+    if (with_xc) {
+      XC_.swap(XC);
+      exc_ = sc[0];
+    }
+  }
+
+  // src/dft.cpp:441-447
+  void energy() {
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n_; ++i)
+      for (int k = 0; k < n_; ++k) {
+        t1 += P_[i + (size_t)k * n_] * H_[k + (size_t)i * n_];
+        t2 += P_[i + (size_t)k * n_] * J_[k + (size_t)i * n_];
+      }
+    e_one_ = 2.0 * t1;
+    e_j_ = 2.0 * t2;
+    et_ = e_one_ + e_j_ + enuc_ + exc_;
+  }
+
+  MoleculeData mol_;
+  MolecularGrid grid_;
+  int n_ = 0, nelec_ = 0;
+  bool first_ = true;
+  std::vector<double> S_, H_, X_, J_, XC_;
+  double* dP_ = nullptr;
+};
+
+}  // namespace
+}  // namespace dftcxx
+
+namespace {
+dftcxx::MoleculeData to_data(const dg_molecule* mol, const dg_basis* basis) {
+  dftcxx::MoleculeData m;
+  m.Z.assign(mol->Z, mol->Z + mol->natoms);
+  m.xyz.assign(mol->xyz, mol->xyz + 3 * mol->natoms);
+  int nprim = 0;
+  for (int s = 0; s < basis->nshell; ++s) nprim = std::max(nprim, basis->offset[s] + basis->nprim[s]);
+  m.shell_atom.assign(basis->atom, basis->atom + basis->nshell);
+  m.shell_L.assign(basis->L, basis->L + basis->nshell);
+  m.shell_nprim.assign(basis->nprim, basis->nprim + basis->nshell);
+  m.shell_offset.assign(basis->offset, basis->offset + basis->nshell);
+  m.alpha.assign(basis->alpha, basis->alpha + nprim);
+  m.coef.assign(basis->coef, basis->coef + nprim);
+  for (int s = 0; s < basis->nshell; ++s) {
+    if (m.shell_L[s] < 0 || m.shell_L[s] > 2) throw std::runtime_error("shell L must be 0, 1 or 2");
+    if (m.shell_atom[s] < 0 || m.shell_atom[s] >= mol->natoms) throw std::runtime_error("shell atom out of range");
+  }
+  return m;
+}
+}  // namespace
+
+extern "C" int dg_one_electron(const dg_molecule* mol, const dg_basis* basis, double* S, double* T, double* V) {
+  if (!mol || !basis || mol->natoms < 1 || basis->nshell < 1) {
+    dg::set_last_error("dg_one_electron: empty molecule or basis");
+    return DG_ERR_INVALID;
+  }
+  try {
+    dftcxx::MoleculeData m = to_data(mol, basis);
+    std::vector<double> s, t, v;
+    dftcxx::one_electron(m, s, t, v);
+    if (S) std::memcpy(S, s.data(), s.size() * 8);
+    if (T) std::memcpy(T, t.data(), t.size() * 8);
+    if (V) std::memcpy(V, v.data(), v.size() * 8);
+    return DG_OK;
+  } catch (const std::exception& e) {
+    dg::set_last_error(std::string("dg_one_electron: ") + e.what());
+    return DG_ERR_INVALID;
+  }
+}
+
+extern "C" int dg_eigh(int n, const double* A, double* w, double* Z) {
+  if (n < 1 || !A || !w || !Z) {
+    dg::set_last_error("dg_eigh: bad argument");
+    return DG_ERR_INVALID;
+  }
+  std::vector<double> a(A, A + (size_t)n * n), ww, zz;
+  dftcxx::eigh(n, a, ww, zz);
+  std::memcpy(w, ww.data(), (size_t)n * 8);
+  std::memcpy(Z, zz.data(), (size_t)n * n * 8);
+  return DG_OK;
+}
+
+extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* grid,
+                          const dg_scf_params* scf, dg_scf_result* result, double* energies, double* P_out) {
+  if (!mol || !basis || !grid || !scf || !result) {
+    dg::set_last_error("dg_scf_run: null argument");
+    return DG_ERR_INVALID;
+  }
+  try {
+    dftcxx::MoleculeData m = to_data(mol, basis);
+    if (m.nelec() % 2 != 0) throw std::runtime_error("closed-shell SCF needs an even electron count");
+    const auto t0 = std::chrono::steady_clock::now();
+    dftcxx::DFT dft(m, *grid);
+    dft.mixing = scf->mixing;
+    dft.construct_matrices();
+    const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double sec_iter = 0.0;
+    const int it = dft.scf(scf->threshold, scf->min_iter, scf->max_iter, energies, &sec_iter);
+    result->iterations = it;
+    result->energy = dft.et_;
+    result->e_one = dft.e_one_;
+    result->e_j = dft.e_j_;
+    result->e_xc = dft.exc_;
+    result->e_nuc = dft.enuc_;
+    result->nelec = dft.nelec_grid_;
+    result->seconds_setup = setup;
+    result->seconds_iter = sec_iter;
+    if (P_out) std::memcpy(P_out, dft.P_.data(), sizeof(double) * dft.P_.size());
+    return DG_OK;
+  } catch (const std::exception& e) {
+    dg::set_last_error(std::string("dg_scf_run: ") + e.what());
+    return DG_ERR_STATE;
+  }
+}

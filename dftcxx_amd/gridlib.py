@@ -1,0 +1,349 @@
+"""Python binding of the dftgrid C ABI (include/dftgrid.h).
+
+:class:`MolecularGrid` mirrors the reference's ``MolecularGrid`` class
+(src/moleculargrid.h:48-190): same method names, same meaning of arguments,
+errors raised as exceptions.  Everything runs in libdftgrid.so on the GPU; if
+the library is missing or fails to load, :func:`lib` raises -- there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import lebedev as _leb
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdftgrid.so")
+_lib = None
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+# every entry point declared in include/dftgrid.h
+EXPORTS = (
+    "dg_last_error", "dg_version", "dg_lebedev_available", "dg_create", "dg_destroy", "dg_info",
+    "dg_set_stream", "dg_get_stream", "dg_set_density", "dg_correct_densities",
+    "dg_calculate_density", "dg_hartree", "dg_xc", "dg_get_weights", "dg_get_becke_weights",
+    "dg_get_positions", "dg_get_densities", "dg_get_amplitudes", "dg_get_hartree_potential",
+    "dg_get_hartree_parts", "dg_get_xc_parts", "dg_build_device", "dg_profile_enable",
+    "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_scf_run", "dg_one_electron",
+    "dg_eigh",
+)
+
+
+class DgError(RuntimeError):
+    """A dftgrid call returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"dftgrid error {code}: {msg}")
+        self.code = code
+
+
+class _Molecule(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("Z", _I), ("xyz", _D)]
+
+
+class _Basis(C.Structure):
+    _fields_ = [("nshell", C.c_int), ("atom", _I), ("L", _I), ("nprim", _I), ("offset", _I),
+                ("alpha", _D), ("coef", _D)]
+
+
+class _GridParams(C.Structure):
+    _fields_ = [("n_radial", C.c_int), ("n_angular", C.c_int), ("lmax", C.c_int),
+                ("precision", C.c_int)]
+
+
+class _ScfParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
+                ("mixing", C.c_double)]
+
+
+class _ScfResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("energy", C.c_double), ("e_one", C.c_double),
+                ("e_j", C.c_double), ("e_xc", C.c_double), ("e_nuc", C.c_double),
+                ("nelec", C.c_double), ("seconds_setup", C.c_double),
+                ("seconds_iter", C.c_double)]
+
+
+def lib():
+    """Load libdftgrid.so (raises if it is missing: no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m dftcxx_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.dg_last_error.restype = C.c_char_p
+        L.dg_create.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(_GridParams),
+                                C.POINTER(C.c_void_p)]
+        L.dg_destroy.argtypes = [C.c_void_p]
+        L.dg_info.argtypes = [C.c_void_p, _I, C.POINTER(C.c_long), _I]
+        L.dg_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.dg_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.dg_set_density.argtypes = [C.c_void_p, _D, C.c_int]
+        L.dg_correct_densities.argtypes = [C.c_void_p, _D]
+        L.dg_calculate_density.argtypes = [C.c_void_p, _D]
+        L.dg_hartree.argtypes = [C.c_void_p, _D]
+        L.dg_xc.argtypes = [C.c_void_p, _D, _D]
+        for n in ("dg_get_weights", "dg_get_becke_weights", "dg_get_positions", "dg_get_densities",
+                  "dg_get_amplitudes", "dg_get_hartree_potential"):
+            getattr(L, n).argtypes = [C.c_void_p, _D]
+        L.dg_get_hartree_parts.argtypes = [C.c_void_p, _D, _D, _D]
+        L.dg_get_xc_parts.argtypes = [C.c_void_p, _D, _D]
+        L.dg_build_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dg_profile_enable.argtypes = [C.c_void_p, C.c_int]
+        L.dg_profile_read.argtypes = [C.c_void_p, C.c_int, C.c_char_p, _D, _I, _I]
+        L.dg_profile_reset.argtypes = [C.c_void_p]
+        L.dg_selftest_mfma.argtypes = [_I]
+        L.dg_scf_run.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(_GridParams),
+                                 C.POINTER(_ScfParams), C.POINTER(_ScfResult), _D, _D]
+        L.dg_one_electron.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), _D, _D, _D]
+        L.dg_eigh.argtypes = [C.c_int, _D, _D, _D]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DgError(rc, lib().dg_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _c_inputs(mol, nrad, nang, lmax, precision):
+    s = mol.shells()
+    keep = {"Z": _i32(mol.Z), "xyz": _f64(mol.xyz), "atom": _i32(s["atom"]), "L": _i32(s["L"]),
+            "nprim": _i32(s["nprim"]), "offset": _i32(s["offset"]), "alpha": _f64(s["alpha"]),
+            "coef": _f64(s["coef"])}
+    m = _Molecule(mol.natoms, _i(keep["Z"]), _d(keep["xyz"]))
+    b = _Basis(len(keep["atom"]), _i(keep["atom"]), _i(keep["L"]), _i(keep["nprim"]),
+               _i(keep["offset"]), _d(keep["alpha"]), _d(keep["coef"]))
+    g = _GridParams(nrad, nang, lmax, precision)
+    return m, b, g, keep
+
+
+def selftest_mfma():
+    ok = C.c_int(0)
+    _check(lib().dg_selftest_mfma(C.byref(ok)))
+    return bool(ok.value)
+
+
+class MolecularGrid:
+    """GPU molecular grid with the reference's MolecularGrid call surface.
+
+    ``MolecularGrid(mol)`` then ``set_grid_parameters(radial, lebedev_order,
+    lmax)`` and ``create_grid()`` as in src/dft.cpp:57-61, where
+    ``lebedev_order`` is the reference's enum index (0..10 -> 6..194 points);
+    or pass ``radial_points``/``angular_points`` (a point count, e.g. 302)
+    directly, which builds the grid at once.
+    """
+
+    def __init__(self, mol, radial_points=None, angular_points=None, lmax=8, precision=PREC_FP64):
+        self.mol = mol
+        self.h = None
+        self.radial_points = radial_points
+        self.angular_points = angular_points
+        self.lmax = lmax
+        self.precision = precision
+        if radial_points is not None and angular_points is not None:
+            self.create_grid()
+
+    # -- set-up (src/moleculargrid.cpp:175-261) ------------------------------
+    def set_grid_parameters(self, radial_points, lebedev_order, lmax):
+        self.radial_points = int(radial_points)
+        self.angular_points = _leb.points_from_reference_index(int(lebedev_order))
+        self.lmax = int(lmax)
+
+    def create_grid(self):
+        if self.h is not None:
+            raise RuntimeError("grid already created")
+        m, b, g, keep = _c_inputs(self.mol, self.radial_points, self.angular_points, self.lmax,
+                                  self.precision)
+        h = C.c_void_p()
+        _check(lib().dg_create(C.byref(m), C.byref(b), C.byref(g), C.byref(h)))
+        self.h = h
+        nbf, npts, nat = C.c_int(), C.c_long(), C.c_int()
+        _check(lib().dg_info(self.h, C.byref(nbf), C.byref(npts), C.byref(nat)))
+        self.nbf, self.npts, self.natoms = nbf.value, npts.value, nat.value
+        self.nlm = (self.lmax + 1) ** 2
+
+    def close(self):
+        if self.h is not None:
+            lib().dg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _need(self):
+        if self.h is None:
+            raise RuntimeError("create_grid() has not been called")
+
+    # -- per-iteration work ---------------------------------------------------
+    def set_density(self, P):
+        """rho = 2 phi^T P phi at every point (src/moleculargrid.cpp:48-53)."""
+        self._need()
+        P = _f64(np.asarray(P).T)  # column-major
+        if P.shape != (self.nbf, self.nbf):
+            raise ValueError(f"P must be {self.nbf}x{self.nbf}")
+        _check(lib().dg_set_density(self.h, _d(P), self.nbf))
+
+    def correct_densities(self):
+        """Scale rho to the electron count; returns the integral before (src/moleculargrid.cpp:132-146)."""
+        self._need()
+        before = C.c_double()
+        _check(lib().dg_correct_densities(self.h, C.byref(before)))
+        return before.value
+
+    def calculate_density(self):
+        self._need()
+        n = C.c_double()
+        _check(lib().dg_calculate_density(self.h, C.byref(n)))
+        return n.value
+
+    def calculate_hartree_potential(self):
+        """J matrix from the Becke-Poisson Hartree potential (src/moleculargrid.cpp:336-389)."""
+        self._need()
+        J = np.empty((self.nbf, self.nbf))
+        _check(lib().dg_hartree(self.h, _d(J)))
+        return J.T.copy()
+
+    def calculate_exchange_correlation(self):
+        """(XC matrix, E_xc) as DFT::calculate_exchange_correlation_matrix (src/dft.cpp:394-433)."""
+        self._need()
+        XC = np.empty((self.nbf, self.nbf))
+        e = C.c_double()
+        _check(lib().dg_xc(self.h, _d(XC), C.byref(e)))
+        return XC.T.copy(), e.value
+
+    # -- getters (src/moleculargrid.cpp:61-127) ---------------------------------
+    def _vec(self, fn, n):
+        self._need()
+        out = np.empty(n)
+        _check(fn(self.h, _d(out)))
+        return out
+
+    def get_weights(self):
+        return self._vec(lib().dg_get_weights, self.npts)
+
+    def get_becke_weights(self):
+        return self._vec(lib().dg_get_becke_weights, self.npts)
+
+    def get_densities(self):
+        return self._vec(lib().dg_get_densities, self.npts)
+
+    def get_hartree_potential(self):
+        return self._vec(lib().dg_get_hartree_potential, self.npts)
+
+    def get_positions(self):
+        return self._vec(lib().dg_get_positions, 3 * self.npts).reshape(self.npts, 3)
+
+    def get_amplitudes(self):
+        """Amplitudes as (npts, nbf); the reference returns the transpose (nbf x npts)."""
+        return self._vec(lib().dg_get_amplitudes, self.npts * self.nbf).reshape(self.npts, self.nbf)
+
+    def get_hartree_parts(self):
+        self._need()
+        shape = (self.natoms, self.radial_points, self.nlm)
+        rho_lm, U, vcell = np.empty(shape), np.empty(shape), np.empty(self.npts)
+        _check(lib().dg_get_hartree_parts(self.h, _d(rho_lm), _d(U), _d(vcell)))
+        return {"rho_lm": rho_lm, "U_lm": U, "vcell": vcell}
+
+    def get_xc_parts(self):
+        self._need()
+        e, wv = np.empty(self.npts), np.empty(self.npts)
+        _check(lib().dg_get_xc_parts(self.h, _d(e), _d(wv)))
+        return e, wv
+
+    # -- device-pointer build and profiling ---------------------------------
+    def set_stream(self, stream_ptr):
+        self._need()
+        _check(lib().dg_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def build_device(self, dP, dJ=0, dXC=0, dscalars=0):
+        """Enqueue one full grid build on device pointers (ints), see dg_build_device."""
+        self._need()
+        _check(lib().dg_build_device(self.h, C.c_void_p(dP), C.c_void_p(dJ), C.c_void_p(dXC),
+                                     C.c_void_p(dscalars)))
+
+    def profile_enable(self, on=True):
+        self._need()
+        _check(lib().dg_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._need()
+        _check(lib().dg_profile_reset(self.h))
+
+    def profile_read(self):
+        self._need()
+        maxn = 64
+        names = C.create_string_buffer(48 * maxn)
+        ms = np.zeros(maxn)
+        cnt = np.zeros(maxn, dtype=np.int32)
+        n = C.c_int()
+        _check(lib().dg_profile_read(self.h, maxn, names, _d(ms), _i(cnt), C.byref(n)))
+        out = {}
+        for i in range(min(n.value, maxn)):
+            nm = names.raw[48 * i:48 * i + 48].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
+
+
+def one_electron(mol):
+    """(S, T, V) one-electron matrices on the host (src/dft.cpp:184-201)."""
+    m, b, g, keep = _c_inputs(mol, 5, 6, 0, PREC_FP64)
+    n = mol.nbf()
+    S, T, V = (np.empty((n, n)) for _ in range(3))
+    _check(lib().dg_one_electron(C.byref(m), C.byref(b), _d(S), _d(T), _d(V)))
+    return S.T.copy(), T.T.copy(), V.T.copy()
+
+
+def eigh(A):
+    """Ascending eigenvalues and eigenvectors of a symmetric matrix (host)."""
+    A = _f64(np.asarray(A).T)
+    n = A.shape[0]
+    w, Z = np.empty(n), np.empty((n, n))
+    _check(lib().dg_eigh(n, _d(A), _d(w), _d(Z)))
+    return w, Z.T.copy()
+
+
+def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5,
+            precision=PREC_FP64):
+    """Reference SCF (src/dft.cpp:84-142) with the grid work on the GPU.
+
+    Returns (energies per iteration, result dict, P)."""
+    m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision)
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing)
+    res = _ScfResult()
+    en = np.zeros(max_iter)
+    n = mol.nbf()
+    P = np.zeros((n, n))
+    _check(lib().dg_scf_run(C.byref(m), C.byref(b), C.byref(g), C.byref(sp), C.byref(res), _d(en), _d(P)))
+    r = {k: getattr(res, k) for k, _ in _ScfResult._fields_}
+    return en[:res.iterations], r, P.T.copy()

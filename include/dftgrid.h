@@ -1,0 +1,174 @@
+/*
+ * dftgrid -- C ABI of the MI355X grid-integration library.
+ *
+ * One handle owns the device state of one molecular grid: points, weights,
+ * basis amplitudes, density, Hartree potential and the matrices built from
+ * them.  It is the drop-in for the reference's MolecularGrid (src/moleculargrid.h)
+ * together with the exchange-correlation assembly of DFT
+ * (src/dft.cpp:394-433).  The reference has no C or FFI interface of its own;
+ * each entry point below names the reference member function whose result it
+ * reproduces.
+ *
+ * Conventions
+ *  - Every function returns DG_OK (0) or an error code; dg_last_error() gives
+ *    a message for the calling thread.  No C++ exception crosses the ABI.
+ *  - Matrices are nbf x nbf, column-major (as the reference's Eigen
+ *    matrices), double precision.  P is the reference's half density matrix
+ *    (sum over occupied orbitals of C C^T, src/dft.cpp:345-352).
+ *  - Coordinates are in bohr.
+ *  - Grid points are ordered atom by atom (input order), then radial node
+ *    p = 1..N (r decreasing), then Lebedev point (src/atomicgrid.cpp:50-82).
+ *  - The basis is a list of shells in the reference's CGF order
+ *    (src/molecule.cpp:135-238); shell L = 0, 1, 2 expands to 1, 3, 6 CGFs in
+ *    the order s; px, py, pz; dxx, dxy, dxz, dyy, dyz, dzz (src/cgf.h:208-221).
+ *  - A handle is not thread-safe.  All its work runs on one HIP stream (its own
+ *    by default, or one set with dg_set_stream).  "_device" entry points take
+ *    device pointers and enqueue work without synchronising; the host-pointer
+ *    entry points synchronise before they return.
+ */
+#ifndef DFTGRID_H
+#define DFTGRID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG_OK 0
+#define DG_ERR_INVALID 1  /* bad argument or size */
+#define DG_ERR_HIP 2      /* a HIP runtime call failed */
+#define DG_ERR_ALLOC 3    /* device or host allocation failed */
+#define DG_ERR_STATE 4    /* call out of order (e.g. hartree before set_density) */
+#define DG_ERR_UNSUPPORTED 5
+
+#define DG_PREC_FP64 0  /* every stage in FP64 */
+#define DG_PREC_MIXED 1 /* FP64 density/energies, reduced-precision Fock contractions */
+
+typedef struct dg_handle dg_handle;
+
+typedef struct {
+  int natoms;
+  const int* Z;       /* [natoms] nuclear charges (1..18) */
+  const double* xyz;  /* [natoms][3] positions in bohr */
+} dg_molecule;
+
+typedef struct {
+  int nshell;
+  const int* atom;      /* [nshell] centre atom index */
+  const int* L;         /* [nshell] 0, 1 or 2 */
+  const int* nprim;     /* [nshell] primitives per shell */
+  const int* offset;    /* [nshell] first primitive in alpha/coef */
+  const double* alpha;  /* primitive exponents */
+  const double* coef;   /* contraction coefficients (unnormalised primitives) */
+} dg_basis;
+
+typedef struct {
+  int n_radial;   /* Gauss-Chebyshev radial nodes per atom */
+  int n_angular;  /* Lebedev points per shell (a point count, e.g. 110, 302) */
+  int lmax;       /* spherical-harmonic cut-off of the Hartree expansion */
+  int precision;  /* DG_PREC_FP64 or DG_PREC_MIXED */
+} dg_grid_params;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* dg_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int dg_version(void);
+/* 1 if a Lebedev rule with that many points is compiled in, else 0. */
+int dg_lebedev_available(int npoints);
+
+/* Build the molecular grid on the current HIP device: points, Becke weights
+ * and basis amplitudes.  Replaces MolecularGrid(mol) + set_grid_parameters()
+ * + create_grid() (src/moleculargrid.h:84,104,118; src/moleculargrid.cpp:175-261). */
+int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* params,
+              dg_handle** out);
+int dg_destroy(dg_handle* h);
+int dg_info(const dg_handle* h, int* nbf, long* npts, int* natoms);
+/* Use this HIP stream (a hipStream_t) for all later work; NULL = own stream. */
+int dg_set_stream(dg_handle* h, void* stream);
+int dg_get_stream(const dg_handle* h, void** stream);
+
+/* --- host-pointer API, one call per reference member function ---------- */
+/* MolecularGrid::set_density(P) (src/moleculargrid.cpp:48-53): rho = 2 phi^T P phi. */
+int dg_set_density(dg_handle* h, const double* P, int nbf);
+/* MolecularGrid::correct_densities() (src/moleculargrid.cpp:132-146): scale rho to
+ * sum(Z) electrons.  nelec_before (may be NULL) receives the integral before. */
+int dg_correct_densities(dg_handle* h, double* nelec_before);
+/* MolecularGrid::calculate_density() (src/moleculargrid.cpp:157-166). */
+int dg_calculate_density(dg_handle* h, double* nelec);
+/* MolecularGrid::calculate_hartree_potential() (src/moleculargrid.cpp:336-389): J. */
+int dg_hartree(dg_handle* h, double* J);
+/* DFT::calculate_exchange_correlation_matrix() (src/dft.cpp:394-433): XC and E_xc. */
+int dg_xc(dg_handle* h, double* XC, double* exc);
+/* Getters (src/moleculargrid.cpp:61-127): weights include the Becke factor. */
+int dg_get_weights(dg_handle* h, double* w);            /* [npts] */
+int dg_get_becke_weights(dg_handle* h, double* wb);     /* [npts] */
+int dg_get_positions(dg_handle* h, double* xyz);        /* [npts][3] */
+int dg_get_densities(dg_handle* h, double* rho);        /* [npts] */
+int dg_get_amplitudes(dg_handle* h, double* phi);       /* [npts][nbf] */
+int dg_get_hartree_potential(dg_handle* h, double* V);  /* [npts], after dg_hartree */
+/* Hartree intermediates after dg_hartree: rho_lm, U_lm [natoms][nrad][nlm]
+ * (src/atomicgrid.cpp:241-296, 301-432) and the own-cell potential [npts]
+ * (src/atomicgrid.cpp:434-462).  Any pointer may be NULL. */
+int dg_get_hartree_parts(dg_handle* h, double* rho_lm, double* U_lm, double* vcell);
+/* Pointwise XC after dg_xc: energy density e_x+e_c and w*(v_xa+v_xb+v_ca+v_cb)/2 [npts]. */
+int dg_get_xc_parts(dg_handle* h, double* exc_density, double* wv);
+
+/* --- device-pointer API --------------------------------------------------- */
+/* One full grid build for one SCF iteration, enqueued on the handle's stream:
+ * set_density(P) + correct_densities() + calculate_hartree_potential() +
+ * the XC assembly.  dP, dJ, dXC are device arrays nbf*nbf (column-major);
+ * dscalars (device, 4 doubles) receives {E_xc, N_elec before correction,
+ * N_elec after correction, 0}.  Any output pointer may be NULL. */
+int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, double* dscalars);
+
+/* --- profiling ------------------------------------------------------------- */
+/* When enabled, every kernel of dg_build_device is bracketed by HIP events on
+ * the handle's stream and its time accumulated per kernel name. */
+int dg_profile_enable(dg_handle* h, int enable);
+/* Copies up to max entries: name (NUL-terminated, <= 47 chars) into names[i*48],
+ * total milliseconds and launch count.  *n receives the number of kernels. */
+int dg_profile_read(dg_handle* h, int max, char* names, double* total_ms, int* launches, int* n);
+int dg_profile_reset(dg_handle* h);
+
+/* Checks the lane layout of the FP64 16x16x4 matrix-core instruction the
+ * contraction kernels rely on; *ok = 1 if it matches. */
+int dg_selftest_mfma(int* ok);
+
+/* --- SCF driver (host C++, src/dft.cpp:84-142, 160-227, 300-366, 441-447) --- */
+typedef struct {
+  double threshold;  /* stop when |dE| <= threshold (after min_iter) */
+  int min_iter;      /* reference: 3 */
+  int max_iter;      /* reference: 100 */
+  double mixing;     /* reference: 0.5 (weight of the old P) */
+} dg_scf_params;
+
+typedef struct {
+  int iterations;
+  double energy;        /* total */
+  double e_one;         /* 2 tr(P H) */
+  double e_j;           /* 2 tr(P J) */
+  double e_xc;
+  double e_nuc;
+  double nelec;         /* grid electron count after the last iteration */
+  double seconds_setup; /* integrals + grid construction */
+  double seconds_iter;  /* mean wall seconds per SCF iteration */
+} dg_scf_result;
+
+/* Runs the reference SCF with the grid work on the GPU.  energies (may be NULL)
+ * receives the total energy of each iteration (max_iter doubles); P_out (may
+ * be NULL) the final density matrix. */
+int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* grid,
+               const dg_scf_params* scf, dg_scf_result* result, double* energies, double* P_out);
+
+/* --- host utilities (no GPU needed) ----------------------------------------- */
+/* Overlap, kinetic and nuclear-attraction matrices (nbf x nbf, column-major)
+ * of the basis, as DFT::construct_matrices builds them (src/dft.cpp:184-201,
+ * src/integrals.cpp:43-180).  Any output may be NULL. */
+int dg_one_electron(const dg_molecule* mol, const dg_basis* basis, double* S, double* T, double* V);
+/* Eigenvalues (ascending) and column-major eigenvectors of a symmetric
+ * column-major n x n matrix (stands in for Eigen::SelfAdjointEigenSolver). */
+int dg_eigh(int n, const double* A, double* w, double* Z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFTGRID_H */
