@@ -198,8 +198,8 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 
 // One wave handles 16 points; lane (c = lane%16, g = lane/16) keeps
 // phi[p0+c][kk(s,g)] for every reduction step s in registers (B operand).
-// For output tile b the wave forms T[p][16b+4g+v] = (P phi_p)_j with 4*NT
-// MFMAs, then dots it with phi[p][16b+4g+v].  The P fragments of one tile are
+// For output tile b the wave forms T[p][16b+4v+g] = (P phi_p)_j (result
+// register v) with 4*NT MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one tile are
 // staged in LDS and shared by the block's 4 waves.
 template <int NT>
 __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
@@ -222,8 +222,9 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
     dbl4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4 * NT; ++s) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
-    const double* rb = row + 16 * b + 4 * g;
-    acc += d[0] * rb[0] + d[1] * rb[1] + d[2] * rb[2] + d[3] * rb[3];
+    // register v of lane (c, g) holds T[p0+c][16b + 4v + g]
+    const double* rb = row + 16 * b + g;
+    acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
   }
   acc += __shfl_xor(acc, 16);
   acc += __shfl_xor(acc, 32);
@@ -231,16 +232,16 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
 }
 
 // out[a] = sum over the points of cell a of x*y, fixed order (deterministic).
-__global__ __launch_bounds__(256) void k_cell_sum(const double* __restrict__ x,
-                                                  const double* __restrict__ y, long cell,
-                                                  double* out) {
-  __shared__ double red[256];
+__global__ __launch_bounds__(1024) void k_cell_sum(const double* __restrict__ x,
+                                                   const double* __restrict__ y, long cell,
+                                                   double* out) {
+  __shared__ double red[1024];
   const long base = (long)blockIdx.x * cell;
   double s = 0.0;
-  for (long i = threadIdx.x; i < cell; i += 256) s += x[base + i] * y[base + i];
+  for (long i = threadIdx.x; i < cell; i += 1024) s += x[base + i] * y[base + i];
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = 512; st > 0; st >>= 1) {
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
@@ -344,28 +345,48 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
 // Hartree potential (src/atomicgrid.cpp:241-466, src/moleculargrid.cpp:336-389)
 // ---------------------------------------------------------------------------
 
-// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p,  one block per (a, i).
-__global__ void k_rho_lm(const double* __restrict__ rho, const double* __restrict__ wb,
-                         const double* __restrict__ leb, const double* __restrict__ ylm, int nang,
-                         int nlm, double* rho_lm) {
+// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p,  one block per (a, i):
+// the shell's rho w_j wb_p go to LDS, wave w sums the points j = w (mod 4) for
+// lm = lane, lane+64, lane+128, and the four partial sums are added in order.
+__global__ __launch_bounds__(256) void k_rho_lm(const double* __restrict__ rho,
+                                                const double* __restrict__ wb,
+                                                const double* __restrict__ leb,
+                                                const double* __restrict__ ylm, int nang, int nlm,
+                                                double* rho_lm) {
+  extern __shared__ double sh[];
+  double* f = sh;             // nang
+  double* part = sh + nang;   // 4 x 192
   const long ai = blockIdx.x;  // a*nrad + i
-  const int lm = threadIdx.x;
   const long base = ai * nang;
-  if (lm >= nlm) return;
-  double acc = 0.0;
-  for (int j = 0; j < nang; ++j) {
-    const double f = rho[base + j] * leb[4 * j + 3] * wb[base + j];
-    acc += f * ylm[(long)j * nlm + lm];
+  for (int j = threadIdx.x; j < nang; j += 256) f[j] = rho[base + j] * leb[4 * j + 3] * wb[base + j];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int j = wave; j < nang; j += 4) {
+    const double fj = f[j];
+    const double* y = ylm + (long)j * nlm;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (lane + 64 * q < nlm) acc[q] += fj * y[lane + 64 * q];
   }
-  rho_lm[ai * nlm + lm] = acc * (4.0 * kPiD);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) part[wave * 192 + lane + 64 * q] = acc[q];
+  __syncthreads();
+  for (int lm = threadIdx.x; lm < nlm; lm += 256)
+    rho_lm[ai * nlm + lm] = (part[lm] + part[192 + lm] + part[384 + lm] + part[576 + lm]) * (4.0 * kPiD);
 }
 
-// U[a][i][lm] = sum_k Minv_l[i+1][k] g_k with g_0 = sqrt(4 pi) q_a [lm = 0],
-// g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm], g_{N+1} = 0 (src/atomicgrid.cpp:395-432).
-// minvT[l][k][i] holds rows 1..N of the inverse FD matrix, transposed.
+// Radial Poisson solve and spline slopes for one (a, lm) per block
+// (src/atomicgrid.cpp:395-432, src/cspline.cpp:66-127, src/atomicgrid.cpp:535-556):
+// U_i = sum_k KU_l[k][i] g_k  with g_0 = sqrt(4 pi) q_a [lm = 0],
+// g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm], g_{N+1} = 0, KU_l = rows 1..N of the
+// inverse FD matrix; the not-a-knot slopes D of U over the ascending knots are
+// a fixed linear map of U, so D_t = sum_k KD_l[k][t] g_k with KD_l = S KU_l.
+// tab[a][t][lm] = {U at knot t, D_t}; knot t is radial node N-1-t.
 __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
-                          const double* __restrict__ rad, const double* __restrict__ minvT, int nrad,
-                          int nlm, double* U) {
+                          const double* __restrict__ rad, const double* __restrict__ KU,
+                          const double* __restrict__ KD, int nrad, int nlm, double* U,
+                          double* tab) {
   extern __shared__ double gv[];
   const int a = blockIdx.x / nlm, lm = blockIdx.x % nlm;
   int l = 0;
@@ -382,73 +403,43 @@ __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __res
   __syncthreads();
   const int i = threadIdx.x;
   if (i >= nrad) return;
-  const double* M = minvT + (long)l * n2 * nrad;
-  double acc = 0.0;
-  for (int k = 0; k < n2; ++k) acc += M[(long)k * nrad + i] * gv[k];
-  U[((long)a * nrad + i) * nlm + lm] = acc;
-}
-
-// Not-a-knot spline slopes of U_lm over the ascending knots (reversed radial
-// nodes), one thread per (a, lm) (src/cspline.cpp:66-127,
-// src/atomicgrid.cpp:535-556).  tab[a][t][lm] = {y_t, D_t}.
-__global__ void k_spline(const double* __restrict__ U, const double* __restrict__ kn,
-                         const double* __restrict__ sa, const double* __restrict__ sden,
-                         const double* __restrict__ scmod, int nrad, int nlm, int natoms,
-                         double* tab) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= natoms * nlm) return;
-  const int a = g / nlm, lm = g % nlm;
-  const int n = nrad;
-  auto yv = [&](int t) { return U[((long)a * nrad + (n - 1 - t)) * nlm + lm]; };
-  double* T = tab + (long)a * n * nlm * 2 + lm * 2;
-  const long st = (long)nlm * 2;
-  // right-hand side and forward sweep; y'_t is parked in the D slot
-  double h0 = kn[1] - kn[0], h1 = kn[2] - kn[1];
-  double y0 = yv(0), y1 = yv(1), y2 = yv(2);
-  double r0 = (y1 - y0) / h0, r1 = (y2 - y1) / h1;
-  double Y = r0 * (3 * h0 * h1 + 2 * h1 * h1) + r1 * h0 * h0;
-  double yp = Y / sden[0];
-  T[0] = y0;
-  T[1] = yp;
-  for (int t = 1; t < n - 1; ++t) {
-    const double ym = yv(t - 1), yc = yv(t), yn = yv(t + 1);
-    h0 = kn[t] - kn[t - 1];
-    h1 = kn[t + 1] - kn[t];
-    r0 = (yc - ym) / h0;
-    r1 = (yn - yc) / h1;
-    Y = 3 * (r0 * h1 + r1 * h0);
-    yp = (Y - sa[t] * yp) / sden[t];
-    T[t * st] = yc;
-    T[t * st + 1] = yp;
+  const double* Mu = KU + (long)l * n2 * nrad;
+  const double* Md = KD + (long)l * n2 * nrad;
+  double u = 0.0, d = 0.0;
+  for (int k = 0; k < n2; ++k) {
+    const double gk = gv[k];
+    u += Mu[(long)k * nrad + i] * gk;
+    d += Md[(long)k * nrad + i] * gk;
   }
-  Y = r0 * h1 * h1 + r1 * (3 * h0 * h1 + 2 * h0 * h0);
-  yp = (Y - sa[n - 1] * yp) / sden[n - 1];
-  T[(n - 1) * st] = yv(n - 1);
-  T[(n - 1) * st + 1] = yp;
-  double D = yp;
-  for (int t = n - 2; t >= 0; --t) {
-    D = T[t * st + 1] - scmod[t] * D;
-    T[t * st + 1] = D;
-  }
+  U[((long)a * nrad + i) * nlm + lm] = u;
+  double* T = tab + (long)a * nrad * nlm * 2 + lm * 2;
+  T[(long)(nrad - 1 - i) * nlm * 2] = u;
+  T[(long)i * nlm * 2 + 1] = d;
 }
 
 // Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
-// (src/atomicgrid.cpp:434-462).  ylmT is [lm][j].
-__global__ void k_vcell(const double* __restrict__ U, const double* __restrict__ ylmT,
-                        const double* __restrict__ atoms, const double* __restrict__ px,
-                        const double* __restrict__ py, const double* __restrict__ pz, long npts,
-                        int nrad, int nang, int nlm, double* vcell) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npts) return;
-  const int j = (int)(p % nang);
-  const long ai = p / nang;
+// (src/atomicgrid.cpp:434-462), one block per (a, i); ylmT is [lm][j].
+__global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U,
+                                               const double* __restrict__ ylmT,
+                                               const double* __restrict__ atoms,
+                                               const double* __restrict__ px,
+                                               const double* __restrict__ py,
+                                               const double* __restrict__ pz, int nrad, int nang,
+                                               int nlm, double* vcell) {
+  extern __shared__ double u[];
+  const long ai = blockIdx.x;
   const int a = (int)(ai / nrad);
-  const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
-  const double rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-  const double* u = U + ai * nlm;
-  double acc = 0.0;
-  for (int lm = 0; lm < nlm; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
-  vcell[p] = rinv * acc;
+  for (int lm = threadIdx.x; lm < nlm; lm += 256) u[lm] = U[ai * nlm + lm];
+  __syncthreads();
+  const double Ax = atoms[4 * a], Ay = atoms[4 * a + 1], Az = atoms[4 * a + 2];
+  for (int j = threadIdx.x; j < nang; j += 256) {
+    const long p = ai * nang + j;
+    const double dx = px[p] - Ax, dy = py[p] - Ay, dz = pz[p] - Az;
+    const double rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    double acc = 0.0;
+    for (int lm = 0; lm < nlm; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
+    vcell[p] = rinv * acc;
+  }
 }
 
 struct CrossArgs {
@@ -619,7 +610,8 @@ __global__ void k_vreduce(const double* __restrict__ vpart, const double* __rest
 // ---------------------------------------------------------------------------
 // Block (bi <= bj) of 32x32 outputs per workgroup, split-K over points.  Each
 // wave accumulates 2x2 tiles of 16x16 with the FP64 16x16x4 MFMA: lane
-// (c, g) feeds A = phi[p_g][i0 + c] and B = u[p_g] phi[p_g][j0 + c]; waves
+// (c, g) feeds A = phi[p_g][i0 + c] and B = u[p_g] phi[p_g][j0 + c] and
+// receives C[i0 + 4v + g][j0 + c] in result register v; waves
 // interleave 4-point steps; the 4 waves are summed in LDS in a fixed order.
 // part[split][blk][o][32][32].
 template <int NOUT>
@@ -675,36 +667,46 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
       const double s = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
       const int ln = e & 63, v = (e >> 6) & 3, ab = e >> 8;
       const int ta = ab >> 1, tb = ab & 1;
-      const int il = 16 * ta + 4 * (ln >> 4) + v, jl = 16 * tb + (ln & 15);
+      const int il = 16 * ta + 4 * v + (ln >> 4), jl = 16 * tb + (ln & 15);
       dst[il * 32 + jl] = s;
     }
   }
 }
 
-// out_o[i + j nbf] = sum over splits of the block holding (i, j) (or (j, i)).
+// out_o = sum over splits of the partial blocks; one thread per block element
+// (coalesced over splits); the upper block triangle is mirrored, so the
+// output is exactly symmetric.
 __global__ void k_contract_reduce(const double* __restrict__ part, int nsplit, int nblk, int nout,
-                                  int nb32, int nbf, double* out0, double* out1) {
+                                  const int* __restrict__ blocks, int nbf, double* out0,
+                                  double* out1) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)nbf * nbf) return;
-  int i = (int)(e % nbf), j = (int)(e / nbf);
-  if (i / 32 > j / 32) {
-    const int t = i;
-    i = j;
-    j = t;
-  }
-  const int bi = i / 32, bj = j / 32;
-  const int blk = bi * nb32 - bi * (bi - 1) / 2 + (bj - bi);
-  const int idx = (i % 32) * 32 + (j % 32);
-  for (int o = 0; o < nout; ++o) {
-    double s = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) s += part[(((long)sp * nblk + blk) * nout + o) * 1024 + idx];
-    double* out = (o == 0) ? out0 : out1;
-    if (out) out[e] = s;
-  }
+  if (e >= (long)nblk * nout * 1024) return;
+  const int idx = (int)(e & 1023);
+  const int o = (int)((e >> 10) % nout);
+  const int blk = (int)((e >> 10) / nout);
+  const int bi = blocks[2 * blk], bj = blocks[2 * blk + 1];
+  const int il = idx >> 5, jl = idx & 31;
+  const int i = 32 * bi + il, j = 32 * bj + jl;
+  if (i >= nbf || j >= nbf || (bi == bj && il > jl)) return;
+  double s = 0.0;
+  const long stride = (long)nblk * nout * 1024;
+  const double* src = part + ((long)blk * nout + o) * 1024 + idx;
+  for (int sp = 0; sp < nsplit; ++sp) s += src[sp * stride];
+  double* out = (o == 0) ? out0 : out1;
+  if (!out) return;
+  out[i + (long)j * nbf] = s;
+  out[j + (long)i * nbf] = s;
+}
+
+// One wave: d[v*64 + lane] = register v of lane after D = mfma(a[lane], b[lane], 0).
+__global__ void k_mfma_raw(const double* a, const double* b, double* d) {
+  const int lane = threadIdx.x;
+  dbl4 r = mfma16x16x4(a[lane], b[lane], dbl4{0.0, 0.0, 0.0, 0.0});
+  for (int v = 0; v < 4; ++v) d[v * 64 + lane] = r[v];
 }
 
 // Probe of the 16x16x4 FP64 MFMA lane layout assumed above:
-// A[i][k] on lane i + 16k, B[k][j] on lane j + 16k, D[4g+v][c] in register v of lane c + 16g.
+// A[i][k] on lane i + 16k, B[k][j] on lane j + 16k, D[4v+g][c] in register v of lane c + 16g.
 __global__ void k_mfma_probe(double* out) {
   const int lane = threadIdx.x;
   const int c = lane & 15, g = lane >> 4;
@@ -764,7 +766,7 @@ struct dg_handle {
   double *uJ = nullptr, *V = nullptr, *vcell = nullptr, *vpart = nullptr;
   double *q = nullptr, *exc_part = nullptr, *scalars = nullptr;
   double *ylm = nullptr, *ylmT = nullptr, *minvT = nullptr, *shc = nullptr;
-  double *knots = nullptr, *invh = nullptr, *sa = nullptr, *sden = nullptr, *scmod = nullptr;
+  double *knots = nullptr, *invh = nullptr, *KD = nullptr;
   double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
   int* blocks = nullptr;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
@@ -780,8 +782,7 @@ struct dg_handle {
     double* bufs[] = {atoms, invR,  rad,   wrad,   leb,   px,     py,    pz,   w,     wb,
                       phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
                       V,     vcell, vpart, q,      exc_part, scalars, ylm, ylmT, minvT, shc,
-                      knots, invh,  sa,    sden,   scmod, rho_lm, U,     tab,  cpart, J,
-                      XC};
+                      knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks};
@@ -872,14 +873,14 @@ int stage_density(dg_handle* h, const double* dP) {
 
 int stage_correct(dg_handle* h) {
   DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
   }));
   DG_TRY(run(h, "scale", [&] {
     hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
                        h->npts_pad, h->q, h->natoms, h->zsum, h->scalars);
   }));
   DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
   }));
   return DG_OK;
 }
@@ -922,21 +923,17 @@ void launch_cross(dg_handle* h, CrossArgs& a, dim3 grid) {
 
 int stage_hartree(dg_handle* h) {
   DG_TRY(run(h, "rho_lm", [&] {
-    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * h->nrad), dim3(round_up(h->nlm, 64)), 0, h->stream,
-                       h->rho, h->wb, h->leb, h->ylm, h->nang, h->nlm, h->rho_lm);
+    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * h->nrad), dim3(256), (h->nang + 4 * 192) * sizeof(double),
+                       h->stream, h->rho, h->wb, h->leb, h->ylm, h->nang, h->nlm, h->rho_lm);
   }));
   DG_TRY(run(h, "poisson", [&] {
     hipLaunchKernelGGL(k_poisson, dim3(h->natoms * h->nlm), dim3(round_up(h->nrad + 2, 64)),
-                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->q, h->rad, h->minvT,
-                       h->nrad, h->nlm, h->U);
-  }));
-  DG_TRY(run(h, "spline", [&] {
-    hipLaunchKernelGGL(k_spline, dim3(nblocks((long)h->natoms * h->nlm, 64)), dim3(64), 0, h->stream,
-                       h->U, h->knots, h->sa, h->sden, h->scmod, h->nrad, h->nlm, h->natoms, h->tab);
+                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->q, h->rad, h->minvT, h->KD,
+                       h->nrad, h->nlm, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    hipLaunchKernelGGL(k_vcell, dim3(nblocks(h->npts, 256)), dim3(256), 0, h->stream, h->U, h->ylmT,
-                       h->atoms, h->px, h->py, h->pz, h->npts, h->nrad, h->nang, h->nlm, h->vcell);
+    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(256), h->nlm * sizeof(double), h->stream, h->U,
+                       h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
   }));
   if (h->natoms > 1) {
     CrossArgs a;
@@ -989,8 +986,8 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
                          h->chunk_k, h->npts_pad, h->nblk, h->cpart);
   }));
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->nbf * h->nbf, 256)), dim3(256), 0, h->stream,
-                       h->cpart, h->nsplit, h->nblk, nout, h->nb32, h->nbf, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->nblk * nout * 1024, 256)), dim3(256), 0,
+                       h->stream, h->cpart, h->nsplit, h->nblk, nout, h->blocks, h->nbf, out0, out1);
   });
 }
 
@@ -1123,11 +1120,22 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     for (int k = 0; k < n2; ++k)
       for (int i = 0; i < h->nrad; ++i) minvT[((size_t)l * n2 + k) * h->nrad + i] = M[(size_t)(i + 1) * n2 + k];
   }
-  // spline system over ascending knots
+  // spline over the ascending knots: slopes D = S y, then KD_l = S KU_l (knots reversed)
   std::vector<double> knots(h->nrad), invh(h->nrad, 0.0);
   for (int t = 0; t < h->nrad; ++t) knots[t] = rad[h->nrad - 1 - t];
   for (int t = 1; t < h->nrad; ++t) invh[t] = 1.0 / (knots[t] - knots[t - 1]);
-  dg::SplineSystem sys = dg::spline_system(knots);
+  const std::vector<double> Sspl = dg::spline_slope_matrix(knots);  // [t][t'] row-major
+  std::vector<double> KD(minvT.size(), 0.0);
+  for (int l = 0; l <= h->lmax; ++l)
+    for (int k = 0; k < n2; ++k) {
+      const double* ku = &minvT[((size_t)l * n2 + k) * h->nrad];  // ku[i], i = radial node
+      double* kd = &KD[((size_t)l * n2 + k) * h->nrad];
+      for (int t = 0; t < h->nrad; ++t) {
+        double acc = 0.0;
+        for (int tp = 0; tp < h->nrad; ++tp) acc += Sspl[(size_t)t * h->nrad + tp] * ku[h->nrad - 1 - tp];
+        kd[t] = acc;
+      }
+    }
   // spherical-harmonic recurrence constants per (l, m >= 0): {N_lm, (2l-1)/(l-m), (l+m-1)/(l-m)}
   std::vector<double> shc(3 * (kMaxL + 1) * (kMaxL + 2) / 2, 0.0);
   for (int l = 0; l <= kMaxL; ++l)
@@ -1155,9 +1163,9 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     }
   h->nblk = (int)blocks.size() / 2;
   {
-    const long target_wg = 2048;
+    const long target_wg = 1024;
     long ns = std::max(1L, target_wg / h->nblk);
-    ns = std::min(ns, h->npts_pad / 64);
+    ns = std::min(std::min(ns, 256L), h->npts_pad / 64);
     h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
     h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
   }
@@ -1215,9 +1223,7 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->shc, shc.size()));
   DG_TRY(dalloc(&h->knots, h->nrad));
   DG_TRY(dalloc(&h->invh, h->nrad));
-  DG_TRY(dalloc(&h->sa, h->nrad));
-  DG_TRY(dalloc(&h->sden, h->nrad));
-  DG_TRY(dalloc(&h->scmod, h->nrad));
+  DG_TRY(dalloc(&h->KD, KD.size()));
   DG_TRY(dalloc(&h->rho_lm, (size_t)h->natoms * h->nrad * h->nlm));
   DG_TRY(dalloc(&h->U, (size_t)h->natoms * h->nrad * h->nlm));
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
@@ -1249,9 +1255,7 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   HIP_TRY(up(h->shc, shc.data(), shc.size() * 8));
   HIP_TRY(up(h->knots, knots.data(), knots.size() * 8));
   HIP_TRY(up(h->invh, invh.data(), invh.size() * 8));
-  HIP_TRY(up(h->sa, sys.a.data(), sys.a.size() * 8));
-  HIP_TRY(up(h->sden, sys.den.data(), sys.den.size() * 8));
-  HIP_TRY(up(h->scmod, sys.cmod.data(), sys.cmod.size() * 8));
+  HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)h->natoms * np * 8, h->stream));
@@ -1346,7 +1350,7 @@ int dg_calculate_density(dg_handle* h, double* nelec) {
   if (!h || !nelec) return fail(DG_ERR_INVALID, "dg_calculate_density: null argument");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_calculate_density: no density set");
   DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
   }));
   std::vector<double> q(h->natoms);
   DG_TRY(copy_out(h, q.data(), h->q, h->natoms));
@@ -1360,7 +1364,7 @@ int dg_hartree(dg_handle* h, double* J) {
   if (!h) return fail(DG_ERR_INVALID, "dg_hartree: null handle");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_hartree: no density set");
   DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(256), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
   }));
   DG_TRY(stage_hartree(h));
   DG_TRY(stage_contract(h, h->uJ, nullptr, h->J, nullptr));
@@ -1501,6 +1505,22 @@ int dg_profile_reset(dg_handle* h) {
   return DG_OK;
 }
 
+int dg_mfma_probe(const double* a, const double* b, double* d) {
+  if (!a || !b || !d) return fail(DG_ERR_INVALID, "dg_mfma_probe: null argument");
+  double* buf = nullptr;
+  DG_TRY(dalloc(&buf, 64 + 64 + 256));
+  hipError_t e = hipMemcpy(buf, a, 64 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(buf + 64, b, 64 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_raw, dim3(1), dim3(64), 0, 0, buf, buf + 64, buf + 128);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(d, buf + 128, 256 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_mfma_probe: ") + hipGetErrorString(e));
+  return DG_OK;
+}
+
 int dg_selftest_mfma(int* ok) {
   if (!ok) return fail(DG_ERR_INVALID, "dg_selftest_mfma: null argument");
   *ok = 0;
@@ -1514,7 +1534,7 @@ int dg_selftest_mfma(int* ok) {
   int good = 1;
   for (int lane = 0; lane < 64; ++lane)
     for (int v = 0; v < 4; ++v) {
-      const int i = 4 * (lane >> 4) + v, j = lane & 15;
+      const int i = 4 * v + (lane >> 4), j = lane & 15;
       double ref = 0.0;
       for (int k = 0; k < 4; ++k) ref += (1.0 + i + 100.0 * k) * (1.0 + 0.5 * j - 3.0 * k);
       if (std::fabs(out[v * 64 + lane] - ref) > 1e-9 * std::fabs(ref) + 1e-12) good = 0;

@@ -221,4 +221,31 @@ SplineSystem spline_system(const std::vector<double>& x) {
   return s;
 }
 
+std::vector<double> spline_slope_matrix(const std::vector<double>& x) {
+  const size_t n = x.size();
+  const SplineSystem sys = spline_system(x);
+  std::vector<double> S(n * n, 0.0), Y(n), yp(n), D(n);
+  for (size_t c = 0; c < n; ++c) {
+    // right-hand side of the slope system for y = e_c
+    auto y = [&](size_t i) { return i == c ? 1.0 : 0.0; };
+    double h0 = x[1] - x[0], h1 = x[2] - x[1];
+    double r0 = (y(1) - y(0)) / h0, r1 = (y(2) - y(1)) / h1;
+    Y[0] = r0 * (3 * h0 * h1 + 2 * h1 * h1) + r1 * h0 * h0;
+    for (size_t i = 1; i + 1 < n; ++i) {
+      h0 = x[i] - x[i - 1];
+      h1 = x[i + 1] - x[i];
+      r0 = (y(i) - y(i - 1)) / h0;
+      r1 = (y(i + 1) - y(i)) / h1;
+      Y[i] = 3 * (r0 * h1 + r1 * h0);
+    }
+    Y[n - 1] = r0 * h1 * h1 + r1 * (3 * h0 * h1 + 2 * h0 * h0);
+    yp[0] = Y[0] / sys.den[0];
+    for (size_t i = 1; i < n; ++i) yp[i] = (Y[i] - sys.a[i] * yp[i - 1]) / sys.den[i];
+    D[n - 1] = yp[n - 1];
+    for (size_t i = n - 1; i > 0; --i) D[i - 1] = yp[i - 1] - sys.cmod[i - 1] * D[i];
+    for (size_t r = 0; r < n; ++r) S[r * n + c] = D[r];
+  }
+  return S;
+}
+
 }  // namespace dg

@@ -50,6 +50,11 @@ struct SplineSystem {
 };
 SplineSystem spline_system(const std::vector<double>& x);
 
+// Matrix S (n x n, row-major) with D = S y: the not-a-knot slopes at the knots
+// x of the spline through the values y (src/cspline.cpp:66-127 applied to the
+// unit vectors; the slope system is linear in y).
+std::vector<double> spline_slope_matrix(const std::vector<double>& x);
+
 }  // namespace dg
 
 #endif

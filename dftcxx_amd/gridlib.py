@@ -30,7 +30,7 @@ EXPORTS = (
     "dg_calculate_density", "dg_hartree", "dg_xc", "dg_get_weights", "dg_get_becke_weights",
     "dg_get_positions", "dg_get_densities", "dg_get_amplitudes", "dg_get_hartree_potential",
     "dg_get_hartree_parts", "dg_get_xc_parts", "dg_build_device", "dg_profile_enable",
-    "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_scf_run", "dg_one_electron",
+    "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_scf_run", "dg_one_electron",
     "dg_eigh",
 )
 
@@ -75,6 +75,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m dftcxx_amd.build`")
+        # PyTorch ships its own libamdhip64.so.7 with the same soname as ROCm's; the
+        # first one loaded serves the whole process.  Load torch's first when torch
+        # is installed, so that torch can still initialise its device afterwards.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.dg_last_error.restype = C.c_char_p
         L.dg_create.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(_GridParams),
@@ -98,6 +105,7 @@ def lib():
         L.dg_profile_read.argtypes = [C.c_void_p, C.c_int, C.c_char_p, _D, _I, _I]
         L.dg_profile_reset.argtypes = [C.c_void_p]
         L.dg_selftest_mfma.argtypes = [_I]
+        L.dg_mfma_probe.argtypes = [_D, _D, _D]
         L.dg_scf_run.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(_GridParams),
                                  C.POINTER(_ScfParams), C.POINTER(_ScfResult), _D, _D]
         L.dg_one_electron.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), _D, _D, _D]
@@ -143,6 +151,14 @@ def selftest_mfma():
     ok = C.c_int(0)
     _check(lib().dg_selftest_mfma(C.byref(ok)))
     return bool(ok.value)
+
+
+def mfma_probe(a, b):
+    """Register file (4 x 64) of one FP64 16x16x4 MFMA with lane inputs a, b."""
+    a, b = _f64(a), _f64(b)
+    d = np.empty(256)
+    _check(lib().dg_mfma_probe(_d(a), _d(b), _d(d)))
+    return d.reshape(4, 64)
 
 
 class MolecularGrid:

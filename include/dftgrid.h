@@ -132,6 +132,9 @@ int dg_profile_reset(dg_handle* h);
 /* Checks the lane layout of the FP64 16x16x4 matrix-core instruction the
  * contraction kernels rely on; *ok = 1 if it matches. */
 int dg_selftest_mfma(int* ok);
+/* Runs the FP64 16x16x4 MFMA once on one wave: lane i supplies a[i] and b[i];
+ * d[v*64 + i] receives result register v of lane i (diagnostics). */
+int dg_mfma_probe(const double* a, const double* b, double* d);
 
 /* --- SCF driver (host C++, src/dft.cpp:84-142, 160-227, 300-366, 441-447) --- */
 typedef struct {

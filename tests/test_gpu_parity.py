@@ -55,8 +55,8 @@ def test_grid_points_and_weights(pair):
     xyz, w, wb = pair.o.points()
     assert pair.g.npts == pair.o.npts
     assert np.abs(pair.g.get_positions() - xyz).max() < 1e-12
-    assert rel_err(pair.g.get_weights(), w) < 1e-13
-    assert np.abs(pair.g.get_becke_weights() - wb).max() < 1e-13
+    assert rel_err(pair.g.get_weights(), w) < 1e-12
+    assert np.abs(pair.g.get_becke_weights() - wb).max() < 1e-12
 
 
 def test_amplitudes(pair):
