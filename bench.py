@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--workload", default="benzene_631g")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -72,6 +74,17 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        # 256 MiB device-to-device copies: the HBM counters of this known traffic
+        # (256 MiB read + 256 MiB written per copy) scale the kernel counters
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 5}))
+        return
 
     from dftcxx_amd import gridlib as G
     from dftcxx_amd import molecule as M
