@@ -180,31 +180,34 @@ __global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* _
 // Density: rho_p = 2 phi_p^T P phi_p  (src/gridpoint.cpp:82-84)
 // ---------------------------------------------------------------------------
 
-// P rearranged into the A-operand order of the 16x16x4 FP64 matrix-core op:
-// frag[b][s][lane] = P(16b + lane%16, kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4.
-// The same permutation of the reduction index is used for phi in k_density, so
-// the result tile lands on the lanes that hold the matching phi values.
+// P in the A-operand order of the 16x16x4 FP64 matrix-core op, halved by
+// symmetry: rho/2 = sum_b phi_b^T T_b with T_b = sum_{b' <= b} P'_{bb'} phi_b',
+// P'_{bb'} = 2 P_{bb'} for b' < b and P_{bb} on the diagonal tile (16-wide tiles).
+// Tile b holds 4(b+1) reduction steps s; entry [s][lane] is
+// P'(16b + lane%16, kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4, stored from
+// offset 128 b (b+1).  The same permutation of the reduction index is used
+// for phi in k_density, so each result lands on the lanes that hold phi.
 __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* frag) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long per = (long)4 * nt * 64;
-  if (e >= per * nt) return;
-  const int b = (int)(e / per);
-  const int rem = (int)(e % per);
+  if (e >= 128L * nt * (nt + 1)) return;
+  int b = 0;
+  while (128L * (b + 1) * (b + 2) <= e) ++b;
+  const int rem = (int)(e - 128L * b * (b + 1));
   const int s = rem / 64, l = rem % 64;
   const int j = 16 * b + (l & 15);
   const int k = 16 * (s / 4) + 4 * (l >> 4) + (s & 3);
-  frag[e] = (j < nbf && k < nbf) ? P[j + (long)k * nbf] : 0.0;
+  const double f = (s / 4 < b) ? 2.0 : 1.0;
+  frag[e] = (j < nbf && k < nbf) ? f * P[j + (long)k * nbf] : 0.0;
 }
 
 // One wave handles 16 points; lane (c = lane%16, g = lane/16) keeps
 // phi[p0+c][kk(s,g)] for every reduction step s in registers (B operand).
-// For output tile b the wave forms T[p][16b+4v+g] = (P phi_p)_j (result
-// register v) with 4*NT MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one tile are
-// staged in LDS and shared by the block's 4 waves.
+// For output tile b the wave forms T_b[p][16b+4v+g] (result register v) with
+// 4(b+1) MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one
+// tile are staged in LDS and shared by the block's 4 waves.  NT = ceil(nbf/16).
 template <int NT>
 __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
-                                                 const double* __restrict__ frag, int nt_used,
-                                                 double* rho) {
+                                                 const double* __restrict__ frag, double* rho) {
   __shared__ double pl[4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
@@ -214,15 +217,17 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
 #pragma unroll
   for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
   double acc = 0.0;
-  for (int b = 0; b < nt_used; ++b) {
+  for (int b = 0; b < NT; ++b) {
+    const int ns = 4 * (b + 1);
     __syncthreads();
-    const double* src = frag + (long)b * 4 * NT * 64;
-    for (int e = threadIdx.x; e < 4 * NT * 64; e += 256) pl[e] = src[e];
+    const double* src = frag + 128L * b * (b + 1);
+    for (int e = threadIdx.x; e < ns * 64; e += 256) pl[e] = src[e];
     __syncthreads();
     dbl4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < 4 * NT; ++s) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
-    // register v of lane (c, g) holds T[p0+c][16b + 4v + g]
+    for (int s = 0; s < 4 * NT; ++s)
+      if (s < ns) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
+    // register v of lane (c, g) holds T_b[p0+c][16b + 4v + g]
     const double* rb = row + 16 * b + g;
     acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
   }
@@ -248,15 +253,19 @@ __global__ __launch_bounds__(1024) void k_cell_sum(const double* __restrict__ x,
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// rho *= sum(Z) / sum_a q_a  (src/moleculargrid.cpp:132-146).  scalars[1] = sum q.
+// rho *= sum(Z) / sum_a q_a  (src/moleculargrid.cpp:132-146); scalars[1] = sum q
+// before; block 0 writes the populations of the corrected density, qc_a = f q_a.
 __global__ void k_scale(double* rho, long n, const double* __restrict__ q, int natoms, double zsum,
-                        double* scalars) {
+                        double* qc, double* scalars) {
   double qs = 0.0;
   for (int a = 0; a < natoms; ++a) qs += q[a];
   const double f = zsum / qs;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) rho[p] *= f;
-  if (p == 0 && scalars) scalars[1] = qs;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0 && scalars) scalars[1] = qs;
+    for (int a = threadIdx.x; a < natoms; a += blockDim.x) qc[a] = f * q[a];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -345,35 +354,53 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
 // Hartree potential (src/atomicgrid.cpp:241-466, src/moleculargrid.cpp:336-389)
 // ---------------------------------------------------------------------------
 
-// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p,  one block per (a, i):
-// the shell's rho w_j wb_p go to LDS, wave w sums the points j = w (mod 4) for
-// lm = lane, lane+64, lane+128, and the four partial sums are added in order.
+// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p.  One block per
+// (a, 4 consecutive radial shells): the shells' rho w_j wb_p go to LDS, wave w
+// sums the points j = w (mod 4) for lm = lane, lane+64, lane+128 of all four
+// shells (each Y row is read once for 4 shells), the four wave sums are added in order.
+constexpr int kShells = 4;
 __global__ __launch_bounds__(256) void k_rho_lm(const double* __restrict__ rho,
                                                 const double* __restrict__ wb,
                                                 const double* __restrict__ leb,
-                                                const double* __restrict__ ylm, int nang, int nlm,
-                                                double* rho_lm) {
+                                                const double* __restrict__ ylm, int nrad, int nang,
+                                                int nlm, double* rho_lm) {
   extern __shared__ double sh[];
-  double* f = sh;             // nang
-  double* part = sh + nang;   // 4 x 192
-  const long ai = blockIdx.x;  // a*nrad + i
-  const long base = ai * nang;
-  for (int j = threadIdx.x; j < nang; j += 256) f[j] = rho[base + j] * leb[4 * j + 3] * wb[base + j];
+  double* f = sh;                      // kShells x nang
+  double* part = sh + kShells * nang;  // 4 waves x kShells x 192
+  const int groups = (nrad + kShells - 1) / kShells;
+  const int a = blockIdx.x / groups, i0 = (blockIdx.x % groups) * kShells;
+  const int ns = min(kShells, nrad - i0);
+  const long base = ((long)a * nrad + i0) * nang;
+  for (int e = threadIdx.x; e < ns * nang; e += 256) {
+    const int j = e % nang;
+    f[e] = rho[base + e] * leb[4 * j + 3] * wb[base + e];
+  }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[kShells][3] = {};
   for (int j = wave; j < nang; j += 4) {
-    const double fj = f[j];
     const double* y = ylm + (long)j * nlm;
+    double yv[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (lane + 64 * q < nlm) acc[q] += fj * y[lane + 64 * q];
+    for (int q = 0; q < 3; ++q) yv[q] = (lane + 64 * q < nlm) ? y[lane + 64 * q] : 0.0;
+#pragma unroll
+    for (int sh_ = 0; sh_ < kShells; ++sh_) {
+      const double fj = (sh_ < ns) ? f[sh_ * nang + j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[sh_][q] += fj * yv[q];
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 3; ++q) part[wave * 192 + lane + 64 * q] = acc[q];
+  for (int sh_ = 0; sh_ < kShells; ++sh_)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) part[(wave * kShells + sh_) * 192 + lane + 64 * q] = acc[sh_][q];
   __syncthreads();
-  for (int lm = threadIdx.x; lm < nlm; lm += 256)
-    rho_lm[ai * nlm + lm] = (part[lm] + part[192 + lm] + part[384 + lm] + part[576 + lm]) * (4.0 * kPiD);
+  for (int e = threadIdx.x; e < ns * nlm; e += 256) {
+    const int sh_ = e / nlm, lm = e % nlm;
+    const double v = part[(0 * kShells + sh_) * 192 + lm] + part[(1 * kShells + sh_) * 192 + lm] +
+                     part[(2 * kShells + sh_) * 192 + lm] + part[(3 * kShells + sh_) * 192 + lm];
+    rho_lm[(((long)a * nrad + i0 + sh_) * nlm) + lm] = v * (4.0 * kPiD);
+  }
 }
 
 // Radial Poisson solve and spline slopes for one (a, lm) per block
@@ -418,7 +445,9 @@ __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __res
 }
 
 // Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
-// (src/atomicgrid.cpp:434-462), one block per (a, i); ylmT is [lm][j].
+// (src/atomicgrid.cpp:434-462).  One block per (a, 4 radial shells); the
+// shells' U rows sit in LDS and each Y_lm(Omega_j) (ylmT is [lm][j]) is read
+// once for the four shells.
 __global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U,
                                                const double* __restrict__ ylmT,
                                                const double* __restrict__ atoms,
@@ -426,19 +455,26 @@ __global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U,
                                                const double* __restrict__ py,
                                                const double* __restrict__ pz, int nrad, int nang,
                                                int nlm, double* vcell) {
-  extern __shared__ double u[];
-  const long ai = blockIdx.x;
-  const int a = (int)(ai / nrad);
-  for (int lm = threadIdx.x; lm < nlm; lm += 256) u[lm] = U[ai * nlm + lm];
+  extern __shared__ double u[];  // kShells x nlm
+  const int groups = (nrad + kShells - 1) / kShells;
+  const int a = blockIdx.x / groups, i0 = (blockIdx.x % groups) * kShells;
+  const int ns = min(kShells, nrad - i0);
+  const long ai0 = (long)a * nrad + i0;
+  for (int e = threadIdx.x; e < ns * nlm; e += 256) u[e] = U[ai0 * nlm + e];
   __syncthreads();
   const double Ax = atoms[4 * a], Ay = atoms[4 * a + 1], Az = atoms[4 * a + 2];
   for (int j = threadIdx.x; j < nang; j += 256) {
-    const long p = ai * nang + j;
-    const double dx = px[p] - Ax, dy = py[p] - Ay, dz = pz[p] - Az;
-    const double rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    double acc = 0.0;
-    for (int lm = 0; lm < nlm; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
-    vcell[p] = rinv * acc;
+    double acc[kShells] = {};
+    for (int lm = 0; lm < nlm; ++lm) {
+      const double y = ylmT[(long)lm * nang + j];
+#pragma unroll
+      for (int sh_ = 0; sh_ < kShells; ++sh_) acc[sh_] += y * u[(sh_ < ns ? sh_ : 0) * nlm + lm];
+    }
+    for (int sh_ = 0; sh_ < ns; ++sh_) {
+      const long p = (ai0 + sh_) * nang + j;
+      const double dx = px[p] - Ax, dy = py[p] - Ay, dz = pz[p] - Az;
+      vcell[p] = acc[sh_] / sqrt(dx * dx + dy * dy + dz * dz);
+    }
   }
 }
 
@@ -451,15 +487,17 @@ struct CrossArgs {
   const double* invh;   // 1/(x_t - x_{t-1})
   const double* tab;    // [natoms][nrad][nlm]{y, D}
   const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
+  const int* items;     // {k, p_begin, p_end} point ranges outside cell k, k ascending
   double* vpart;        // [natoms][npts_pad]
-  long npts, npts_pad, cell, chunk;
-  int nrad, nlm, l0, l1, accumulate;
+  long npts_pad;
+  int nitems, nrad, nlm, l0, l1, accumulate;
 };
 
 // Potential of atom k's spherical-harmonic expansion at every point outside
 // cell k: V_k(p) = sum_lm Y_lm(r_p - R_k) S_{k,lm}(|r_p - R_k|) / |r_p - R_k|
-// (src/moleculargrid.cpp:349-376).  Grid: (point chunk, k).  Atom k's spline
-// table for l in [l0, l1) sits in LDS; each thread walks its points, runs the
+// (src/moleculargrid.cpp:349-376).  Persistent: work-group g walks the items
+// [g n/G, (g+1) n/G) of the (atom, point range) list, reloading atom k's spline
+// table for l in [l0, l1) into LDS when k changes.  Each thread runs the
 // associated-Legendre / Chebyshev recurrences once per (p, k) and forms the
 // four sums sum Y y_{s-1}, sum Y D_{s-1}, sum Y y_s, sum Y D_s, from which the
 // cubic on interval s is evaluated once (linear in the per-lm coefficients).
@@ -468,7 +506,6 @@ struct CrossArgs {
 template <int LMAX>
 __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   extern __shared__ double lds[];
-  const int k = blockIdx.y;
   const int n = A.nrad;
   const int lmb = A.l0 * A.l0, nlmc = A.l1 * A.l1 - lmb;
   double* kn = lds;
@@ -478,19 +515,25 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
     kn[i] = A.knots[i];
     ih[i] = A.invh[i];
   }
-  const double* src = A.tab + (long)k * n * A.nlm * 2;
-  for (int i = threadIdx.x; i < n * nlmc * 2; i += blockDim.x) {
-    const int t = i / (nlmc * 2), rem = i % (nlmc * 2);
-    tab[i] = src[(long)t * A.nlm * 2 + lmb * 2 + rem];
+  const int it0 = (int)((long)blockIdx.x * A.nitems / gridDim.x);
+  const int it1 = (int)((long)(blockIdx.x + 1) * A.nitems / gridDim.x);
+  int cur = -1;
+  for (int it = it0; it < it1; ++it) {
+  const int k = A.items[3 * it];
+  if (k != cur) {
+    __syncthreads();
+    const double* src = A.tab + (long)k * n * A.nlm * 2;
+    for (int i = threadIdx.x; i < n * nlmc * 2; i += blockDim.x) {
+      const int t = i / (nlmc * 2), rem = i % (nlmc * 2);
+      tab[i] = src[(long)t * A.nlm * 2 + lmb * 2 + rem];
+    }
+    __syncthreads();
+    cur = k;
   }
-  __syncthreads();
+  const long pb = A.items[3 * it + 1], pe = A.items[3 * it + 2];
   const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
-  const long pb = (long)blockIdx.x * A.chunk;
-  const long pe = min(pb + A.chunk, A.npts);
-  const long clo = (long)k * A.cell, chi = clo + A.cell;
   const double xlo = kn[0], xhi = kn[n - 1];
   for (long p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-    if (p >= clo && p < chi) continue;
     const double dx = A.px[p] - Ax, dy = A.py[p] - Ay, dz = A.pz[p] - Az;
     const double rxy2 = dx * dx + dy * dy;
     const double r = sqrt(rxy2 + dz * dz);
@@ -584,6 +627,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
     else
       *out = val * rinv;
   }
+  }
 }
 
 // V[p] = sum_k (k == own ? vcell[p] : vpart[k][p]), k ascending
@@ -608,89 +652,98 @@ __global__ void k_vreduce(const double* __restrict__ vpart, const double* __rest
 // ---------------------------------------------------------------------------
 // Contractions  C_o = Phi^T diag(u_o) Phi  (src/dft.cpp:424-432, src/atomicgrid.cpp:471-488)
 // ---------------------------------------------------------------------------
-// Block (bi <= bj) of 32x32 outputs per workgroup, split-K over points.  Each
-// wave accumulates 2x2 tiles of 16x16 with the FP64 16x16x4 MFMA: lane
-// (c, g) feeds A = phi[p_g][i0 + c] and B = u[p_g] phi[p_g][j0 + c] and
-// receives C[i0 + 4v + g][j0 + c] in result register v; waves
-// interleave 4-point steps; the 4 waves are summed in LDS in a fixed order.
-// part[split][blk][o][32][32].
+// Output tiles of 16x16 on and above the diagonal, gathered in groups of up to
+// four tiles of one tile row (I; J0..J0+nJ-1) that share the A operand.  A
+// work-group takes one group and a range of points (split-K); its 4 waves
+// interleave 4-point steps.  Lane (c, g) feeds A = phi[p_g][16I + c] and
+// B = u[p_g] phi[p_g][16J + c] and receives C[16I + 4v + g][16J + c] in result
+// register v.  The next step's operands are loaded before the current MFMAs.
+// The 4 waves are summed in LDS in a fixed order; part[split][grp][o][t][v][lane].
 template <int NOUT>
 __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi, int ldphi,
                                                   const double* __restrict__ u0,
                                                   const double* __restrict__ u1,
-                                                  const int* __restrict__ blocks, long chunk,
-                                                  long npts_pad, int nblk, double* part) {
-  __shared__ double red[4 * 16 * 64];
-  const int blk = blockIdx.x, split = blockIdx.y;
-  const int bi = blocks[2 * blk], bj = blocks[2 * blk + 1];
+                                                  const int* __restrict__ groups, long chunk,
+                                                  long npts_pad, int ngroups, double* part) {
+  __shared__ double red[4 * 4 * 4 * 64];
+  const int grp = blockIdx.x, split = blockIdx.y;
+  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long pb = (long)split * chunk;
   const long pe = min(pb + chunk, npts_pad);  // chunk and npts_pad are multiples of 16
-  const int i0 = 32 * bi + c, j0 = 32 * bj + c;
-  dbl4 acc[NOUT][2][2];
+  const int ia = 16 * I + c, jb = 16 * J0 + c;
+  dbl4 acc[NOUT][4];
 #pragma unroll
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[o][a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-  for (long p = pb + 4 * wave + g; p < pe; p += 16) {
+    for (int t = 0; t < 4; ++t) acc[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  long p = pb + 4 * wave + g;
+  double a = 0.0, bb[4] = {0.0, 0.0, 0.0, 0.0}, w0 = 0.0, w1 = 0.0;
+  if (p < pe) {
     const double* row = phi + p * ldphi;
-    const double a0 = row[i0], a1 = row[i0 + 16];
-    const double b0 = row[j0], b1 = row[j0 + 16];
-    const double w0 = u0[p];
-    acc[0][0][0] = mfma16x16x4(a0, w0 * b0, acc[0][0][0]);
-    acc[0][0][1] = mfma16x16x4(a0, w0 * b1, acc[0][0][1]);
-    acc[0][1][0] = mfma16x16x4(a1, w0 * b0, acc[0][1][0]);
-    acc[0][1][1] = mfma16x16x4(a1, w0 * b1, acc[0][1][1]);
-    if (NOUT == 2) {
-      const double w1 = u1[p];
-      acc[NOUT - 1][0][0] = mfma16x16x4(a0, w1 * b0, acc[NOUT - 1][0][0]);
-      acc[NOUT - 1][0][1] = mfma16x16x4(a0, w1 * b1, acc[NOUT - 1][0][1]);
-      acc[NOUT - 1][1][0] = mfma16x16x4(a1, w1 * b0, acc[NOUT - 1][1][0]);
-      acc[NOUT - 1][1][1] = mfma16x16x4(a1, w1 * b1, acc[NOUT - 1][1][1]);
+    a = row[ia];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.0;
+    w0 = u0[p];
+    if (NOUT == 2) w1 = u1[p];
+  }
+  for (; p < pe; p += 16) {
+    const double ca = a, cw0 = w0, cw1 = w1;
+    double cb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) cb[t] = bb[t];
+    const long pn = p + 16;
+    if (pn < pe) {
+      const double* row = phi + pn * ldphi;
+      a = row[ia];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.0;
+      w0 = u0[pn];
+      if (NOUT == 2) w1 = u1[pn];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < nJ) {
+        acc[0][t] = mfma16x16x4(ca, cw0 * cb[t], acc[0][t]);
+        if (NOUT == 2) acc[NOUT - 1][t] = mfma16x16x4(ca, cw1 * cb[t], acc[NOUT - 1][t]);
+      }
     }
   }
 #pragma unroll
   for (int o = 0; o < NOUT; ++o) {
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) red[(wave * 16 + (a * 2 + b) * 4 + v) * 64 + lane] = acc[o][a][b][v];
+      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + v) * 64 + lane] = acc[o][t][v];
     __syncthreads();
-    double* dst = part + (((long)split * nblk + blk) * NOUT + o) * 1024;
-    for (int e = threadIdx.x; e < 1024; e += 256) {
-      const double s = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
-      const int ln = e & 63, v = (e >> 6) & 3, ab = e >> 8;
-      const int ta = ab >> 1, tb = ab & 1;
-      const int il = 16 * ta + 4 * v + (ln >> 4), jl = 16 * tb + (ln & 15);
-      dst[il * 32 + jl] = s;
-    }
+    double* dst = part + (((long)split * ngroups + grp) * NOUT + o) * 1024;
+    for (int e = threadIdx.x; e < nJ * 256; e += 256)
+      dst[e] = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
   }
 }
 
-// out_o = sum over splits of the partial blocks; one thread per block element
-// (coalesced over splits); the upper block triangle is mirrored, so the
-// output is exactly symmetric.
-__global__ void k_contract_reduce(const double* __restrict__ part, int nsplit, int nblk, int nout,
-                                  const int* __restrict__ blocks, int nbf, double* out0,
+// out_o = sum over splits of the partial tiles; one thread per tile element
+// (coalesced over splits).  Tiles on the diagonal contribute their upper
+// triangle only and every element is mirrored, so the output is exactly symmetric.
+__global__ void k_contract_reduce(const double* __restrict__ part, int nsplit, int ngroups, int nout,
+                                  const int* __restrict__ groups, int nbf, double* out0,
                                   double* out1) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)nblk * nout * 1024) return;
+  if (e >= (long)ngroups * nout * 1024) return;
   const int idx = (int)(e & 1023);
   const int o = (int)((e >> 10) % nout);
-  const int blk = (int)((e >> 10) / nout);
-  const int bi = blocks[2 * blk], bj = blocks[2 * blk + 1];
-  const int il = idx >> 5, jl = idx & 31;
-  const int i = 32 * bi + il, j = 32 * bj + jl;
-  if (i >= nbf || j >= nbf || (bi == bj && il > jl)) return;
+  const int grp = (int)((e >> 10) / nout);
+  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
+  const int t = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
+  if (t >= nJ) return;
+  const int il = 4 * v + (ln >> 4), jl = ln & 15;
+  const int i = 16 * I + il, j = 16 * (J0 + t) + jl;
+  if (i >= nbf || j >= nbf || (I == J0 + t && il > jl)) return;
   double s = 0.0;
-  const long stride = (long)nblk * nout * 1024;
-  const double* src = part + ((long)blk * nout + o) * 1024 + idx;
+  const long stride = (long)ngroups * nout * 1024;
+  const double* src = part + ((long)grp * nout + o) * 1024 + idx;
   for (int sp = 0; sp < nsplit; ++sp) s += src[sp * stride];
   double* out = (o == 0) ? out0 : out1;
   if (!out) return;
@@ -750,8 +803,8 @@ struct dg_handle {
   bool own_stream = true;
   int natoms = 0, nbf = 0, nshell = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0, precision = 0;
   long npts = 0, npts_pad = 0, cell = 0;
-  int ldphi = 0, nt16 = 0, nb32 = 0, nblk = 0, nsplit = 0;
-  long chunk_k = 0, cross_chunk = 0, nlda_blocks = 0;
+  int ldphi = 0, ntb = 0, ngroups = 0, nsplit = 0;
+  long chunk_k = 0, nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
   VwnConst vwn{};
   std::vector<std::pair<int, int>> lchunks;  // l ranges of the cross kernel
@@ -764,11 +817,13 @@ struct dg_handle {
   double *pr_alpha = nullptr, *pr_cn0 = nullptr, *pr_cn1 = nullptr;
   double *P = nullptr, *pfrag = nullptr, *rho = nullptr, *uxc = nullptr, *exd = nullptr;
   double *uJ = nullptr, *V = nullptr, *vcell = nullptr, *vpart = nullptr;
-  double *q = nullptr, *exc_part = nullptr, *scalars = nullptr;
+  double *q = nullptr, *qc = nullptr, *exc_part = nullptr, *scalars = nullptr;
   double *ylm = nullptr, *ylmT = nullptr, *minvT = nullptr, *shc = nullptr;
   double *knots = nullptr, *invh = nullptr, *KD = nullptr;
   double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
   int* blocks = nullptr;
+  int* items = nullptr;
+  int nitems = 0, cross_wgs = 0;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
   // state
   bool has_density = false, has_hartree = false, has_xc = false;
@@ -781,11 +836,11 @@ struct dg_handle {
   ~dg_handle() {
     double* bufs[] = {atoms, invR,  rad,   wrad,   leb,   px,     py,    pz,   w,     wb,
                       phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
-                      V,     vcell, vpart, q,      exc_part, scalars, ylm, ylmT, minvT, shc,
+                      V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, ylmT, minvT, shc,
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks};
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items};
     for (int* b : ibufs)
       if (b) (void)hipFree(b);
     for (auto& e : pending) {
@@ -840,18 +895,20 @@ unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 template <int NT>
 void launch_density_nt(dg_handle* h) {
   hipLaunchKernelGGL(k_density<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
-                     h->ldphi, h->pfrag, std::min(NT, (h->nbf + 15) / 16), h->rho);
+                     h->ldphi, h->pfrag, h->rho);
 }
 
 int launch_density(dg_handle* h) {
   return run(h, "density", [&] {
-    switch (h->nt16) {
+    switch (h->ntb) {
 #define DG_NT(N) \
   case N:        \
     launch_density_nt<N>(h); \
     break;
-      DG_NT(2) DG_NT(4) DG_NT(6) DG_NT(8) DG_NT(10) DG_NT(12) DG_NT(14) DG_NT(16) DG_NT(18)
-      DG_NT(20) DG_NT(22) DG_NT(24) DG_NT(26) DG_NT(28) DG_NT(30) DG_NT(32)
+      DG_NT(1) DG_NT(2) DG_NT(3) DG_NT(4) DG_NT(5) DG_NT(6) DG_NT(7) DG_NT(8) DG_NT(9) DG_NT(10) DG_NT(11)
+      DG_NT(12) DG_NT(13) DG_NT(14) DG_NT(15) DG_NT(16) DG_NT(17) DG_NT(18) DG_NT(19) DG_NT(20) DG_NT(21)
+      DG_NT(22) DG_NT(23) DG_NT(24) DG_NT(25) DG_NT(26) DG_NT(27) DG_NT(28) DG_NT(29) DG_NT(30) DG_NT(31)
+      DG_NT(32)
 #undef DG_NT
       default:
         break;
@@ -860,9 +917,9 @@ int launch_density(dg_handle* h) {
 }
 
 int stage_density(dg_handle* h, const double* dP) {
-  const long nfrag = (long)h->nt16 * 4 * h->nt16 * 64;
+  const long nfrag = 128L * h->ntb * (h->ntb + 1);
   DG_TRY(run(h, "pfrag", [&] {
-    hipLaunchKernelGGL(k_pfrag, dim3(nblocks(nfrag, 256)), dim3(256), 0, h->stream, dP, h->nbf, h->nt16,
+    hipLaunchKernelGGL(k_pfrag, dim3(nblocks(nfrag, 256)), dim3(256), 0, h->stream, dP, h->nbf, h->ntb,
                        h->pfrag);
   }));
   DG_TRY(launch_density(h));
@@ -877,10 +934,7 @@ int stage_correct(dg_handle* h) {
   }));
   DG_TRY(run(h, "scale", [&] {
     hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
-                       h->npts_pad, h->q, h->natoms, h->zsum, h->scalars);
-  }));
-  DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
+                       h->npts_pad, h->q, h->natoms, h->zsum, h->qc, h->scalars);
   }));
   return DG_OK;
 }
@@ -923,17 +977,20 @@ void launch_cross(dg_handle* h, CrossArgs& a, dim3 grid) {
 
 int stage_hartree(dg_handle* h) {
   DG_TRY(run(h, "rho_lm", [&] {
-    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * h->nrad), dim3(256), (h->nang + 4 * 192) * sizeof(double),
-                       h->stream, h->rho, h->wb, h->leb, h->ylm, h->nang, h->nlm, h->rho_lm);
+    const int groups = (h->nrad + kShells - 1) / kShells;
+    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * groups), dim3(256),
+                       (kShells * h->nang + 4 * kShells * 192) * sizeof(double), h->stream, h->rho, h->wb, h->leb,
+                       h->ylm, h->nrad, h->nang, h->nlm, h->rho_lm);
   }));
   DG_TRY(run(h, "poisson", [&] {
     hipLaunchKernelGGL(k_poisson, dim3(h->natoms * h->nlm), dim3(round_up(h->nrad + 2, 64)),
-                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->q, h->rad, h->minvT, h->KD,
+                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->qc, h->rad, h->minvT, h->KD,
                        h->nrad, h->nlm, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(256), h->nlm * sizeof(double), h->stream, h->U,
-                       h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
+    const int groups = (h->nrad + kShells - 1) / kShells;
+    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * groups), dim3(256), kShells * h->nlm * sizeof(double), h->stream,
+                       h->U, h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
   }));
   if (h->natoms > 1) {
     CrossArgs a;
@@ -945,14 +1002,13 @@ int stage_hartree(dg_handle* h) {
     a.invh = h->invh;
     a.tab = h->tab;
     a.shc = h->shc;
+    a.items = h->items;
     a.vpart = h->vpart;
-    a.npts = h->npts;
     a.npts_pad = h->npts_pad;
-    a.cell = h->cell;
-    a.chunk = h->cross_chunk;
+    a.nitems = h->nitems;
     a.nrad = h->nrad;
     a.nlm = h->nlm;
-    dim3 grid(nblocks(h->npts, (int)h->cross_chunk), h->natoms);
+    dim3 grid(h->cross_wgs);
     for (size_t ci = 0; ci < h->lchunks.size(); ++ci) {
       a.l0 = h->lchunks[ci].first;
       a.l1 = h->lchunks[ci].second;
@@ -977,23 +1033,23 @@ int stage_lda(dg_handle* h) {
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
   DG_TRY(run(h, "contract", [&] {
-    dim3 grid(h->nblk, h->nsplit);
+    dim3 grid(h->ngroups, h->nsplit);
     if (nout == 2)
       hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
-                         h->chunk_k, h->npts_pad, h->nblk, h->cpart);
+                         h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
     else
       hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
-                         h->chunk_k, h->npts_pad, h->nblk, h->cpart);
+                         h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
   }));
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->nblk * nout * 1024, 256)), dim3(256), 0,
-                       h->stream, h->cpart, h->nsplit, h->nblk, nout, h->blocks, h->nbf, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 256)), dim3(256), 0,
+                       h->stream, h->cpart, h->nsplit, h->ngroups, nout, h->blocks, h->nbf, out0, out1);
   });
 }
 
 int stage_scalars(dg_handle* h) {
   return run(h, "scalars", [&] {
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks, h->q,
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks, h->qc,
                        h->natoms, h->scalars);
   });
 }
@@ -1073,9 +1129,8 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   }
   h->nbf = nbf;
   h->ldphi = (int)round_up(nbf, 32);
-  h->nt16 = h->ldphi / 16;
-  h->nb32 = h->ldphi / 32;
-  if (h->nt16 > 32) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 512 basis functions");
+  h->ntb = (nbf + 15) / 16;
+  if (h->ntb > 32) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 512 basis functions");
   std::vector<double> alpha(nprim_tot), cn0(nprim_tot), cn1(nprim_tot);
   for (int s = 0; s < h->nshell; ++s) {
     const int L = basis->L[s];
@@ -1154,17 +1209,18 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     const double Q = std::sqrt(4.0 * c - b * b), X0 = x0 * x0 + b * x0 + c;
     h->vwn = VwnConst{A, x0, b, c, Q, X0, (2.0 * b / Q) * (1.0 - x0 * (2.0 * x0 + b) / X0), b * x0 / X0};
   }
-  // contraction blocks and split-K
+  // contraction: groups of up to 4 upper tiles of one tile row, split-K over points
   std::vector<int> blocks;
-  for (int bi = 0; bi < h->nb32; ++bi)
-    for (int bj = bi; bj < h->nb32; ++bj) {
-      blocks.push_back(bi);
-      blocks.push_back(bj);
+  for (int I = 0; I < h->ntb; ++I)
+    for (int J0 = I; J0 < h->ntb; J0 += 4) {
+      blocks.push_back(I);
+      blocks.push_back(J0);
+      blocks.push_back(std::min(4, h->ntb - J0));
     }
-  h->nblk = (int)blocks.size() / 2;
+  h->ngroups = (int)blocks.size() / 3;
   {
     const long target_wg = 1024;
-    long ns = std::max(1L, target_wg / h->nblk);
+    long ns = std::max(1L, target_wg / h->ngroups);
     ns = std::min(std::min(ns, 256L), h->npts_pad / 64);
     h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
     h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
@@ -1188,9 +1244,26 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
       l0 = l1;
     }
     h->cross_lds = maxb;
-    // about two work-groups per CU over all (chunk, atom) pairs, >= 4 points per thread
-    const long ch = round_up(std::max(4096L, h->npts * h->natoms / 512), 1024);
-    h->cross_chunk = std::min(ch, round_up(h->npts, 1024));
+  }
+  // cross-potential work list: point ranges of <= 1024 outside each cell
+  std::vector<int> items;
+  for (int k = 0; k < h->natoms && h->natoms > 1; ++k) {
+    auto add = [&](long b, long e) {
+      for (long p = b; p < e; p += 1024) {
+        items.push_back(k);
+        items.push_back((int)p);
+        items.push_back((int)std::min(e, p + 1024));
+      }
+    };
+    add(0, (long)k * h->cell);
+    add((long)(k + 1) * h->cell, h->npts);
+  }
+  h->nitems = (int)items.size() / 3;
+  {
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(h->cross_lds, 1)));
+    h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
   }
 
   // allocations
@@ -1212,9 +1285,10 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->pr_cn0, nprim_tot));
   DG_TRY(dalloc(&h->pr_cn1, nprim_tot));
   DG_TRY(dalloc(&h->P, (size_t)h->nbf * h->nbf));
-  DG_TRY(dalloc(&h->pfrag, (size_t)h->nt16 * 4 * h->nt16 * 64));
+  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1)));
   DG_TRY(dalloc(&h->vpart, (size_t)h->natoms * np));
   DG_TRY(dalloc(&h->q, h->natoms));
+  DG_TRY(dalloc(&h->qc, h->natoms));
   DG_TRY(dalloc(&h->exc_part, h->nlda_blocks));
   DG_TRY(dalloc(&h->scalars, 4));
   DG_TRY(dalloc(&h->ylm, ylm.size()));
@@ -1228,7 +1302,8 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->U, (size_t)h->natoms * h->nrad * h->nlm));
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
   DG_TRY(dalloc(&h->blocks, blocks.size()));
-  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->nblk * 2 * 1024));
+  DG_TRY(dalloc(&h->items, items.size()));
+  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->ngroups * 2 * 1024));
   DG_TRY(dalloc(&h->J, (size_t)h->nbf * h->nbf));
   DG_TRY(dalloc(&h->XC, (size_t)h->nbf * h->nbf));
 
@@ -1257,6 +1332,7 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   HIP_TRY(up(h->invh, invh.data(), invh.size() * 8));
   HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
+  if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)h->natoms * np * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->rho, 0, (size_t)np * 8, h->stream));
@@ -1363,8 +1439,9 @@ int dg_calculate_density(dg_handle* h, double* nelec) {
 int dg_hartree(dg_handle* h, double* J) {
   if (!h) return fail(DG_ERR_INVALID, "dg_hartree: null handle");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_hartree: no density set");
+  // cell populations of the current density (the Poisson boundary values)
   DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->qc);
   }));
   DG_TRY(stage_hartree(h));
   DG_TRY(stage_contract(h, h->uJ, nullptr, h->J, nullptr));
