@@ -445,35 +445,38 @@ __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __res
 }
 
 // Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
-// (src/atomicgrid.cpp:434-462).  One block per (a, 4 radial shells); the
-// shells' U rows sit in LDS and each Y_lm(Omega_j) (ylmT is [lm][j]) is read
-// once for the four shells.
-__global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U,
-                                               const double* __restrict__ ylmT,
-                                               const double* __restrict__ atoms,
-                                               const double* __restrict__ px,
-                                               const double* __restrict__ py,
-                                               const double* __restrict__ pz, int nrad, int nang,
-                                               int nlm, double* vcell) {
-  extern __shared__ double u[];  // kShells x nlm
-  const int groups = (nrad + kShells - 1) / kShells;
-  const int a = blockIdx.x / groups, i0 = (blockIdx.x % groups) * kShells;
-  const int ns = min(kShells, nrad - i0);
-  const long ai0 = (long)a * nrad + i0;
-  for (int e = threadIdx.x; e < ns * nlm; e += 256) u[e] = U[ai0 * nlm + e];
+// (src/atomicgrid.cpp:434-462).  One block per (a, i) with U[a][i][:] in LDS;
+// thread (j, h) sums half h of the lm range (ylmT is [lm][j], coalesced over j)
+// and the two halves are added in order.
+__global__ __launch_bounds__(1024) void k_vcell(const double* __restrict__ U,
+                                                const double* __restrict__ ylmT,
+                                                const double* __restrict__ atoms,
+                                                const double* __restrict__ px,
+                                                const double* __restrict__ py,
+                                                const double* __restrict__ pz, int nrad, int nang,
+                                                int nlm, double* vcell) {
+  extern __shared__ double sh[];
+  double* u = sh;           // nlm
+  double* half = sh + nlm;  // 512
+  const long ai = blockIdx.x;
+  const int a = (int)(ai / nrad);
+  for (int lm = threadIdx.x; lm < nlm; lm += blockDim.x) u[lm] = U[ai * nlm + lm];
   __syncthreads();
-  const double Ax = atoms[4 * a], Ay = atoms[4 * a + 1], Az = atoms[4 * a + 2];
-  for (int j = threadIdx.x; j < nang; j += 256) {
-    double acc[kShells] = {};
-    for (int lm = 0; lm < nlm; ++lm) {
-      const double y = ylmT[(long)lm * nang + j];
-#pragma unroll
-      for (int sh_ = 0; sh_ < kShells; ++sh_) acc[sh_] += y * u[(sh_ < ns ? sh_ : 0) * nlm + lm];
-    }
-    for (int sh_ = 0; sh_ < ns; ++sh_) {
-      const long p = (ai0 + sh_) * nang + j;
-      const double dx = px[p] - Ax, dy = py[p] - Ay, dz = pz[p] - Az;
-      vcell[p] = acc[sh_] / sqrt(dx * dx + dy * dy + dz * dz);
+  const int hsel = threadIdx.x >> 9, jt = threadIdx.x & 511;
+  const int lmid = nlm / 2;
+  const int lb = hsel ? lmid : 0, le = hsel ? nlm : lmid;
+  for (int j0 = 0; j0 < nang; j0 += 512) {
+    const int j = j0 + jt;
+    double acc = 0.0;
+    if (j < nang)
+      for (int lm = lb; lm < le; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
+    __syncthreads();
+    if (hsel) half[jt] = acc;
+    __syncthreads();
+    if (!hsel && j < nang) {
+      const long p = ai * nang + j;
+      const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
+      vcell[p] = (acc + half[jt]) / sqrt(dx * dx + dy * dy + dz * dz);
     }
   }
 }
@@ -724,31 +727,43 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
   }
 }
 
-// out_o = sum over splits of the partial tiles; one thread per tile element
-// (coalesced over splits).  Tiles on the diagonal contribute their upper
-// triangle only and every element is mirrored, so the output is exactly symmetric.
-__global__ void k_contract_reduce(const double* __restrict__ part, int nsplit, int ngroups, int nout,
-                                  const int* __restrict__ groups, int nbf, double* out0,
-                                  double* out1) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)ngroups * nout * 1024) return;
+// out_o = sum over splits of the partial tiles.  A block of 256 threads takes
+// 16 tile elements; thread (element, r) sums the splits r, r+16, ... and the 16
+// partial sums are added in order, so the result does not depend on timing.
+// Tiles on the diagonal contribute their upper triangle only and every element
+// is mirrored, so the output is exactly symmetric.
+__global__ __launch_bounds__(256) void k_contract_reduce(const double* __restrict__ part, int nsplit,
+                                                         int ngroups, int nout,
+                                                         const int* __restrict__ groups, int nbf,
+                                                         double* out0, double* out1) {
+  __shared__ double red[16][17];
+  const int el = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const long e = (long)blockIdx.x * 16 + el;
+  const bool live = e < (long)ngroups * nout * 1024;
+  double s = 0.0;
   const int idx = (int)(e & 1023);
-  const int o = (int)((e >> 10) % nout);
-  const int grp = (int)((e >> 10) / nout);
+  const int o = live ? (int)((e >> 10) % nout) : 0;
+  const int grp = live ? (int)((e >> 10) / nout) : 0;
+  if (live) {
+    const long stride = (long)ngroups * nout * 1024;
+    const double* src = part + ((long)grp * nout + o) * 1024 + idx;
+    for (int sp = r; sp < nsplit; sp += 16) s += src[sp * stride];
+  }
+  red[el][r] = s;
+  __syncthreads();
+  if (r != 0 || !live) return;
+  double tot = 0.0;
+  for (int q = 0; q < 16; ++q) tot += red[el][q];
   const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
   const int t = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
   if (t >= nJ) return;
   const int il = 4 * v + (ln >> 4), jl = ln & 15;
   const int i = 16 * I + il, j = 16 * (J0 + t) + jl;
   if (i >= nbf || j >= nbf || (I == J0 + t && il > jl)) return;
-  double s = 0.0;
-  const long stride = (long)ngroups * nout * 1024;
-  const double* src = part + ((long)grp * nout + o) * 1024 + idx;
-  for (int sp = 0; sp < nsplit; ++sp) s += src[sp * stride];
   double* out = (o == 0) ? out0 : out1;
   if (!out) return;
-  out[i + (long)j * nbf] = s;
-  out[j + (long)i * nbf] = s;
+  out[i + (long)j * nbf] = tot;
+  out[j + (long)i * nbf] = tot;
 }
 
 // One wave: d[v*64 + lane] = register v of lane after D = mfma(a[lane], b[lane], 0).
@@ -988,8 +1003,7 @@ int stage_hartree(dg_handle* h) {
                        h->nrad, h->nlm, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    const int groups = (h->nrad + kShells - 1) / kShells;
-    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * groups), dim3(256), kShells * h->nlm * sizeof(double), h->stream,
+    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(1024), (h->nlm + 512) * sizeof(double), h->stream,
                        h->U, h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
   }));
   if (h->natoms > 1) {
@@ -1042,7 +1056,7 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
                          h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
   }));
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 16)), dim3(256), 0,
                        h->stream, h->cpart, h->nsplit, h->ngroups, nout, h->blocks, h->nbf, out0, out1);
   });
 }
