@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="benzene_631g")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision comparison")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF energy / s-per-iteration runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -103,39 +106,41 @@ def main():
     C = X @ Cc
     P = C[:, :mol.nelec // 2] @ C[:, :mol.nelec // 2].T
 
-    t0 = time.perf_counter()
-    grid = G.MolecularGrid(mol, nrad, nang, lmax)
-    setup_s = time.perf_counter() - t0
-    nbf, npts = grid.nbf, grid.npts
-    stream = torch.cuda.Stream(device=dev)
-    grid.set_stream(stream.cuda_stream)
-    dP = torch.tensor(P.T.copy(), dtype=torch.float64, device=dev)
-    dJ = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
-    dX = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
-    dS = torch.empty(4, dtype=torch.float64, device=dev)
-    ptrs = (dP.data_ptr(), dJ.data_ptr(), dX.data_ptr(), dS.data_ptr())
-    torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        grid.build_device(*ptrs)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    with torch.cuda.stream(stream):
+    def timed_builds(precision):
+        """(grid, ptrs, wall seconds, device ms) for warmup + steps builds on one stream."""
+        t0 = time.perf_counter()
+        grid = G.MolecularGrid(mol, nrad, nang, lmax, precision=precision)
+        setup = time.perf_counter() - t0
+        nbf = grid.nbf
+        stream = torch.cuda.Stream(device=dev)
+        grid.set_stream(stream.cuda_stream)
+        dP = torch.tensor(P.T.copy(), dtype=torch.float64, device=dev)
+        dJ = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
+        dX = torch.empty((nbf, nbf), dtype=torch.float64, device=dev)
+        dS = torch.empty(4, dtype=torch.float64, device=dev)
+        ptrs = (dP.data_ptr(), dJ.data_ptr(), dX.data_ptr(), dS.data_ptr())
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            grid.build_device(*ptrs)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         e0.record(stream)
-    for _ in range(args.steps):
-        grid.build_device(*ptrs)
-    with torch.cuda.stream(stream):
+        for _ in range(args.steps):
+            grid.build_device(*ptrs)
         e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    dev_ms = e0.elapsed_time(e1)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return grid, ptrs, (dP, dJ, dX, dS), wall, e0.elapsed_time(e1), setup
+
+    prec = G.PREC_MIXED if args.precision == "mixed" else G.PREC_FP64
+    grid, ptrs, bufs, wall, dev_ms, setup_s = timed_builds(prec)
+    nbf, npts = grid.nbf, grid.npts
     ms_per_step = 1e3 * wall / args.steps
     units = float(npts) * nbf * nbf
     value = units * args.steps / wall
-    scal = dS.cpu().numpy()
+    scal = bufs[3].cpu().numpy()
 
     out = {
         "metric": "grid V_xc+J build throughput (N_pts*N_bf^2 per second)",
@@ -148,11 +153,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == G.PREC_FP64 else "f64+bf16",
         "data": "synthetic (core-Hamiltonian guess P)",
         "config": {"workload": args.workload, "molecule": mol.name, "basis": mol.basis,
                    "n_bf": nbf, "n_atoms": mol.natoms, "n_radial": nrad, "n_angular": nang,
-                   "lmax": lmax, "n_pts": npts, "precision": "fp64"},
+                   "lmax": lmax, "n_pts": npts, "precision": args.precision},
         "device_ms_per_step": dev_ms / args.steps,
         "grid_setup_s": setup_s,
         "e_xc": float(scal[0]),
@@ -187,11 +192,38 @@ def main():
         roof["avg_launch_ms"] = avg_s * 1e3
         roof["traffic"] = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+        if args.precision != "fp64":
+            tpath = ""
         if os.path.exists(tpath):
             with open(tpath) as f:
                 t = json.load(f)
             roof["traffic"] = t.get(dom)
         out["roofline"] = roof
+
+    if not args.no_mixed and prec == G.PREC_FP64:
+        # the same build with BF16-split Fock contractions and FP32 high-l Hartree terms
+        gmx, pmx, bmx, wmx, dmx, _ = timed_builds(G.PREC_MIXED)
+        Jm, Xm = bmx[1].cpu().numpy(), bmx[2].cpu().numpy()
+        J6, X6 = bufs[1].cpu().numpy(), bufs[2].cpu().numpy()
+        out["mixed"] = {
+            "value": units * args.steps / wmx, "ms_per_step": 1e3 * wmx / args.steps,
+            "speedup_vs_fp64": wall / wmx,
+            "J_max_rel_err": float(np.abs(Jm - J6).max() / np.abs(J6).max()),
+            "XC_max_rel_err": float(np.abs(Xm - X6).max() / np.abs(X6).max()),
+        }
+        del gmx, bmx
+
+    if not args.no_scf:
+        scf = {}
+        for name, pr in (("fp64", G.PREC_FP64), ("mixed", G.PREC_MIXED)):
+            if name == "mixed" and args.no_mixed:
+                continue
+            en, res, _ = G.scf_run(mol, nrad, nang, lmax, threshold=1e-6, precision=pr)
+            scf[name] = {"energy_Ha": res["energy"], "iterations": res["iterations"],
+                         "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"]}
+        if "mixed" in scf:
+            scf["mixed"]["energy_error_vs_fp64_Ha"] = scf["mixed"]["energy_Ha"] - scf["fp64"]["energy_Ha"]
+        out["scf"] = scf
 
     if not args.no_cpu_baseline:
         dt, threads = cpu_baseline(mol, nrad, nang, lmax, P)

@@ -49,6 +49,8 @@ namespace {
   } while (0)
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -327,17 +329,27 @@ __global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho,
   if (threadIdx.x == 0) exc_part[blockIdx.x] = red[0];
 }
 
-// scalars[0] = sum of exc_part (fixed tree order); scalars[2] = sum q.
+// scalars[0] = sum of exc_part (fixed tree order); scalars[2] = sum q;
+// scalars[3] = E_J = 2 tr(P J) evaluated pointwise: J = (1/2) Phi^T diag(w V) Phi
+// and rho = c 2 phi^T P phi give 2 tr(P J) = sum w V rho / (2c), c = sum Z / q_raw.
 __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_part, long nparts,
-                                                 const double* __restrict__ q, int natoms,
+                                                 const double* __restrict__ ej_part, long nej,
+                                                 const double* __restrict__ q, int natoms, double zsum,
                                                  double* scalars) {
   __shared__ double red[256];
-  double s = 0.0;
+  __shared__ double red2[256];
+  double s = 0.0, s2 = 0.0;
   for (long i = threadIdx.x; i < nparts; i += 256) s += exc_part[i];
+  if (ej_part)
+    for (long i = threadIdx.x; i < nej; i += 256) s2 += ej_part[i];
   red[threadIdx.x] = s;
+  red2[threadIdx.x] = s2;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    if (threadIdx.x < st) {
+      red[threadIdx.x] += red[threadIdx.x + st];
+      red2[threadIdx.x] += red2[threadIdx.x + st];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -347,6 +359,7 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
       for (int a = 0; a < natoms; ++a) qs += q[a];
       scalars[2] = qs;
     }
+    scalars[3] = ej_part ? red2[0] * scalars[1] / (2.0 * zsum) : 0.0;
   }
 }
 
@@ -633,23 +646,171 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   }
 }
 
+// Reduced-precision variant of k_hartree_cross: geometry, interval search,
+// the l = 0 term and the final cubic in FP64; the l >= 1 harmonics, their
+// spline table (FP32 copy in LDS) and the four lm-sums in FP32.  The whole
+// l range is one pass (the FP32 table is half the size).
+template <int LMAX>
+__global__ __launch_bounds__(1024) void k_hartree_cross_mx(CrossArgs A) {
+  extern __shared__ double lds[];
+  const int n = A.nrad, nlm = A.nlm, nh = nlm - 1;
+  double* kn = lds;
+  double* ih = lds + n;
+  double* t00 = lds + 2 * n;             // n x {y, D} of lm = 0, FP64
+  float* tf = (float*)(lds + 4 * n);     // n x nh x {y, D}, FP32
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    kn[i] = A.knots[i];
+    ih[i] = A.invh[i];
+  }
+  const int it0 = (int)((long)blockIdx.x * A.nitems / gridDim.x);
+  const int it1 = (int)((long)(blockIdx.x + 1) * A.nitems / gridDim.x);
+  const double y00 = A.shc[0];
+  int cur = -1;
+  for (int it = it0; it < it1; ++it) {
+    const int k = A.items[3 * it];
+    if (k != cur) {
+      __syncthreads();
+      const double* src = A.tab + (long)k * n * nlm * 2;
+      for (int i = threadIdx.x; i < n * nlm * 2; i += blockDim.x) {
+        const int t = i / (nlm * 2), rem = i % (nlm * 2);
+        if (rem < 2)
+          t00[2 * t + rem] = src[i];
+        else
+          tf[t * nh * 2 + rem - 2] = (float)src[i];
+      }
+      __syncthreads();
+      cur = k;
+    }
+    const long pb = A.items[3 * it + 1], pe = A.items[3 * it + 2];
+    const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
+    const double xlo = kn[0], xhi = kn[n - 1];
+    for (long p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+      const double dx = A.px[p] - Ax, dy = A.py[p] - Ay, dz = A.pz[p] - Az;
+      const double rxy2 = dx * dx + dy * dy;
+      const double r = sqrt(rxy2 + dz * dz);
+      const double rinv = 1.0 / r;
+      const double rxy = sqrt(rxy2);
+      const float ct = (float)(dz * rinv), st = (float)(rxy * rinv);
+      float cp = 1.f, sp = 0.f;
+      if (rxy > 0.0) {
+        const double ir = 1.0 / rxy;
+        cp = (float)(dx * ir);
+        sp = (float)(dy * ir);
+      }
+      int s0, s1;
+      double t = 0.0;
+      bool clamp = true;
+      if (r < xlo) {
+        s0 = s1 = 0;
+      } else if (r >= xhi) {
+        s0 = s1 = n - 1;
+      } else {
+        int lo = 1, hi = n - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (r <= kn[mid])
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        s1 = lo;
+        s0 = lo - 1;
+        t = r - kn[s0];
+        clamp = false;
+      }
+      const float* T0 = tf + s0 * nh * 2;
+      const float* T1 = tf + s1 * nh * 2;
+      float fy0 = 0.f, fd0 = 0.f, fy1 = 0.f, fd1 = 0.f;
+      float pmm = 1.f, cm = 1.f, sm = 0.f;
+#pragma unroll
+      for (int m = 0; m <= LMAX; ++m) {
+        if (m > 0) {
+          pmm *= -(float)(2 * m - 1) * st;
+          const float c2 = cm * cp - sm * sp;
+          sm = sm * cp + cm * sp;
+          cm = c2;
+        }
+        float p1 = 0.f, p2 = 0.f;
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l) {
+          const int ci = 3 * (l * (l + 1) / 2 + m);
+          float pl;
+          if (l == m)
+            pl = pmm;
+          else if (l == m + 1)
+            pl = ct * (float)(2 * m + 1) * pmm;
+          else
+            pl = (float)A.shc[ci + 1] * ct * p1 - (float)A.shc[ci + 2] * p2;
+          p2 = p1;
+          p1 = pl;
+          if (l >= 1) {
+            const float base = (float)A.shc[ci] * pl;
+            const int ip = (l * l + l + m - 1) * 2;
+            const float yp = (m == 0) ? base : base * cm;
+            fy0 += yp * T0[ip];
+            fd0 += yp * T0[ip + 1];
+            fy1 += yp * T1[ip];
+            fd1 += yp * T1[ip + 1];
+            if (m > 0) {
+              const int in = (l * l + l - m - 1) * 2;
+              const float yn = base * sm;
+              fy0 += yn * T0[in];
+              fd0 += yn * T0[in + 1];
+              fy1 += yn * T1[in];
+              fd1 += yn * T1[in + 1];
+            }
+          }
+        }
+      }
+      const double sy0 = y00 * t00[2 * s0] + (double)fy0;
+      const double sd0 = y00 * t00[2 * s0 + 1] + (double)fd0;
+      const double sy1 = y00 * t00[2 * s1] + (double)fy1;
+      const double sd1 = y00 * t00[2 * s1 + 1] + (double)fd1;
+      double val;
+      if (clamp) {
+        val = sy0;
+      } else {
+        const double hi_ = ih[s1];
+        const double dyv = (sy1 - sy0) * hi_;
+        const double c = hi_ * (3.0 * dyv - 2.0 * sd0 - sd1);
+        const double d = hi_ * hi_ * (-2.0 * dyv + sd0 + sd1);
+        val = sy0 + sd0 * t + c * t * t + d * t * t * t;
+      }
+      A.vpart[(long)k * A.npts_pad + p] = val * rinv;
+    }
+  }
+}
+
 // V[p] = sum_k (k == own ? vcell[p] : vpart[k][p]), k ascending
-// (src/moleculargrid.cpp:347-378); uJ[p] = w[p] V[p] / 2 (src/atomicgrid.cpp:475-485).
-__global__ void k_vreduce(const double* __restrict__ vpart, const double* __restrict__ vcell,
-                          const double* __restrict__ w, long npts, long npts_pad, long cell,
-                          int natoms, double* V, double* uJ) {
+// (src/moleculargrid.cpp:347-378); uJ[p] = w[p] V[p] / 2 (src/atomicgrid.cpp:475-485);
+// ej_part[block] = sum w V rho over the block (fixed tree order).
+__global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpart,
+                                                 const double* __restrict__ vcell,
+                                                 const double* __restrict__ w,
+                                                 const double* __restrict__ rho, long npts,
+                                                 long npts_pad, long cell, int natoms, double* V,
+                                                 double* uJ, double* ej_part) {
+  __shared__ double red[256];
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npts_pad) return;
-  if (p >= npts) {
+  double e = 0.0;
+  if (p < npts) {
+    const int own = (int)(p / cell);
+    double v = 0.0;
+    for (int k = 0; k < natoms; ++k) v += (k == own) ? vcell[p] : vpart[(long)k * npts_pad + p];
+    V[p] = v;
+    uJ[p] = (w[p] * v) * 0.5;
+    e = w[p] * v * rho[p];
+  } else if (p < npts_pad) {
     V[p] = 0.0;
     uJ[p] = 0.0;
-    return;
   }
-  const int own = (int)(p / cell);
-  double v = 0.0;
-  for (int k = 0; k < natoms; ++k) v += (k == own) ? vcell[p] : vpart[(long)k * npts_pad + p];
-  V[p] = v;
-  uJ[p] = (w[p] * v) * 0.5;
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ej_part[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -727,6 +888,122 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
   }
 }
 
+// Phi split for the reduced-precision contractions: hi = bf16(phi),
+// lo = bf16(phi - hi), stored point-blocked as [p/32][col][32] so that the 8
+// consecutive points a lane feeds to one BF16 MFMA are one 16-byte load.
+__global__ void k_phi_split(const double* __restrict__ phi, int ldphi, long npts_pad, __bf16* hi,
+                            __bf16* lo) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= npts_pad * ldphi) return;
+  const long p = e / ldphi;
+  const int col = (int)(e % ldphi);
+  const double v = phi[e];
+  const __bf16 h = (__bf16)(float)v;
+  const __bf16 l = (__bf16)(float)(v - (double)(float)h);
+  const long o = ((p >> 5) * ldphi + col) * 32 + (p & 31);
+  hi[o] = h;
+  lo[o] = l;
+}
+
+// Reduced-precision contraction C_o = Phi^T diag(u_o) Phi with the BF16
+// 16x16x32 MFMA (FP32 accumulation).  Column groups of up to 4 tiles
+// (I0..I0+nI-1; J) share the B operand B = u Phi_J, which is formed in FP32
+// from hi + lo and split again into BF16 hi/lo in registers; each tile takes
+// A_hi B_hi + A_hi B_lo + A_lo B_hi.  Lane (c, g) feeds A[c][8g+e] =
+// Phi[p0+8g+e][16I+c], B[8g+e][c] = u Phi[p0+8g+e][16J+c] and receives
+// C[16I + 4g + v][16J + c] in FP32 register v.  The FP32 sums are flushed into
+// FP64 accumulators every 32 steps (1024 points per wave).
+template <int NOUT>
+__global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ phh,
+                                                     const __bf16* __restrict__ phl, int ldphi,
+                                                     const double* __restrict__ u0,
+                                                     const double* __restrict__ u1,
+                                                     const int* __restrict__ groups, long chunk,
+                                                     long npts_pad, int ngroups, double* part) {
+  __shared__ double red[4 * 4 * 4 * 64];
+  const int grp = blockIdx.x, split = blockIdx.y;
+  const int J = groups[3 * grp], I0 = groups[3 * grp + 1], nI = groups[3 * grp + 2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long kb0 = (long)split * chunk / 32;
+  const long kb1 = min((long)split * chunk + chunk, npts_pad) / 32;  // chunk, npts_pad multiples of 128
+  dbl4 accd[NOUT][4];
+  flt4 accf[NOUT][4];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      accd[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+      accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
+    }
+  int steps = 0;
+  for (long kb = kb0 + wave; kb < kb1; kb += 4) {
+    const long bb = (kb * ldphi + 16 * J + c) * 32 + 8 * g;
+    const bf16x8 bh = *(const bf16x8*)(phh + bb);
+    const bf16x8 bl = *(const bf16x8*)(phl + bb);
+    const double* up0 = u0 + kb * 32 + 8 * g;
+    const double* up1 = u1 + kb * 32 + 8 * g;
+    bf16x8 b0h, b0l, b1h, b1l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float f = (float)bh[e] + (float)bl[e];
+      const float x0 = (float)up0[e] * f;
+      b0h[e] = (__bf16)x0;
+      b0l[e] = (__bf16)(x0 - (float)b0h[e]);
+      if (NOUT == 2) {
+        const float x1 = (float)up1[e] * f;
+        b1h[e] = (__bf16)x1;
+        b1l[e] = (__bf16)(x1 - (float)b1h[e]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < nI) {
+        const long ab = (kb * ldphi + 16 * (I0 + t) + c) * 32 + 8 * g;
+        const bf16x8 ah = *(const bf16x8*)(phh + ab);
+        const bf16x8 al = *(const bf16x8*)(phl + ab);
+        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b0h, accf[0][t], 0, 0, 0);
+        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b0l, accf[0][t], 0, 0, 0);
+        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b0h, accf[0][t], 0, 0, 0);
+        if (NOUT == 2) {
+          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1h, accf[NOUT - 1][t], 0, 0, 0);
+          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1l, accf[NOUT - 1][t], 0, 0, 0);
+          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1h, accf[NOUT - 1][t], 0, 0, 0);
+        }
+      }
+    }
+    if (++steps == 32) {
+      steps = 0;
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) accd[o][t][v] += (double)accf[o][t][v];
+          accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) accd[o][t][v] += (double)accf[o][t][v];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + v) * 64 + lane] = accd[o][t][v];
+    __syncthreads();
+    double* dst = part + (((long)split * ngroups + grp) * NOUT + o) * 1024;
+    for (int e = threadIdx.x; e < nI * 256; e += 256)
+      dst[e] = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
+  }
+}
+
 // out_o = sum over splits of the partial tiles.  A block of 256 threads takes
 // 16 tile elements; thread (element, r) sums the splits r, r+16, ... and the 16
 // partial sums are added in order, so the result does not depend on timing.
@@ -735,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
 __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restrict__ part, int nsplit,
                                                          int ngroups, int nout,
                                                          const int* __restrict__ groups, int nbf,
-                                                         double* out0, double* out1) {
+                                                         int colgroups, double* out0, double* out1) {
   __shared__ double red[16][17];
   const int el = threadIdx.x & 15, r = threadIdx.x >> 4;
   const long e = (long)blockIdx.x * 16 + el;
@@ -754,16 +1031,39 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restric
   if (r != 0 || !live) return;
   double tot = 0.0;
   for (int q = 0; q < 16; ++q) tot += red[el][q];
-  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
   const int t = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
-  if (t >= nJ) return;
-  const int il = 4 * v + (ln >> 4), jl = ln & 15;
-  const int i = 16 * I + il, j = 16 * (J0 + t) + jl;
-  if (i >= nbf || j >= nbf || (I == J0 + t && il > jl)) return;
+  int ti, tj, il;
+  if (colgroups) {  // BF16 layout: tile (I0+t, J), result register v holds row 4g + v
+    if (t >= groups[3 * grp + 2]) return;
+    ti = groups[3 * grp + 1] + t;
+    tj = groups[3 * grp];
+    il = 4 * (ln >> 4) + v;
+  } else {  // FP64 layout: tile (I, J0+t), result register v holds row 4v + g
+    if (t >= groups[3 * grp + 2]) return;
+    ti = groups[3 * grp];
+    tj = groups[3 * grp + 1] + t;
+    il = 4 * v + (ln >> 4);
+  }
+  const int jl = ln & 15;
+  const int i = 16 * ti + il, j = 16 * tj + jl;
+  if (i >= nbf || j >= nbf || (ti == tj && il > jl)) return;
   double* out = (o == 0) ? out0 : out1;
   if (!out) return;
   out[i + (long)j * nbf] = tot;
   out[j + (long)i * nbf] = tot;
+}
+
+// One wave of the BF16 16x16x32 MFMA: lane i supplies a[8i..8i+7], b[8i..8i+7]
+// (FP32 values rounded to BF16); d[v*64 + lane] = FP32 result register v.
+__global__ void k_mfma_raw_bf16(const float* a, const float* b, float* d) {
+  const int lane = threadIdx.x;
+  bf16x8 av, bv;
+  for (int e = 0; e < 8; ++e) {
+    av[e] = (__bf16)a[8 * lane + e];
+    bv[e] = (__bf16)b[8 * lane + e];
+  }
+  flt4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) d[v * 64 + lane] = r[v];
 }
 
 // One wave: d[v*64 + lane] = register v of lane after D = mfma(a[lane], b[lane], 0).
@@ -839,6 +1139,13 @@ struct dg_handle {
   int* blocks = nullptr;
   int* items = nullptr;
   int nitems = 0, cross_wgs = 0;
+  // reduced-precision mode
+  __bf16 *phh = nullptr, *phl = nullptr;
+  int* mgroups = nullptr;
+  int ngroups_mx = 0, nsplit_mx = 0;
+  long chunk_mx = 0;
+  size_t cross_lds_mx = 0;
+  double* ej_part = nullptr;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
   // state
   bool has_density = false, has_hartree = false, has_xc = false;
@@ -855,7 +1162,10 @@ struct dg_handle {
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items};
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups};
+    if (phh) (void)hipFree(phh);
+    if (phl) (void)hipFree(phl);
+    if (ej_part) (void)hipFree(ej_part);
     for (int* b : ibufs)
       if (b) (void)hipFree(b);
     for (auto& e : pending) {
@@ -956,12 +1266,18 @@ int stage_correct(dg_handle* h) {
 
 template <int L>
 void launch_cross_l(dg_handle* h, CrossArgs& a, dim3 grid) {
-  hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
+  if (h->precision == DG_PREC_MIXED)
+    hipLaunchKernelGGL(k_hartree_cross_mx<L>, grid, dim3(1024), h->cross_lds_mx, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
 }
 
 template <int L>
 hipError_t allow_cross_lds(size_t bytes) {
-  return hipFuncSetAttribute((const void*)k_hartree_cross<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  hipError_t e = hipFuncSetAttribute((const void*)k_hartree_cross<L>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_hartree_cross_mx<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)bytes);
 }
 
@@ -1023,6 +1339,12 @@ int stage_hartree(dg_handle* h) {
     a.nrad = h->nrad;
     a.nlm = h->nlm;
     dim3 grid(h->cross_wgs);
+    if (h->precision == DG_PREC_MIXED) {  // one pass over all l
+      a.l0 = 0;
+      a.l1 = h->lmax + 1;
+      a.accumulate = 0;
+      DG_TRY(run(h, "hartree_cross", [&] { launch_cross(h, a, grid); }));
+    } else
     for (size_t ci = 0; ci < h->lchunks.size(); ++ci) {
       a.l0 = h->lchunks[ci].first;
       a.l1 = h->lchunks[ci].second;
@@ -1032,7 +1354,8 @@ int stage_hartree(dg_handle* h) {
   }
   DG_TRY(run(h, "vreduce", [&] {
     hipLaunchKernelGGL(k_vreduce, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->vpart,
-                       h->vcell, h->w, h->npts, h->npts_pad, h->cell, h->natoms, h->V, h->uJ);
+                       h->vcell, h->w, h->rho, h->npts, h->npts_pad, h->cell, h->natoms, h->V, h->uJ,
+                       h->ej_part);
   }));
   return DG_OK;
 }
@@ -1046,25 +1369,38 @@ int stage_lda(dg_handle* h) {
 
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
+  const bool mx = h->precision == DG_PREC_MIXED;
+  const int ng = mx ? h->ngroups_mx : h->ngroups, ns = mx ? h->nsplit_mx : h->nsplit;
+  const int* grp = mx ? h->mgroups : h->blocks;
   DG_TRY(run(h, "contract", [&] {
-    dim3 grid(h->ngroups, h->nsplit);
-    if (nout == 2)
-      hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
-                         h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
-    else
-      hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
-                         h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
+    if (mx) {
+      dim3 grid(h->ngroups_mx, h->nsplit_mx);
+      if (nout == 2)
+        hipLaunchKernelGGL(k_contract_mx<2>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u1,
+                           h->mgroups, h->chunk_mx, h->npts_pad, h->ngroups_mx, h->cpart);
+      else
+        hipLaunchKernelGGL(k_contract_mx<1>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u0,
+                           h->mgroups, h->chunk_mx, h->npts_pad, h->ngroups_mx, h->cpart);
+    } else {
+      dim3 grid(h->ngroups, h->nsplit);
+      if (nout == 2)
+        hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
+                           h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
+      else
+        hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
+                           h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
+    }
   }));
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 16)), dim3(256), 0,
-                       h->stream, h->cpart, h->nsplit, h->ngroups, nout, h->blocks, h->nbf, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)ng * nout * 1024, 16)), dim3(256), 0, h->stream,
+                       h->cpart, ns, ng, nout, grp, h->nbf, mx ? 1 : 0, out0, out1);
   });
 }
 
-int stage_scalars(dg_handle* h) {
+int stage_scalars(dg_handle* h, bool with_ej) {
   return run(h, "scalars", [&] {
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks, h->qc,
-                       h->natoms, h->scalars);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks,
+                       with_ej ? h->ej_part : nullptr, h->npts_pad / 256, h->qc, h->natoms, h->zsum, h->scalars);
   });
 }
 
@@ -1239,6 +1575,25 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
     h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
   }
+  // reduced-precision contraction: column groups (J; I0..I0+nI-1) sharing B
+  std::vector<int> mgroups;
+  if (h->precision == DG_PREC_MIXED) {
+    for (int J = 0; J < h->ntb; ++J)
+      for (int I0 = 0; I0 <= J; I0 += 4) {
+        mgroups.push_back(J);
+        mgroups.push_back(I0);
+        mgroups.push_back(std::min(4, J + 1 - I0));
+      }
+    h->ngroups_mx = (int)mgroups.size() / 3;
+    const long target_wg = 1024;
+    long ns = std::max(1L, target_wg / h->ngroups_mx);
+    ns = std::min(std::min(ns, 256L), h->npts_pad / 128);
+    h->chunk_mx = round_up((h->npts_pad + ns - 1) / ns, 128);
+    h->nsplit_mx = (int)((h->npts_pad + h->chunk_mx - 1) / h->chunk_mx);
+    h->cross_lds_mx = (size_t)(4 * h->nrad + h->nrad * (h->nlm - 1)) * sizeof(double);
+    if (h->cross_lds_mx > 150 * 1024)
+      return fail(DG_ERR_UNSUPPORTED, "dg_create: radial grid too large for the mixed Hartree kernel");
+  }
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
   {
@@ -1276,7 +1631,8 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-    const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(h->cross_lds, 1)));
+    const size_t lds = h->precision == DG_PREC_MIXED ? h->cross_lds_mx : h->cross_lds;
+    const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
   }
 
@@ -1317,7 +1673,13 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, items.size()));
-  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->ngroups * 2 * 1024));
+  DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit * h->ngroups, (size_t)h->nsplit_mx * h->ngroups_mx) * 2 * 1024));
+  DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
+  if (h->precision == DG_PREC_MIXED) {
+    DG_TRY(dalloc(&h->mgroups, mgroups.size()));
+    HIP_TRY(hipMalloc((void**)&h->phh, (size_t)np * h->ldphi * sizeof(__bf16)));
+    HIP_TRY(hipMalloc((void**)&h->phl, (size_t)np * h->ldphi * sizeof(__bf16)));
+  }
   DG_TRY(dalloc(&h->J, (size_t)h->nbf * h->nbf));
   DG_TRY(dalloc(&h->XC, (size_t)h->nbf * h->nbf));
 
@@ -1347,13 +1709,14 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
   if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
+  if (!mgroups.empty()) HIP_TRY(up(h->mgroups, mgroups.data(), mgroups.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)h->natoms * np * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->rho, 0, (size_t)np * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->vcell, 0, (size_t)np * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->scalars, 0, 4 * 8, h->stream));
 
-  HIP_TRY(allow_cross_lds_l(h->lmax, h->cross_lds));
+  HIP_TRY(allow_cross_lds_l(h->lmax, std::max(h->cross_lds, h->cross_lds_mx)));
   if ((size_t)h->natoms * 64 * 8 > 160 * 1024)
     return fail(DG_ERR_UNSUPPORTED, "dg_create: too many atoms for the Becke kernel's LDS");
   HIP_TRY(hipFuncSetAttribute((const void*)k_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1371,6 +1734,12 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
                        h->nshell, h->sh_atom, h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha,
                        h->pr_cn0, h->pr_cn1, h->npts, h->px, h->py, h->pz, h->ldphi, h->phi);
   }));
+  if (h->precision == DG_PREC_MIXED) {
+    DG_TRY(run(h, "phi_split", [&] {
+      hipLaunchKernelGGL(k_phi_split, dim3(nblocks(np * h->ldphi, 256)), dim3(256), 0, h->stream, h->phi,
+                         h->ldphi, np, h->phh, h->phl);
+    }));
+  }
   DG_TRY(sync(h));
   guard.release();
   *out = h;
@@ -1469,7 +1838,7 @@ int dg_xc(dg_handle* h, double* XC, double* exc) {
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_xc: no density set");
   DG_TRY(stage_lda(h));
   DG_TRY(stage_contract(h, h->uxc, nullptr, h->XC, nullptr));
-  DG_TRY(stage_scalars(h));
+  DG_TRY(stage_scalars(h, false));
   h->has_xc = true;
   if (XC) DG_TRY(copy_out(h, XC, h->XC, (size_t)h->nbf * h->nbf));
   if (exc) {
@@ -1487,7 +1856,7 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
   DG_TRY(stage_lda(h));
   DG_TRY(stage_hartree(h));
   DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J));
-  DG_TRY(stage_scalars(h));
+  DG_TRY(stage_scalars(h, true));
   if (dscalars)
     HIP_TRY(hipMemcpyAsync(dscalars, h->scalars, 4 * 8, hipMemcpyDeviceToDevice, h->stream));
   h->has_hartree = h->has_xc = true;
@@ -1609,6 +1978,23 @@ int dg_mfma_probe(const double* a, const double* b, double* d) {
   if (e == hipSuccess) e = hipMemcpy(d, buf + 128, 256 * 8, hipMemcpyDeviceToHost);
   (void)hipFree(buf);
   if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_mfma_probe: ") + hipGetErrorString(e));
+  return DG_OK;
+}
+
+int dg_mfma_probe_bf16(const float* a, const float* b, float* d) {
+  if (!a || !b || !d) return fail(DG_ERR_INVALID, "dg_mfma_probe_bf16: null argument");
+  float* buf = nullptr;
+  hipError_t e = hipMalloc((void**)&buf, (512 + 512 + 256) * sizeof(float));
+  if (e != hipSuccess) return fail(DG_ERR_ALLOC, "dg_mfma_probe_bf16: hipMalloc failed");
+  e = hipMemcpy(buf, a, 512 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(buf + 512, b, 512 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_raw_bf16, dim3(1), dim3(64), 0, 0, buf, buf + 512, buf + 1024);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(d, buf + 1024, 256 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_mfma_probe_bf16: ") + hipGetErrorString(e));
   return DG_OK;
 }
 

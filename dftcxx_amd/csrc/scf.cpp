@@ -302,7 +302,8 @@ void gemm(int n, const double* A, bool ta, const double* B, double* C) {  // C =
 // The reference's DFT class (src/dft.h:41-216) with the grid work on the GPU.
 class DFT {
  public:
-  DFT(const MoleculeData& mol, const dg_grid_params& grid) : mol_(mol), grid_(mol) {
+  DFT(const MoleculeData& mol, const dg_grid_params& grid)
+      : mol_(mol), grid_(mol), mixed_(grid.precision == DG_PREC_MIXED) {
     grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision);
     grid_.create_grid();
     n_ = grid_.nbf();
@@ -423,6 +424,7 @@ class DFT {
         hipStreamSynchronize(st) != hipSuccess)
       throw std::runtime_error("copy of grid results failed");
     nelec_grid_ = sc[2];
+    ej_point_ = sc[3];
     if (with_xc) {
       XC_.swap(XC);
       exc_ = sc[0];
@@ -438,12 +440,16 @@ class DFT {
         t2 += P_[i + (size_t)k * n_] * J_[k + (size_t)i * n_];
       }
     e_one_ = 2.0 * t1;
-    e_j_ = 2.0 * t2;
+    // In reduced precision J carries BF16-product errors that would enter 2 tr(PJ)
+    // at first order; the same quantity evaluated pointwise in FP64 does not.
+    e_j_ = mixed_ ? ej_point_ : 2.0 * t2;
     et_ = e_one_ + e_j_ + enuc_ + exc_;
   }
 
   MoleculeData mol_;
   MolecularGrid grid_;
+  bool mixed_ = false;
+  double ej_point_ = 0.0;
   int n_ = 0, nelec_ = 0;
   bool first_ = true;
   std::vector<double> S_, H_, X_, J_, XC_;
