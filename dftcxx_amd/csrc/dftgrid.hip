@@ -888,6 +888,307 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of E_xc (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// Full P in fragment order (no symmetric halving): frag[b][s][lane] =
+// P(16b + lane%16, kk(s, lane/16)) for all 4*nt reduction steps.
+__global__ void k_pfrag_full(const double* __restrict__ P, int nbf, int nt, double* frag) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = 256L * nt;
+  if (e >= per * nt) return;
+  const int b = (int)(e / per);
+  const int rem = (int)(e % per);
+  const int s = rem / 64, l = rem % 64;
+  const int j = 16 * b + (l & 15);
+  const int k = 16 * (s / 4) + 4 * (l >> 4) + (s & 3);
+  frag[e] = (j < nbf && k < nbf) ? P[j + (long)k * nbf] : 0.0;
+}
+
+// T = Phi P (T[p][j] = sum_k phi_pk P_kj) with the FP64 MFMA, same operand
+// arrangement as k_density; T is written [npts_pad][ldphi].
+template <int NT>
+__global__ __launch_bounds__(256) void k_tmat(const double* __restrict__ phi, int ldphi,
+                                              const double* __restrict__ frag, double* tmat) {
+  __shared__ double pl[4 * NT * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const double* row = phi + (p0 + c) * ldphi;
+  double f[4 * NT];
+#pragma unroll
+  for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  for (int b = 0; b < NT; ++b) {
+    __syncthreads();
+    const double* src = frag + 256L * NT * b;
+    for (int e = threadIdx.x; e < 4 * NT * 64; e += 256) pl[e] = src[e];
+    __syncthreads();
+    dbl4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4 * NT; ++s) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
+    double* out = tmat + (p0 + c) * ldphi + 16 * b + g;
+    out[0] = d[0];
+    out[4] = d[1];
+    out[8] = d[2];
+    out[12] = d[3];
+  }
+}
+
+// Value and gradient of component `comp` of a shell with radial sums
+// r0 = sum cN0 e, d0 = sum cN0 (-2 alpha) e (and r1, d1 for the mixed d
+// components): phi = R x^l y^m z^n, d/dx phi = l x^(l-1) y^m z^n R + x phi' with R' = d.
+__device__ __forceinline__ void shell_grad(int L, int comp, double r0, double d0, double r1, double d1,
+                                           double x, double y, double z, double* gr) {
+  int l = 0, m = 0, n = 0;
+  bool mixedd = false;
+  if (L == 1) {
+    l = comp == 0;
+    m = comp == 1;
+    n = comp == 2;
+  } else if (L == 2) {
+    const int t[6][3] = {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}};
+    l = t[comp][0];
+    m = t[comp][1];
+    n = t[comp][2];
+    mixedd = (comp == 1 || comp == 2 || comp == 4);
+  }
+  const double R = mixedd ? r1 : r0, Rp = mixedd ? d1 : d0;
+  auto pw = [](double v, int k) { return k == 0 ? 1.0 : (k == 1 ? v : v * v); };
+  const double px = pw(x, l), py = pw(y, m), pz = pw(z, n);
+  const double poly = px * py * pz;
+  const double dpx = l == 0 ? 0.0 : (l == 1 ? 1.0 : 2.0 * x);
+  const double dpy = m == 0 ? 0.0 : (m == 1 ? 1.0 : 2.0 * y);
+  const double dpz = n == 0 ? 0.0 : (n == 1 ? 1.0 : 2.0 * z);
+  gr[0] = dpx * py * pz * R + x * poly * Rp;
+  gr[1] = px * dpy * pz * R + y * poly * Rp;
+  gr[2] = px * py * dpz * R + z * poly * Rp;
+}
+
+// Wave-level sum in a fixed butterfly order (all 64 lanes end with the total).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Basis-centre and grid-motion terms of dE_xc/dR, one wave per 64 points.
+// For point p and atom A: g_A(p) = ub_p sum_{i on A} T_pi grad phi_i(r_p).
+// part[(A*3+d)*nblk + blk] = -4 sum_{p in blk} g_A(p)  (functions move with A)
+// mot[d*npts + p]          = +4 sum_A g_A(p)         (the point moves with its atom)
+// Shells are visited atom by atom through ash_start/ash_list (shells of atom
+// A are ash_list[ash_start[A] .. ash_start[A+1])).
+__global__ __launch_bounds__(64) void k_force_basis(
+    const double* __restrict__ atoms, int natoms, const int* __restrict__ ash_start,
+    const int* __restrict__ ash_list, const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
+    const int* __restrict__ sh_off, const int* __restrict__ sh_cgf, const double* __restrict__ alpha,
+    const double* __restrict__ cn0, const double* __restrict__ cn1, long npts,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    int ldphi, const double* __restrict__ tmat, const double* __restrict__ ub, long nblk, double* part,
+    double* mot) {
+  const long p = (long)blockIdx.x * 64 + threadIdx.x;
+  const bool valid = p < npts;
+  const double x = valid ? px[p] : 0.0, y = valid ? py[p] : 0.0, z = valid ? pz[p] : 0.0;
+  const double u = valid ? ub[p] : 0.0;
+  const double* trow = tmat + (valid ? p : 0) * ldphi;
+  double G[3] = {0.0, 0.0, 0.0};
+  for (int A = 0; A < natoms; ++A) {
+    double g[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+      const double dx = x - atoms[4 * A], dy = y - atoms[4 * A + 1], dz = z - atoms[4 * A + 2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      for (int t = ash_start[A]; t < ash_start[A + 1]; ++t) {
+        const int s = ash_list[t];
+        double r0 = 0.0, d0 = 0.0, r1 = 0.0, d1 = 0.0;
+        for (int k = 0; k < sh_nprim[s]; ++k) {
+          const int q = sh_off[s] + k;
+          const double e = exp(-alpha[q] * r2);
+          const double m2a = -2.0 * alpha[q];
+          r0 += cn0[q] * e;
+          r1 += cn1[q] * e;
+          d0 += cn0[q] * m2a * e;
+          d1 += cn1[q] * m2a * e;
+        }
+        const int L = sh_L[s], nc = (L + 1) * (L + 2) / 2;
+        for (int c = 0; c < nc; ++c) {
+          double gr[3];
+          shell_grad(L, c, r0, d0, r1, d1, dx, dy, dz, gr);
+          const double tv = u * trow[sh_cgf[s] + c];
+          g[0] += tv * gr[0];
+          g[1] += tv * gr[1];
+          g[2] += tv * gr[2];
+        }
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      G[d] += g[d];
+      const double t = wave_sum(g[d]);
+      if (threadIdx.x == 0) part[((long)A * 3 + d) * nblk + blockIdx.x] = -4.0 * t;
+    }
+  }
+  if (valid)
+    for (int d = 0; d < 3; ++d) mot[d * npts + p] = 4.0 * G[d];
+}
+
+// Becke-weight term: sum_p w0_p a_p d wb_p / d R_B (w0 = weight without the
+// Becke factor), one wave per 64 points with the atom distances and the cell
+// functions P_n in LDS.  The B != owner terms are added to part[B][d][blk]
+// (after k_force_basis wrote it), the owner's share to mot[d][p].
+// d wb/d R_B = (dP_a/dR_B)/Z - (wb/Z) sum_n dP_n/dR_B for B != a, and
+// d wb/d R_a = -sum_{B != a} d wb/d R_B (translating everything leaves wb fixed).
+// dP_n/dR_B = P_n sum_m (s'/s)(mu_nm) d mu_nm/d R_B with
+// d mu_nm/d R_n = -(u_n + mu_nm e_nm)/R_nm, d mu_nm/d R_m = (u_m + mu_nm e_nm)/R_nm,
+// u_k = (r - R_k)/|r - R_k|, e_nm = (R_n - R_m)/R_nm.
+__device__ __forceinline__ void cutoff_and_ratio(double mu, double& s, double& ratio) {
+  // s = (1 - f(f(f(mu))))/2, s' = -f'(f2) f'(f1) f'(mu)/2, f'(x) = 1.5 - 1.5 x^2
+  const double f1 = 1.5 * mu - 0.5 * mu * mu * mu;
+  const double f2 = 1.5 * f1 - 0.5 * f1 * f1 * f1;
+  const double f3 = 1.5 * f2 - 0.5 * f2 * f2 * f2;
+  s = 0.5 * (1.0 - f3);
+  const double sp = -0.5 * (1.5 - 1.5 * f2 * f2) * (1.5 - 1.5 * f1 * f1) * (1.5 - 1.5 * mu * mu);
+  ratio = (s > 0.0) ? sp / s : 0.0;
+}
+
+__global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ atoms,
+                                                    const double* __restrict__ invR, int natoms,
+                                                    long npts, long cell, int nrad, int nang,
+                                                    const double* __restrict__ rad,
+                                                    const double* __restrict__ wrad,
+                                                    const double* __restrict__ leb,
+                                                    const double* __restrict__ px,
+                                                    const double* __restrict__ py,
+                                                    const double* __restrict__ pz,
+                                                    const double* __restrict__ ab, long nblk,
+                                                    double* part, double* mot) {
+  extern __shared__ double sh[];
+  double* dist = sh;                 // natoms x 64
+  double* Pn = sh + natoms * 64;     // natoms x 64
+  const int lane = threadIdx.x;
+  const long p = (long)blockIdx.x * 64 + lane;
+  const bool valid = p < npts;
+  const double x = valid ? px[p] : 0.0, y = valid ? py[p] : 0.0, z = valid ? pz[p] : 0.0;
+  for (int m = 0; m < natoms; ++m) {
+    const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
+    dist[m * 64 + lane] = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
+  }
+  double Z = 0.0;
+  for (int n = 0; n < natoms; ++n) {
+    double prod = 1.0;
+    for (int m = 0; m < natoms; ++m) {
+      if (m == n) continue;
+      double sv, rt;
+      cutoff_and_ratio((dist[n * 64 + lane] - dist[m * 64 + lane]) * invR[n * natoms + m], sv, rt);
+      prod *= sv;
+    }
+    Pn[n * 64 + lane] = prod;
+    Z += prod;
+  }
+  const int own = valid ? (int)(p / cell) : 0;
+  double pref = 0.0;  // w0 a_p / Z
+  if (valid && Z > 0.0) {
+    const int j = (int)(p % nang);
+    const int i = (int)((p / nang) % nrad);
+    const double ox = leb[4 * j] * rad[i], oy = leb[4 * j + 1] * rad[i], oz = leb[4 * j + 2] * rad[i];
+    const double w0 = wrad[i] * leb[4 * j + 3] * (ox * ox + oy * oy + oz * oz) * 4.0 * kPiD;
+    pref = w0 * ab[p] / Z;
+  }
+  const double wbv = (valid && Z > 0.0) ? Pn[own * 64 + lane] / Z : 0.0;
+  double own_acc[3] = {0.0, 0.0, 0.0};
+  for (int B = 0; B < natoms; ++B) {
+    double gB[3] = {0.0, 0.0, 0.0};
+    if (valid && B != own && pref != 0.0) {
+      const double dB = dist[B * 64 + lane];
+      const double uB[3] = {(x - atoms[4 * B]) / dB, (y - atoms[4 * B + 1]) / dB, (z - atoms[4 * B + 2]) / dB};
+      double sumdP[3] = {0.0, 0.0, 0.0}, dPa[3] = {0.0, 0.0, 0.0};
+      // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) * (-(u_B + mu_Bm e_Bm)/R_Bm)
+      double acc[3] = {0.0, 0.0, 0.0};
+      for (int m = 0; m < natoms; ++m) {
+        if (m == B) continue;
+        const double iR = invR[B * natoms + m];
+        const double mu = (dB - dist[m * 64 + lane]) * iR;
+        double sv, rt;
+        cutoff_and_ratio(mu, sv, rt);
+        const double e[3] = {(atoms[4 * B] - atoms[4 * m]) * iR, (atoms[4 * B + 1] - atoms[4 * m + 1]) * iR,
+                             (atoms[4 * B + 2] - atoms[4 * m + 2]) * iR};
+        for (int d = 0; d < 3; ++d) acc[d] += rt * (-(uB[d] + mu * e[d]) * iR);
+      }
+      const double PB = Pn[B * 64 + lane];
+      for (int d = 0; d < 3; ++d) sumdP[d] = PB * acc[d];
+      // n != B: dP_n/dR_B = P_n ratio(mu_nB) (u_B + mu_nB e_nB)/R_nB
+      for (int n = 0; n < natoms; ++n) {
+        if (n == B) continue;
+        const double iR = invR[n * natoms + B];
+        const double mu = (dist[n * 64 + lane] - dB) * iR;
+        double sv, rt;
+        cutoff_and_ratio(mu, sv, rt);
+        const double e[3] = {(atoms[4 * n] - atoms[4 * B]) * iR, (atoms[4 * n + 1] - atoms[4 * B + 1]) * iR,
+                             (atoms[4 * n + 2] - atoms[4 * B + 2]) * iR};
+        const double f = Pn[n * 64 + lane] * rt * iR;
+        for (int d = 0; d < 3; ++d) {
+          const double t = f * (uB[d] + mu * e[d]);
+          sumdP[d] += t;
+          if (n == own) dPa[d] = t;
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        gB[d] = pref * (dPa[d] - wbv * sumdP[d]);
+        own_acc[d] -= gB[d];
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      const double t = wave_sum(gB[d]);
+      if (lane == 0) part[((long)B * 3 + d) * nblk + blockIdx.x] += t;
+    }
+  }
+  // the owner atom's share (translation invariance), per point
+  if (valid)
+    for (int d = 0; d < 3; ++d) mot[d * npts + p] += own_acc[d];
+}
+
+// part[blk] = sum over the block's 256 points of x*y (fixed tree order).
+__global__ __launch_bounds__(256) void k_dot_parts(const double* __restrict__ x,
+                                                   const double* __restrict__ y, long n, double* part) {
+  __shared__ double red[256];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  red[threadIdx.x] = p < n ? x[p] * y[p] : 0.0;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[r] = sum_c in[r * len + c], one block per row, fixed order.
+__global__ __launch_bounds__(1024) void k_row_sum(const double* __restrict__ in, long len, double* out) {
+  __shared__ double red[1024];
+  const double* src = in + (long)blockIdx.x * len;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < len; i += 1024) s += src[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Per-point factors of the adjoint, k = lam/Z (0 with c held), c = Z / scalars[1]:
+// ub = c w (v - k), ab = e - k rho, udp = 2 c w (v - k) (the dE/dP weights);
+// uxc holds w v and exd the energy density e.
+__global__ void k_adjoint_weights(const double* __restrict__ uxc, const double* __restrict__ w,
+                                  const double* __restrict__ rho, const double* __restrict__ exd,
+                                  const double* __restrict__ scal, const double* __restrict__ lam,
+                                  double zsum, int hold_c, long n, double* ub, double* ab, double* udp) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double c = zsum / scal[1];
+  const double k = hold_c ? 0.0 : lam[0] / zsum;
+  const double wv = uxc[p] - k * w[p];
+  ub[p] = c * wv;
+  ab[p] = exd[p] - k * rho[p];
+  udp[p] = 2.0 * c * wv;
+}
+
 // Phi split for the reduced-precision contractions: hi = bf16(phi),
 // lo = bf16(phi - hi), stored point-blocked as [p/32][col][32] so that the 8
 // consecutive points a lane feeds to one BF16 MFMA are one 16-byte load.
@@ -1147,8 +1448,13 @@ struct dg_handle {
   size_t cross_lds_mx = 0;
   double* ej_part = nullptr;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
+  // adjoint of E_xc (allocated on its first use)
+  double *adj_ub = nullptr, *adj_ab = nullptr, *adj_udp = nullptr, *adj_T = nullptr, *adj_frag = nullptr;
+  double *adj_part = nullptr, *adj_mot = nullptr, *adj_lam = nullptr, *adj_out = nullptr, *adj_dP = nullptr;
+  int *ash_start = nullptr, *ash_list = nullptr;
   // state
-  bool has_density = false, has_hartree = false, has_xc = false;
+  bool has_density = false, has_hartree = false, has_xc = false, corrected = false;
+  const double* last_P = nullptr;  // device density matrix of the last density evaluation
   // profiling
   bool prof = false;
   std::vector<ProfEntry> pending;
@@ -1159,10 +1465,11 @@ struct dg_handle {
     double* bufs[] = {atoms, invR,  rad,   wrad,   leb,   px,     py,    pz,   w,     wb,
                       phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
                       V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, ylmT, minvT, shc,
-                      knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC};
+                      knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC,
+                      adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups};
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups, ash_start, ash_list};
     if (phh) (void)hipFree(phh);
     if (phl) (void)hipFree(phl);
     if (ej_part) (void)hipFree(ej_part);
@@ -1241,6 +1548,36 @@ int launch_density(dg_handle* h) {
   });
 }
 
+template <int NT>
+void launch_tmat_nt(dg_handle* h) {
+  hipLaunchKernelGGL(k_tmat<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi, h->ldphi,
+                     h->adj_frag, h->adj_T);
+}
+
+// T = Phi P for the force terms (P = the last density matrix, full fragments).
+int launch_tmat(dg_handle* h) {
+  DG_TRY(run(h, "adj_pfrag", [&] {
+    const long nf = 256L * h->ntb * h->ntb;
+    hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(nf, 256)), dim3(256), 0, h->stream, h->last_P, h->nbf, h->ntb,
+                       h->adj_frag);
+  }));
+  return run(h, "adj_tmat", [&] {
+    switch (h->ntb) {
+#define DG_NT(N)          \
+  case N:                 \
+    launch_tmat_nt<N>(h); \
+    break;
+      DG_NT(1) DG_NT(2) DG_NT(3) DG_NT(4) DG_NT(5) DG_NT(6) DG_NT(7) DG_NT(8) DG_NT(9) DG_NT(10) DG_NT(11)
+      DG_NT(12) DG_NT(13) DG_NT(14) DG_NT(15) DG_NT(16) DG_NT(17) DG_NT(18) DG_NT(19) DG_NT(20) DG_NT(21)
+      DG_NT(22) DG_NT(23) DG_NT(24) DG_NT(25) DG_NT(26) DG_NT(27) DG_NT(28) DG_NT(29) DG_NT(30) DG_NT(31)
+      DG_NT(32)
+#undef DG_NT
+      default:
+        break;
+    }
+  });
+}
+
 int stage_density(dg_handle* h, const double* dP) {
   const long nfrag = 128L * h->ntb * (h->ntb + 1);
   DG_TRY(run(h, "pfrag", [&] {
@@ -1248,6 +1585,8 @@ int stage_density(dg_handle* h, const double* dP) {
                        h->pfrag);
   }));
   DG_TRY(launch_density(h));
+  h->last_P = dP;
+  h->corrected = false;
   h->has_density = true;
   h->has_hartree = h->has_xc = false;
   return DG_OK;
@@ -1261,6 +1600,7 @@ int stage_correct(dg_handle* h) {
     hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
                        h->npts_pad, h->q, h->natoms, h->zsum, h->qc, h->scalars);
   }));
+  h->corrected = true;
   return DG_OK;
 }
 
@@ -1915,6 +2255,90 @@ int dg_get_xc_parts(dg_handle* h, double* exd, double* wv) {
   if (exd) DG_TRY(copy_out(h, exd, h->exd, h->npts));
   if (wv) DG_TRY(copy_out(h, wv, h->uxc, h->npts));
   return DG_OK;
+}
+
+int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
+  if (!h) return fail(DG_ERR_INVALID, "dg_xc_adjoint: null handle");
+  if (!h->has_xc || !h->corrected || !h->last_P)
+    return fail(DG_ERR_STATE, "dg_xc_adjoint: needs a corrected density and dg_xc (or a build) first");
+  const long np = h->npts_pad, nblk = np / 64;
+  const int na = h->natoms;
+  const size_t becke_lds = (size_t)2 * na * 64 * sizeof(double);
+  if (grad && becke_lds > 150 * 1024)
+    return fail(DG_ERR_UNSUPPORTED, "dg_xc_adjoint: too many atoms for the Becke-gradient kernel's LDS");
+  if (!h->adj_ub) {
+    for (double** b : {&h->adj_ub, &h->adj_ab, &h->adj_udp}) DG_TRY(dalloc(b, np));
+    DG_TRY(dalloc(&h->adj_lam, np / 256 + 1));
+    DG_TRY(dalloc(&h->adj_dP, (size_t)h->nbf * h->nbf));
+    DG_TRY(dalloc(&h->adj_out, (size_t)6 * na));
+  }
+  if (grad && !h->adj_T) {
+    DG_TRY(dalloc(&h->adj_T, (size_t)np * h->ldphi));
+    DG_TRY(dalloc(&h->adj_frag, (size_t)256 * h->ntb * h->ntb));
+    DG_TRY(dalloc(&h->adj_part, (size_t)3 * na * nblk));
+    DG_TRY(dalloc(&h->adj_mot, (size_t)3 * h->npts));
+    // shells grouped by centre atom
+    std::vector<int> at(h->nshell), start(na + 1, 0), list;
+    HIP_TRY(hipMemcpyAsync(at.data(), h->sh_atom, h->nshell * 4, hipMemcpyDeviceToHost, h->stream));
+    DG_TRY(sync(h));
+    for (int a = 0; a < na; ++a) {
+      start[a] = (int)list.size();
+      for (int s = 0; s < h->nshell; ++s)
+        if (at[s] == a) list.push_back(s);
+    }
+    start[na] = (int)list.size();
+    DG_TRY(dalloc(&h->ash_start, na + 1));
+    DG_TRY(dalloc(&h->ash_list, std::max<size_t>(list.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(h->ash_start, start.data(), (na + 1) * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->ash_list, list.data(), list.size() * 4, hipMemcpyHostToDevice, h->stream));
+    if (becke_lds > 64 * 1024)
+      HIP_TRY(hipFuncSetAttribute((const void*)k_force_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)becke_lds));
+    DG_TRY(sync(h));
+  }
+  // Lambda = sum w v rho (uxc = w v), then the per-point factors
+  const long nparts = np / 256;
+  DG_TRY(run(h, "adj_lambda", [&] {
+    hipLaunchKernelGGL(k_dot_parts, dim3(nparts), dim3(256), 0, h->stream, h->uxc, h->rho, h->npts, h->adj_lam + 1);
+    hipLaunchKernelGGL(k_row_sum, dim3(1), dim3(1024), 0, h->stream, h->adj_lam + 1, nparts, h->adj_lam);
+  }));
+  DG_TRY(run(h, "adj_weights", [&] {
+    hipLaunchKernelGGL(k_adjoint_weights, dim3(nblocks(np, 256)), dim3(256), 0, h->stream, h->uxc, h->w, h->rho,
+                       h->exd, h->scalars, h->adj_lam, h->zsum, hold_c ? 1 : 0, np, h->adj_ub, h->adj_ab,
+                       h->adj_udp);
+  }));
+  if (dEdP) {
+    // dE/dP = Phi^T diag(udp) Phi with the FP64 contraction in either precision mode
+    const int saved = h->precision;
+    h->precision = DG_PREC_FP64;
+    const int rc = stage_contract(h, h->adj_udp, nullptr, h->adj_dP, nullptr);
+    h->precision = saved;
+    DG_TRY(rc);
+    DG_TRY(copy_out(h, dEdP, h->adj_dP, (size_t)h->nbf * h->nbf));
+  }
+  if (grad) {
+    DG_TRY(launch_tmat(h));
+    DG_TRY(run(h, "adj_force_basis", [&] {
+      hipLaunchKernelGGL(k_force_basis, dim3(nblk), dim3(64), 0, h->stream, h->atoms, na, h->ash_start, h->ash_list,
+                         h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->pr_cn0, h->pr_cn1, h->npts,
+                         h->px, h->py, h->pz, h->ldphi, h->adj_T, h->adj_ub, nblk, h->adj_part, h->adj_mot);
+    }));
+    DG_TRY(run(h, "adj_force_becke", [&] {
+      hipLaunchKernelGGL(k_force_becke, dim3(nblk), dim3(64), becke_lds, h->stream, h->atoms, h->invR, na, h->npts,
+                         h->cell, h->nrad, h->nang, h->rad, h->wrad, h->leb, h->px, h->py, h->pz, h->adj_ab, nblk,
+                         h->adj_part, h->adj_mot);
+    }));
+    DG_TRY(run(h, "adj_force_sum", [&] {
+      hipLaunchKernelGGL(k_row_sum, dim3(3 * na), dim3(1024), 0, h->stream, h->adj_part, nblk, h->adj_out);
+      hipLaunchKernelGGL(k_row_sum, dim3(3 * na), dim3(1024), 0, h->stream, h->adj_mot, h->cell,
+                         h->adj_out + 3 * na);
+    }));
+    std::vector<double> o((size_t)6 * na);
+    DG_TRY(copy_out(h, o.data(), h->adj_out, o.size()));
+    for (int a = 0; a < na; ++a)
+      for (int d = 0; d < 3; ++d) grad[3 * a + d] = o[3 * a + d] + o[3 * na + d * na + a];
+  }
+  return sync(h);
 }
 
 int dg_profile_enable(dg_handle* h, int enable) {

@@ -29,7 +29,7 @@ EXPORTS = (
     "dg_set_stream", "dg_get_stream", "dg_set_density", "dg_correct_densities",
     "dg_calculate_density", "dg_hartree", "dg_xc", "dg_get_weights", "dg_get_becke_weights",
     "dg_get_positions", "dg_get_densities", "dg_get_amplitudes", "dg_get_hartree_potential",
-    "dg_get_hartree_parts", "dg_get_xc_parts", "dg_build_device", "dg_profile_enable",
+    "dg_get_hartree_parts", "dg_get_xc_parts", "dg_xc_adjoint", "dg_build_device", "dg_profile_enable",
     "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_mfma_probe_bf16", "dg_scf_run", "dg_one_electron",
     "dg_eigh",
 )
@@ -100,6 +100,7 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p, _D]
         L.dg_get_hartree_parts.argtypes = [C.c_void_p, _D, _D, _D]
         L.dg_get_xc_parts.argtypes = [C.c_void_p, _D, _D]
+        L.dg_xc_adjoint.argtypes = [C.c_void_p, C.c_int, _D, _D]
         L.dg_build_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.dg_profile_read.argtypes = [C.c_void_p, C.c_int, C.c_char_p, _D, _I, _I]
@@ -307,6 +308,20 @@ class MolecularGrid:
         e, wv = np.empty(self.npts), np.empty(self.npts)
         _check(lib().dg_get_xc_parts(self.h, _d(e), _d(wv)))
         return e, wv
+
+    def xc_adjoint(self, hold_c=False, dEdP=True, grad=True):
+        """(dE_xc/dP [nbf, nbf], dE_xc/dR [natoms, 3]) of the last XC evaluation.
+
+        hold_c=True keeps the renormalisation factor c fixed (dE/dP = 2 c XC);
+        False differentiates through c as well.  Either output can be skipped.
+        """
+        self._need()
+        n = self.nbf
+        dP = np.empty((n, n)) if dEdP else None
+        g = np.empty((self.natoms, 3)) if grad else None
+        _check(lib().dg_xc_adjoint(self.h, 1 if hold_c else 0,
+                                   _d(dP) if dP is not None else None, _d(g) if g is not None else None))
+        return (dP.T.copy() if dP is not None else None), g
 
     # -- device-pointer build and profiling ---------------------------------
     def set_stream(self, stream_ptr):

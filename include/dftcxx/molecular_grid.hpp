@@ -99,6 +99,14 @@ class MolecularGrid {
     dg_throw(dg_xc(h_, XC.data(), exc));
     return XC;
   }
+  // Derivatives of the last E_xc (no counterpart in the reference): dE/dP
+  // (column-major nbf x nbf) and dE/dR (atom-major xyz); hold_c keeps the
+  // renormalisation factor fixed.  Either pointer may be null.
+  void xc_adjoint(bool hold_c, std::vector<double>* dEdP, std::vector<double>* grad) {
+    if (dEdP) dEdP->assign((size_t)nbf_ * nbf_, 0.0);
+    if (grad) grad->assign((size_t)3 * mol_.natoms(), 0.0);
+    dg_throw(dg_xc_adjoint(h_, hold_c ? 1 : 0, dEdP ? dEdP->data() : nullptr, grad ? grad->data() : nullptr));
+  }
   // src/moleculargrid.h:146-162
   std::vector<double> get_weights() { return vec(dg_get_weights, npts_); }
   std::vector<double> get_densities() { return vec(dg_get_densities, npts_); }

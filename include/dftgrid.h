@@ -112,12 +112,27 @@ int dg_get_hartree_parts(dg_handle* h, double* rho_lm, double* U_lm, double* vce
 /* Pointwise XC after dg_xc: energy density e_x+e_c and w*(v_xa+v_xb+v_ca+v_cb)/2 [npts]. */
 int dg_get_xc_parts(dg_handle* h, double* exc_density, double* wv);
 
+/* --- adjoint of E_xc ------------------------------------------------------ */
+/* No counterpart in the reference: its GTO gradient (src/cgf.cpp:67-94,
+ * 164-172) is never called on this path.  Differentiates the E_xc of the
+ * last dg_xc (or build), E_xc = sum_p w_p e(c rho_raw_p) with the
+ * renormalisation c = sum Z / sum_p w_p rho_raw_p of src/moleculargrid.cpp:132-146.
+ *   hold_c = 1: c is a constant, and dEdP = 2 c XC;
+ *   hold_c = 0: c varies with P and R (the derivative of the E_xc the SCF reports).
+ * dEdP [nbf*nbf] column-major; grad [natoms*3] = dE_xc/dR_A (atom-major xyz;
+ * the force is -grad), including the basis-centre, grid-motion and Becke-weight
+ * terms.  Needs a corrected density (dg_correct_densities or a build) and the
+ * density matrix of that evaluation still in the device buffer it was read from.
+ * Either output may be NULL.  Returns DG_ERR_STATE before dg_xc. */
+int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad);
+
 /* --- device-pointer API --------------------------------------------------- */
 /* One full grid build for one SCF iteration, enqueued on the handle's stream:
  * set_density(P) + correct_densities() + calculate_hartree_potential() +
  * the XC assembly.  dP, dJ, dXC are device arrays nbf*nbf (column-major);
  * dscalars (device, 4 doubles) receives {E_xc, N_elec before correction,
- * N_elec after correction, 0}.  Any output pointer may be NULL. */
+ * N_elec after correction, E_J = 2 tr(P J) evaluated pointwise as
+ * sum_p w_p V_p rho_p / (2c)}.  Any output pointer may be NULL. */
 int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, double* dscalars);
 
 /* --- profiling ------------------------------------------------------------- */

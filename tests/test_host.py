@@ -187,3 +187,18 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_cpp_facade_compiles(tmp_path):
+    # the reference-shaped C++ facade is header-only over the C ABI
+    src = tmp_path / "use_facade.cpp"
+    src.write_text('#include "dftcxx/molecular_grid.hpp"\n'
+                   "int main() { dftcxx::MoleculeData m; dftcxx::MolecularGrid g(m);\n"
+                   "  std::vector<double> dP, gr; (void)&dftcxx::MolecularGrid::xc_adjoint;\n"
+                   "  return dg_version() >= 100 ? 0 : 1; }\n")
+    exe = tmp_path / "use_facade"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        str(src), "-o", str(exe), "-L", os.path.dirname(G.LIB_PATH), "-ldftgrid",
+                        "-Wl,-rpath," + os.path.dirname(G.LIB_PATH)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert subprocess.run([str(exe)]).returncode == 0
