@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF energy / s-per-iteration runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip timing dE_xc/dP and dE_xc/dR")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM-counter calibration)")
     args = ap.parse_args()
@@ -199,6 +200,23 @@ def main():
                 t = json.load(f)
             roof["traffic"] = t.get(dom)
         out["roofline"] = roof
+
+    if not args.no_adjoint:
+        # dE_xc/dP and dE_xc/dR of the build above (c varying), second call timed
+        grid.build_device(*ptrs)
+        torch.cuda.synchronize()
+        grid.xc_adjoint(hold_c=False)
+        grid.profile_enable(True)
+        grid.profile_reset()
+        t0 = time.perf_counter()
+        _, grad = grid.xc_adjoint(hold_c=False)
+        adj_s = time.perf_counter() - t0
+        prof = grid.profile_read()
+        grid.profile_enable(False)
+        out["adjoint"] = {"ms": 1e3 * adj_s,
+                          "kernels_ms": {k: ms for k, (ms, _) in sorted(prof.items())},
+                          "max_abs_grad": float(np.abs(grad).max()),
+                          "max_abs_sum_grad": float(np.abs(grad.sum(0)).max())}
 
     if not args.no_mixed and prec == G.PREC_FP64:
         # the same build with BF16-split Fock contractions and FP32 high-l Hartree terms

@@ -66,11 +66,12 @@ def test_adjoint_identities_benzene_full_grid():
     S = g2.correct_densities()
     c = float(np.sum(mol.Z)) / S
     assert np.abs(dP_c - 2 * c * XC).max() < 1e-12 * np.abs(XC).max()
-    # c variable differs by a multiple of the grid overlap
+    # c variable: dE/dP gains -2 c (Lambda / Z) S_grid, Lambda = sum w v rho
     Sg = g.get_amplitudes().T @ (g.get_weights()[:, None] * g.get_amplitudes())
-    coef = ((dP_c - dP) * Sg).sum() / (Sg * Sg).sum()
-    assert np.abs(dP_c - dP - coef * Sg).max() < 1e-10 * np.abs(dP).max()
-    assert coef > 0
+    _, wv = g.get_xc_parts()
+    lam = float(wv @ g.get_densities())
+    expect = 2 * c * lam / float(np.sum(mol.Z))
+    assert np.abs(dP_c - dP - expect * Sg).max() < 1e-10 * np.abs(dP).max()
     for gr in (grad, grad_c):
         assert np.abs(gr.sum(0)).max() < 1e-9 * np.abs(gr).max()
     # symmetric-equivalent carbons carry gradients of equal length
@@ -116,3 +117,35 @@ def test_adjoint_needs_xc_first():
     g.set_density(_density(mol))
     with pytest.raises(G.DgError):
         g.xc_adjoint()  # density not yet corrected / no XC
+
+
+def test_c10h22_forces_and_dEdP_full_grid():
+    # decane, 6-31G, 32 atoms x 75 x 302: forces against central differences
+    # (h = 1e-4 bohr) within 1e-5 Ha/bohr, and dE/dP|_c = 2 c V_xc
+    mol = M.alkane(10)
+    P = _density(mol)
+    nrad, nang, lmax = 75, 302, 8
+    print("[case] adjoint C10H22 6-31G 75x302", flush=True)
+    g, XC, exc, dP_c, _ = _lib_adjoint(mol, nrad, nang, lmax, P, True)
+    dP, grad = g.xc_adjoint(hold_c=False)
+    assert np.abs(grad.sum(0)).max() < 1e-9 * np.abs(grad).max()
+    g2 = G.MolecularGrid(mol, nrad, nang, lmax)
+    g2.set_density(P)
+    c = float(np.sum(mol.Z)) / g2.correct_densities()
+    del g2
+    assert np.abs(dP_c - 2 * c * XC).max() < 1e-12 * np.abs(XC).max()
+    h = 1e-4
+    # a terminal carbon, a middle carbon and a hydrogen, one component each
+    for a, d in ((0, 0), (4, 1), (12, 2)):
+        e = []
+        for sgn in (1, -1):
+            m2 = M.Molecule(mol.Z, mol.xyz.copy(), mol.basis, mol.name)
+            m2.xyz[a, d] += sgn * h
+            gd = G.MolecularGrid(m2, nrad, nang, lmax)
+            gd.set_density(P)
+            gd.correct_densities()
+            e.append(gd.calculate_exchange_correlation()[1])
+            del gd
+        fd = (e[0] - e[1]) / (2 * h)
+        print(f"  atom {a} dir {d}: analytic {grad[a, d]:.10f} fd {fd:.10f}", flush=True)
+        assert abs(fd - grad[a, d]) < 1e-5, (a, d, fd, grad[a, d])
