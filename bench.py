@@ -56,6 +56,71 @@ def cpu_baseline(mol, nrad, nang, lmax, P):
     return dt, threads
 
 
+def bench_batch(args, G, M, np, torch):
+    """256 seeded small molecules (STO-3G / 6-31G), 75 x 302, one graph launch per step."""
+    mols = M.batch()
+    nrad, nang, lmax = 75, 302, 8
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    b = G.Batch(mols, nrad, nang, lmax)
+    setup = time.perf_counter() - t0
+    Ps = []
+    for m in mols:
+        S, T, V = G.one_electron(m)
+        w, U = np.linalg.eigh(S)
+        X = U / np.sqrt(w)
+        _, Cc = np.linalg.eigh(X.T @ (T + V) @ X)
+        C = X @ Cc
+        Ps.append(C[:, :m.nelec // 2] @ C[:, :m.nelec // 2].T)
+    stream = torch.cuda.Stream(device=dev)
+    b.set_stream(stream.cuda_stream)
+    dP = torch.tensor(b.pack(Ps), dtype=torch.float64, device=dev)
+    dJ = torch.empty(b.packed, dtype=torch.float64, device=dev)
+    dX = torch.empty(b.packed, dtype=torch.float64, device=dev)
+    dS = torch.empty(4 * b.nmol, dtype=torch.float64, device=dev)
+    ptrs = (dP.data_ptr(), dJ.data_ptr(), dX.data_ptr(), dS.data_ptr())
+    torch.cuda.synchronize()
+    b.build_device(*ptrs)
+    torch.cuda.synchronize()
+    tuned = b.autotune()
+    for _ in range(args.warmup):
+        b.build_device(*ptrs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.build_device(*ptrs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    units = float(sum(int(npt) * int(n) ** 2 for npt, n in zip(np.diff(b.point0), b.nbf)))
+    out = {
+        "metric": "grid V_xc+J build throughput (N_pts*N_bf^2 per second)",
+        "value": units * args.steps / wall, "unit": "pts*bf^2/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded geometries, core-Hamiltonian guess P)",
+        "config": {"workload": "batch256", "molecules": b.nmol, "n_pts": b.npts, "n_radial": nrad,
+                   "n_angular": nang, "lmax": lmax, "graph": True},
+        "grid_setup_s": setup,
+        "autotune": {"contract_chunk": tuned[0], "cross_item": tuned[1], "ms": list(map(float, tuned[2]))},
+    }
+    if not args.no_profile:
+        b.profile_enable(True)
+        b.profile_reset()
+        for _ in range(3):
+            b.build_device(*ptrs)
+        torch.cuda.synchronize()
+        prof = b.profile_read()
+        b.profile_enable(False)
+        out["kernels_ms_per_step"] = {k: ms / 3 for k, (ms, _) in sorted(prof.items())}
+    if not args.no_scf:
+        res, its = G.batch_scf_run(mols, nrad, nang, lmax, threshold=1e-6)
+        out["scf"] = {"batch_iterations": its, "s_per_iter": res[0]["seconds_iter"],
+                      "setup_s": res[0]["seconds_setup"],
+                      "max_molecule_iterations": max(r["iterations"] for r in res)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +158,8 @@ def main():
     from dftcxx_amd import gridlib as G
     from dftcxx_amd import molecule as M
 
+    if args.workload == "batch256":
+        return bench_batch(args, G, M, np, torch)
     make, nrad, nang = M.WORKLOADS[args.workload]
     lmax = 8
     mol = make()

@@ -136,17 +136,13 @@ __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms,
 // primitives share one exponential per point; the CGF value is
 // sum_k c_k N_k x^l y^m z^n exp(-alpha_k r^2) (src/cgf.cpp:49-57, 146-154).
 // cn0 = c N for (L,0,0)-type components, cn1 = c N for mixed d components.
-__global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* __restrict__ sh_atom,
-                      const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
-                      const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
-                      const double* __restrict__ alpha, const double* __restrict__ cn0,
-                      const double* __restrict__ cn1, long npts, const double* __restrict__ px,
-                      const double* __restrict__ py, const double* __restrict__ pz, int ldphi,
-                      double* phi) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= npts * nshell) return;
-  const long p = g / nshell;
-  const int s = (int)(g % nshell);
+__device__ __forceinline__ void phi_shell(const double* __restrict__ atoms, const int* __restrict__ sh_atom,
+                                          const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
+                                          const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
+                                          const double* __restrict__ alpha, const double* __restrict__ cn0,
+                                          const double* __restrict__ cn1, const double* __restrict__ px,
+                                          const double* __restrict__ py, const double* __restrict__ pz,
+                                          int ldphi, double* phi, long p, int s) {
   const int a = sh_atom[s];
   const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
   const double r2 = dx * dx + dy * dy + dz * dz;
@@ -178,6 +174,19 @@ __global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* _
   }
 }
 
+__global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* __restrict__ sh_atom,
+                      const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
+                      const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
+                      const double* __restrict__ alpha, const double* __restrict__ cn0,
+                      const double* __restrict__ cn1, long npts, const double* __restrict__ px,
+                      const double* __restrict__ py, const double* __restrict__ pz, int ldphi,
+                      double* phi) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts * nshell) return;
+  phi_shell(atoms, sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, alpha, cn0, cn1, px, py, pz, ldphi, phi, g / nshell,
+            (int)(g % nshell));
+}
+
 // ---------------------------------------------------------------------------
 // Density: rho_p = 2 phi_p^T P phi_p  (src/gridpoint.cpp:82-84)
 // ---------------------------------------------------------------------------
@@ -189,9 +198,7 @@ __global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* _
 // P'(16b + lane%16, kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4, stored from
 // offset 128 b (b+1).  The same permutation of the reduction index is used
 // for phi in k_density, so each result lands on the lanes that hold phi.
-__global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* frag) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 128L * nt * (nt + 1)) return;
+__device__ __forceinline__ double pfrag_value(const double* __restrict__ P, int nbf, long e) {
   int b = 0;
   while (128L * (b + 1) * (b + 2) <= e) ++b;
   const int rem = (int)(e - 128L * b * (b + 1));
@@ -199,7 +206,13 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
   const int j = 16 * b + (l & 15);
   const int k = 16 * (s / 4) + 4 * (l >> 4) + (s & 3);
   const double f = (s / 4 < b) ? 2.0 : 1.0;
-  frag[e] = (j < nbf && k < nbf) ? f * P[j + (long)k * nbf] : 0.0;
+  return (j < nbf && k < nbf) ? f * P[j + (long)k * nbf] : 0.0;
+}
+
+__global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* frag) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 128L * nt * (nt + 1)) return;
+  frag[e] = pfrag_value(P, nbf, e);
 }
 
 // One wave handles 16 points; lane (c = lane%16, g = lane/16) keeps
@@ -504,8 +517,8 @@ struct CrossArgs {
   const double* tab;    // [natoms][nrad][nlm]{y, D}
   const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
   const int* items;     // {k, p_begin, p_end} point ranges outside cell k, k ascending
-  double* vpart;        // [natoms][npts_pad]
-  long npts_pad;
+  double* vpart;        // partial potentials, atom k's at vpart + vbase[k] + p
+  const long* vbase;
   int nitems, nrad, nlm, l0, l1, accumulate;
 };
 
@@ -637,7 +650,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       const double d = hi_ * hi_ * (-2.0 * dy + sd0 + sd1);
       val = sy0 + sd0 * t + c * t * t + d * t * t * t;
     }
-    double* out = A.vpart + (long)k * A.npts_pad + p;
+    double* out = A.vpart + A.vbase[k] + p;
     if (A.accumulate)
       *out += val * rinv;
     else
@@ -646,157 +659,32 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   }
 }
 
-// Reduced-precision variant of k_hartree_cross: geometry, interval search,
-// the l = 0 term and the final cubic in FP64; the l >= 1 harmonics, their
-// spline table (FP32 copy in LDS) and the four lm-sums in FP32.  The whole
-// l range is one pass (the FP32 table is half the size).
-template <int LMAX>
-__global__ __launch_bounds__(1024) void k_hartree_cross_mx(CrossArgs A) {
-  extern __shared__ double lds[];
-  const int n = A.nrad, nlm = A.nlm, nh = nlm - 1;
-  double* kn = lds;
-  double* ih = lds + n;
-  double* t00 = lds + 2 * n;             // n x {y, D} of lm = 0, FP64
-  float* tf = (float*)(lds + 4 * n);     // n x nh x {y, D}, FP32
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    kn[i] = A.knots[i];
-    ih[i] = A.invh[i];
-  }
-  const int it0 = (int)((long)blockIdx.x * A.nitems / gridDim.x);
-  const int it1 = (int)((long)(blockIdx.x + 1) * A.nitems / gridDim.x);
-  const double y00 = A.shc[0];
-  int cur = -1;
-  for (int it = it0; it < it1; ++it) {
-    const int k = A.items[3 * it];
-    if (k != cur) {
-      __syncthreads();
-      const double* src = A.tab + (long)k * n * nlm * 2;
-      for (int i = threadIdx.x; i < n * nlm * 2; i += blockDim.x) {
-        const int t = i / (nlm * 2), rem = i % (nlm * 2);
-        if (rem < 2)
-          t00[2 * t + rem] = src[i];
-        else
-          tf[t * nh * 2 + rem - 2] = (float)src[i];
-      }
-      __syncthreads();
-      cur = k;
-    }
-    const long pb = A.items[3 * it + 1], pe = A.items[3 * it + 2];
-    const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
-    const double xlo = kn[0], xhi = kn[n - 1];
-    for (long p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-      const double dx = A.px[p] - Ax, dy = A.py[p] - Ay, dz = A.pz[p] - Az;
-      const double rxy2 = dx * dx + dy * dy;
-      const double r = sqrt(rxy2 + dz * dz);
-      const double rinv = 1.0 / r;
-      const double rxy = sqrt(rxy2);
-      const float ct = (float)(dz * rinv), st = (float)(rxy * rinv);
-      float cp = 1.f, sp = 0.f;
-      if (rxy > 0.0) {
-        const double ir = 1.0 / rxy;
-        cp = (float)(dx * ir);
-        sp = (float)(dy * ir);
-      }
-      int s0, s1;
-      double t = 0.0;
-      bool clamp = true;
-      if (r < xlo) {
-        s0 = s1 = 0;
-      } else if (r >= xhi) {
-        s0 = s1 = n - 1;
-      } else {
-        int lo = 1, hi = n - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (r <= kn[mid])
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-        s1 = lo;
-        s0 = lo - 1;
-        t = r - kn[s0];
-        clamp = false;
-      }
-      const float* T0 = tf + s0 * nh * 2;
-      const float* T1 = tf + s1 * nh * 2;
-      float fy0 = 0.f, fd0 = 0.f, fy1 = 0.f, fd1 = 0.f;
-      float pmm = 1.f, cm = 1.f, sm = 0.f;
-#pragma unroll
-      for (int m = 0; m <= LMAX; ++m) {
-        if (m > 0) {
-          pmm *= -(float)(2 * m - 1) * st;
-          const float c2 = cm * cp - sm * sp;
-          sm = sm * cp + cm * sp;
-          cm = c2;
-        }
-        float p1 = 0.f, p2 = 0.f;
-#pragma unroll
-        for (int l = m; l <= LMAX; ++l) {
-          const int ci = 3 * (l * (l + 1) / 2 + m);
-          float pl;
-          if (l == m)
-            pl = pmm;
-          else if (l == m + 1)
-            pl = ct * (float)(2 * m + 1) * pmm;
-          else
-            pl = (float)A.shc[ci + 1] * ct * p1 - (float)A.shc[ci + 2] * p2;
-          p2 = p1;
-          p1 = pl;
-          if (l >= 1) {
-            const float base = (float)A.shc[ci] * pl;
-            const int ip = (l * l + l + m - 1) * 2;
-            const float yp = (m == 0) ? base : base * cm;
-            fy0 += yp * T0[ip];
-            fd0 += yp * T0[ip + 1];
-            fy1 += yp * T1[ip];
-            fd1 += yp * T1[ip + 1];
-            if (m > 0) {
-              const int in = (l * l + l - m - 1) * 2;
-              const float yn = base * sm;
-              fy0 += yn * T0[in];
-              fd0 += yn * T0[in + 1];
-              fy1 += yn * T1[in];
-              fd1 += yn * T1[in + 1];
-            }
-          }
-        }
-      }
-      const double sy0 = y00 * t00[2 * s0] + (double)fy0;
-      const double sd0 = y00 * t00[2 * s0 + 1] + (double)fd0;
-      const double sy1 = y00 * t00[2 * s1] + (double)fy1;
-      const double sd1 = y00 * t00[2 * s1 + 1] + (double)fd1;
-      double val;
-      if (clamp) {
-        val = sy0;
-      } else {
-        const double hi_ = ih[s1];
-        const double dyv = (sy1 - sy0) * hi_;
-        const double c = hi_ * (3.0 * dyv - 2.0 * sd0 - sd1);
-        const double d = hi_ * hi_ * (-2.0 * dyv + sd0 + sd1);
-        val = sy0 + sd0 * t + c * t * t + d * t * t * t;
-      }
-      A.vpart[(long)k * A.npts_pad + p] = val * rinv;
-    }
-  }
-}
-
-// V[p] = sum_k (k == own ? vcell[p] : vpart[k][p]), k ascending
-// (src/moleculargrid.cpp:347-378); uJ[p] = w[p] V[p] / 2 (src/atomicgrid.cpp:475-485);
-// ej_part[block] = sum w V rho over the block (fixed tree order).
+// V[p] = sum_k (k == own ? vcell[p] : vpart_k[p]) over the atoms k of the
+// point's molecule, k ascending (src/moleculargrid.cpp:347-378); uJ[p] = w[p] V[p] / 2
+// (src/atomicgrid.cpp:475-485); ej_part[block] = sum w V rho over the block (fixed
+// tree order).  atom_mol/mol_atom0 (NULL for one molecule) give the atom range.
 __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpart,
+                                                 const long* __restrict__ vbase,
                                                  const double* __restrict__ vcell,
                                                  const double* __restrict__ w,
                                                  const double* __restrict__ rho, long npts,
-                                                 long npts_pad, long cell, int natoms, double* V,
+                                                 long npts_pad, long cell, int natoms,
+                                                 const int* __restrict__ atom_mol,
+                                                 const int* __restrict__ mol_atom0, double* V,
                                                  double* uJ, double* ej_part) {
   __shared__ double red[256];
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (p < npts) {
     const int own = (int)(p / cell);
+    int k0 = 0, k1 = natoms;
+    if (atom_mol) {
+      const int m = atom_mol[own];
+      k0 = mol_atom0[m];
+      k1 = mol_atom0[m + 1];
+    }
     double v = 0.0;
-    for (int k = 0; k < natoms; ++k) v += (k == own) ? vcell[p] : vpart[(long)k * npts_pad + p];
+    for (int k = k0; k < k1; ++k) v += (k == own) ? vcell[p] : vpart[vbase[k] + p];
     V[p] = v;
     uJ[p] = (w[p] * v) * 0.5;
     e = w[p] * v * rho[p];
@@ -1047,6 +935,18 @@ __device__ __forceinline__ void cutoff_and_ratio(double mu, double& s, double& r
   ratio = (s > 0.0) ? sp / s : 0.0;
 }
 
+__device__ __forceinline__ double becke_s(double mu) {
+  for (int it = 0; it < 3; ++it) mu = 1.5 * mu - 0.5 * (mu * mu * mu);
+  return 0.5 * (1.0 - mu);
+}
+
+// Terms of an atom n whose cell function P_n is at most kBeckeSkip times
+// Z = sum_n P_n at the point are skipped.  Such a term is s'(mu) times the
+// other factors of P_n (at most 1); with |s'| <= 10.5 s^{7/8} (exact
+// arithmetic, the bound is approached as mu -> 1) it is at most
+// 10.5 (kBeckeSkip Z)^{7/8} ~ 3e-17 Z^{7/8}.
+constexpr double kBeckeSkip = 1e-20;
+
 __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ atoms,
                                                     const double* __restrict__ invR, int natoms,
                                                     long npts, long cell, int nrad, int nang,
@@ -1059,8 +959,9 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
                                                     const double* __restrict__ ab, long nblk,
                                                     double* part, double* mot) {
   extern __shared__ double sh[];
-  double* dist = sh;                 // natoms x 64
-  double* Pn = sh + natoms * 64;     // natoms x 64
+  double* dist = sh;                          // natoms x 64
+  double* Pn = sh + natoms * 64;              // natoms x 64
+  int* act = (int*)(sh + 2 * natoms * 64);    // natoms x 64: the point's atoms above the skip level
   const int lane = threadIdx.x;
   const long p = (long)blockIdx.x * 64 + lane;
   const bool valid = p < npts;
@@ -1074,13 +975,15 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
     double prod = 1.0;
     for (int m = 0; m < natoms; ++m) {
       if (m == n) continue;
-      double sv, rt;
-      cutoff_and_ratio((dist[n * 64 + lane] - dist[m * 64 + lane]) * invR[n * natoms + m], sv, rt);
-      prod *= sv;
+      prod *= becke_s((dist[n * 64 + lane] - dist[m * 64 + lane]) * invR[n * natoms + m]);
     }
     Pn[n * 64 + lane] = prod;
     Z += prod;
   }
+  const double skip = kBeckeSkip * Z;
+  int nact = 0;
+  for (int n = 0; n < natoms; ++n)
+    if (Pn[n * 64 + lane] > skip) act[(nact++) * 64 + lane] = n;
   const int own = valid ? (int)(p / cell) : 0;
   double pref = 0.0;  // w0 a_p / Z
   if (valid && Z > 0.0) {
@@ -1098,22 +1001,25 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
       const double dB = dist[B * 64 + lane];
       const double uB[3] = {(x - atoms[4 * B]) / dB, (y - atoms[4 * B + 1]) / dB, (z - atoms[4 * B + 2]) / dB};
       double sumdP[3] = {0.0, 0.0, 0.0}, dPa[3] = {0.0, 0.0, 0.0};
-      // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) * (-(u_B + mu_Bm e_Bm)/R_Bm)
-      double acc[3] = {0.0, 0.0, 0.0};
-      for (int m = 0; m < natoms; ++m) {
-        if (m == B) continue;
-        const double iR = invR[B * natoms + m];
-        const double mu = (dB - dist[m * 64 + lane]) * iR;
-        double sv, rt;
-        cutoff_and_ratio(mu, sv, rt);
-        const double e[3] = {(atoms[4 * B] - atoms[4 * m]) * iR, (atoms[4 * B + 1] - atoms[4 * m + 1]) * iR,
-                             (atoms[4 * B + 2] - atoms[4 * m + 2]) * iR};
-        for (int d = 0; d < 3; ++d) acc[d] += rt * (-(uB[d] + mu * e[d]) * iR);
-      }
       const double PB = Pn[B * 64 + lane];
-      for (int d = 0; d < 3; ++d) sumdP[d] = PB * acc[d];
+      if (PB > skip) {
+        // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) (-(u_B + mu_Bm e_Bm)/R_Bm)
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < natoms; ++m) {
+          if (m == B) continue;
+          const double iR = invR[B * natoms + m];
+          const double mu = (dB - dist[m * 64 + lane]) * iR;
+          double sv, rt;
+          cutoff_and_ratio(mu, sv, rt);
+          const double e[3] = {(atoms[4 * B] - atoms[4 * m]) * iR, (atoms[4 * B + 1] - atoms[4 * m + 1]) * iR,
+                               (atoms[4 * B + 2] - atoms[4 * m + 2]) * iR};
+          for (int d = 0; d < 3; ++d) acc[d] += rt * (-(uB[d] + mu * e[d]) * iR);
+        }
+        for (int d = 0; d < 3; ++d) sumdP[d] = PB * acc[d];
+      }
       // n != B: dP_n/dR_B = P_n ratio(mu_nB) (u_B + mu_nB e_nB)/R_nB
-      for (int n = 0; n < natoms; ++n) {
+      for (int t = 0; t < nact; ++t) {
+        const int n = act[t * 64 + lane];
         if (n == B) continue;
         const double iR = invR[n * natoms + B];
         const double mu = (dist[n * 64 + lane] - dB) * iR;
@@ -1123,9 +1029,9 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
                              (atoms[4 * n + 2] - atoms[4 * B + 2]) * iR};
         const double f = Pn[n * 64 + lane] * rt * iR;
         for (int d = 0; d < 3; ++d) {
-          const double t = f * (uB[d] + mu * e[d]);
-          sumdP[d] += t;
-          if (n == own) dPa[d] = t;
+          const double tv = f * (uB[d] + mu * e[d]);
+          sumdP[d] += tv;
+          if (n == own) dPa[d] = tv;
         }
       }
       for (int d = 0; d < 3; ++d) {
@@ -1445,9 +1351,24 @@ struct dg_handle {
   int* mgroups = nullptr;
   int ngroups_mx = 0, nsplit_mx = 0;
   long chunk_mx = 0;
-  size_t cross_lds_mx = 0;
   double* ej_part = nullptr;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
+  long* vbase = nullptr;  // [natoms] offset of atom k's partial potentials in vpart (minus first point)
+  // batch mode: several molecules in one handle (nmol = 0 for one molecule)
+  int nmol = 0;
+  int *atom_mol = nullptr, *mol_atom0 = nullptr, *mol_shell0 = nullptr, *mol_nbf = nullptr;
+  long* mol_poff = nullptr;  // offsets of the packed nbf_m x nbf_m matrices
+  double *mol_z = nullptr, *mol_fac = nullptr, *mol_qraw = nullptr, *atom_exc = nullptr, *atom_ej = nullptr;
+  int* bchunks = nullptr;   // contraction chunks {m, p_begin, p_end}
+  int* mol_chunk0 = nullptr;
+  int nbchunks = 0, max_mol_shells = 0, max_mol_atoms = 0, bchunk_len = 2048, item_len = 1024;
+  long packed = 0;  // sum of nbf_m^2
+  std::vector<int> h_mol_atom0, h_mol_nbf;
+  std::vector<long> h_mol_poff;
+  size_t items_cap = 0, bchunks_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
   // adjoint of E_xc (allocated on its first use)
   double *adj_ub = nullptr, *adj_ab = nullptr, *adj_udp = nullptr, *adj_T = nullptr, *adj_frag = nullptr;
   double *adj_part = nullptr, *adj_mot = nullptr, *adj_lam = nullptr, *adj_out = nullptr, *adj_dP = nullptr;
@@ -1466,10 +1387,16 @@ struct dg_handle {
                       phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
                       V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, ylmT, minvT, shc,
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC,
-                      adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP};
+                      adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP,
+                      mol_z, mol_fac, mol_qraw, atom_exc, atom_ej};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups, ash_start, ash_list};
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups, ash_start, ash_list,
+                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0};
+    if (vbase) (void)hipFree(vbase);
+    if (mol_poff) (void)hipFree(mol_poff);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (phh) (void)hipFree(phh);
     if (phl) (void)hipFree(phl);
     if (ej_part) (void)hipFree(ej_part);
@@ -1606,18 +1533,12 @@ int stage_correct(dg_handle* h) {
 
 template <int L>
 void launch_cross_l(dg_handle* h, CrossArgs& a, dim3 grid) {
-  if (h->precision == DG_PREC_MIXED)
-    hipLaunchKernelGGL(k_hartree_cross_mx<L>, grid, dim3(1024), h->cross_lds_mx, h->stream, a);
-  else
-    hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
+  hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
 }
 
 template <int L>
 hipError_t allow_cross_lds(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_hartree_cross<L>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_hartree_cross_mx<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  return hipFuncSetAttribute((const void*)k_hartree_cross<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)bytes);
 }
 
@@ -1674,17 +1595,11 @@ int stage_hartree(dg_handle* h) {
     a.shc = h->shc;
     a.items = h->items;
     a.vpart = h->vpart;
-    a.npts_pad = h->npts_pad;
+    a.vbase = h->vbase;
     a.nitems = h->nitems;
     a.nrad = h->nrad;
     a.nlm = h->nlm;
     dim3 grid(h->cross_wgs);
-    if (h->precision == DG_PREC_MIXED) {  // one pass over all l
-      a.l0 = 0;
-      a.l1 = h->lmax + 1;
-      a.accumulate = 0;
-      DG_TRY(run(h, "hartree_cross", [&] { launch_cross(h, a, grid); }));
-    } else
     for (size_t ci = 0; ci < h->lchunks.size(); ++ci) {
       a.l0 = h->lchunks[ci].first;
       a.l1 = h->lchunks[ci].second;
@@ -1694,8 +1609,8 @@ int stage_hartree(dg_handle* h) {
   }
   DG_TRY(run(h, "vreduce", [&] {
     hipLaunchKernelGGL(k_vreduce, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->vpart,
-                       h->vcell, h->w, h->rho, h->npts, h->npts_pad, h->cell, h->natoms, h->V, h->uJ,
-                       h->ej_part);
+                       h->vbase, h->vcell, h->w, h->rho, h->npts, h->npts_pad, h->cell, h->natoms,
+                       h->atom_mol, h->mol_atom0, h->V, h->uJ, h->ej_part);
   }));
   return DG_OK;
 }
@@ -1759,13 +1674,31 @@ int copy_out(dg_handle* h, double* dst, const double* src, size_t n) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+namespace {
+
+// Batch mode (several molecules in one handle; defined in batch.inl).
+// Atoms, shells and primitives of the molecules are concatenated molecule by
+// molecule; mol_atom0 / mol_shell0 hold each molecule's first atom / shell.
+struct BatchSpec {
+  int nmol = 0;
+  std::vector<int> mol_atom0, mol_shell0;
+};
+std::vector<int> cross_items(const dg_handle* h, int len);
+int batch_alloc(dg_handle* h, const BatchSpec* bs, const std::vector<int>& amol);
+int batch_setup(dg_handle* h);
+
+}  // namespace
+
 extern "C" {
 
 const char* dg_last_error(void) { return g_last_error.c_str(); }
 int dg_version(void) { return 100; }
 int dg_lebedev_available(int npoints) { return dg::lebedev_rule(npoints) != nullptr; }
 
-int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* prm, dg_handle** out) {
+// Shared by dg_create (bs = NULL) and dg_batch_create (bs lists the molecules
+// whose atoms and shells are concatenated in mol and basis).
+static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* prm,
+                       const BatchSpec* bs, dg_handle** out) {
   g_last_error.clear();
   if (!mol || !basis || !prm || !out) return fail(DG_ERR_INVALID, "dg_create: null argument");
   *out = nullptr;
@@ -1782,13 +1715,21 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     return fail(DG_ERR_INVALID, "dg_create: unknown precision mode");
   for (int a = 0; a < mol->natoms; ++a)
     if (mol->Z[a] < 1 || mol->Z[a] > 18) return fail(DG_ERR_INVALID, "dg_create: Z outside 1..18");
+  std::vector<int> amol(mol->natoms, 0);
+  if (bs)
+    for (int m = 0; m < bs->nmol; ++m)
+      for (int a = bs->mol_atom0[m]; a < bs->mol_atom0[m + 1]; ++a) amol[a] = m;
   for (int i = 0; i < mol->natoms; ++i)
-    for (int j = i + 1; j < mol->natoms; ++j) {
+    for (int j = i + 1; j < mol->natoms && amol[j] == amol[i]; ++j) {
       const double dx = mol->xyz[3 * i] - mol->xyz[3 * j], dy = mol->xyz[3 * i + 1] - mol->xyz[3 * j + 1],
                    dz = mol->xyz[3 * i + 2] - mol->xyz[3 * j + 2];
       if (dx * dx + dy * dy + dz * dz == 0.0) return fail(DG_ERR_INVALID, "dg_create: two atoms coincide");
     }
-  if (mol->natoms > 1024) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 1024 atoms");
+  if (!bs && mol->natoms > 1024) return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 1024 atoms");
+  if ((long)mol->natoms * prm->n_radial * prm->n_angular > (1L << 31) - 4096)
+    return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 2^31 grid points");
+  if (bs && prm->precision != DG_PREC_FP64)
+    return fail(DG_ERR_UNSUPPORTED, "dg_batch_create: batches run in FP64");
 
   auto h = new dg_handle();
   std::unique_ptr<dg_handle> guard(h);
@@ -1807,15 +1748,32 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   // shells -> CGF offsets, primitive tables
   h->nshell = basis->nshell;
   std::vector<int> cgf_off(h->nshell);
-  int nbf = 0, nprim_tot = 0;
+  int nbf = 0, nprim_tot = 0, cur_mol = 0, max_nbf = 0;
   for (int s = 0; s < h->nshell; ++s) {
     const int L = basis->L[s];
     if (L < 0 || L > 2) return fail(DG_ERR_INVALID, "dg_create: shell L must be 0, 1 or 2");
     if (basis->atom[s] < 0 || basis->atom[s] >= mol->natoms) return fail(DG_ERR_INVALID, "dg_create: shell atom out of range");
     if (basis->nprim[s] < 1 || basis->offset[s] < 0) return fail(DG_ERR_INVALID, "dg_create: bad primitive range");
+    if (bs) {  // CGF offsets restart with every molecule
+      while (cur_mol < bs->nmol && s >= bs->mol_shell0[cur_mol + 1]) {
+        h->h_mol_nbf.push_back(nbf);
+        nbf = 0;
+        ++cur_mol;
+      }
+      if (amol[basis->atom[s]] != cur_mol) return fail(DG_ERR_INVALID, "dg_batch_create: shell atom outside its molecule");
+    }
     cgf_off[s] = nbf;
     nbf += (L + 1) * (L + 2) / 2;
+    max_nbf = std::max(max_nbf, nbf);
     nprim_tot = std::max(nprim_tot, basis->offset[s] + basis->nprim[s]);
+  }
+  if (bs) {
+    h->h_mol_nbf.push_back(nbf);
+    while ((int)h->h_mol_nbf.size() < bs->nmol) h->h_mol_nbf.push_back(0);
+    for (int m = 0; m < bs->nmol; ++m)
+      if (h->h_mol_nbf[m] < 1) return fail(DG_ERR_INVALID, "dg_batch_create: a molecule has no basis functions");
+    if (max_nbf > 32) return fail(DG_ERR_UNSUPPORTED, "dg_batch_create: more than 32 basis functions in a molecule");
+    nbf = 32;  // batch Phi rows are 32 wide; each molecule uses its first nbf_m columns
   }
   h->nbf = nbf;
   h->ldphi = (int)round_up(nbf, 32);
@@ -1930,9 +1888,6 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     ns = std::min(std::min(ns, 256L), h->npts_pad / 128);
     h->chunk_mx = round_up((h->npts_pad + ns - 1) / ns, 128);
     h->nsplit_mx = (int)((h->npts_pad + h->chunk_mx - 1) / h->chunk_mx);
-    h->cross_lds_mx = (size_t)(4 * h->nrad + h->nrad * (h->nlm - 1)) * sizeof(double);
-    if (h->cross_lds_mx > 150 * 1024)
-      return fail(DG_ERR_UNSUPPORTED, "dg_create: radial grid too large for the mixed Hartree kernel");
   }
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
@@ -1955,23 +1910,28 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     h->cross_lds = maxb;
   }
   // cross-potential work list: point ranges of <= 1024 outside each cell
-  std::vector<int> items;
-  for (int k = 0; k < h->natoms && h->natoms > 1; ++k) {
-    auto add = [&](long b, long e) {
-      for (long p = b; p < e; p += 1024) {
-        items.push_back(k);
-        items.push_back((int)p);
-        items.push_back((int)std::min(e, p + 1024));
-      }
-    };
-    add(0, (long)k * h->cell);
-    add((long)(k + 1) * h->cell, h->npts);
+  std::vector<long> vbase(h->natoms);
+  long vtotal = 0;
+  if (bs) {
+    h->nmol = bs->nmol;
+    h->h_mol_atom0 = bs->mol_atom0;
+    for (int k = 0; k < h->natoms; ++k) {
+      const int m = amol[k];
+      const long p0 = (long)bs->mol_atom0[m] * h->cell, p1 = (long)bs->mol_atom0[m + 1] * h->cell;
+      vbase[k] = vtotal - p0;
+      vtotal += p1 - p0;
+    }
+  } else {
+    for (int k = 0; k < h->natoms; ++k) vbase[k] = (long)k * h->npts_pad;
+    vtotal = (long)h->natoms * h->npts_pad;
   }
+  std::vector<int> items = cross_items(h, 1024);
   h->nitems = (int)items.size() / 3;
+  h->items_cap = std::max<size_t>(bs ? cross_items(h, 256).size() : items.size(), 3);
   {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-    const size_t lds = h->precision == DG_PREC_MIXED ? h->cross_lds_mx : h->cross_lds;
+    const size_t lds = h->cross_lds;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
   }
@@ -1994,13 +1954,21 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->pr_alpha, nprim_tot));
   DG_TRY(dalloc(&h->pr_cn0, nprim_tot));
   DG_TRY(dalloc(&h->pr_cn1, nprim_tot));
-  DG_TRY(dalloc(&h->P, (size_t)h->nbf * h->nbf));
-  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1)));
-  DG_TRY(dalloc(&h->vpart, (size_t)h->natoms * np));
+  if (bs) {
+    h->h_mol_poff.assign(bs->nmol + 1, 0);
+    for (int m = 0; m < bs->nmol; ++m) h->h_mol_poff[m + 1] = h->h_mol_poff[m] + (long)h->h_mol_nbf[m] * h->h_mol_nbf[m];
+    h->packed = h->h_mol_poff[bs->nmol];
+  } else {
+    h->packed = (long)h->nbf * h->nbf;
+  }
+  DG_TRY(dalloc(&h->P, (size_t)h->packed));
+  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1) * std::max(1, h->nmol)));
+  DG_TRY(dalloc(&h->vpart, (size_t)vtotal));
+  DG_TRY(dalloc(&h->vbase, (size_t)h->natoms));
   DG_TRY(dalloc(&h->q, h->natoms));
   DG_TRY(dalloc(&h->qc, h->natoms));
   DG_TRY(dalloc(&h->exc_part, h->nlda_blocks));
-  DG_TRY(dalloc(&h->scalars, 4));
+  DG_TRY(dalloc(&h->scalars, 4 * (size_t)std::max(1, h->nmol)));
   DG_TRY(dalloc(&h->ylm, ylm.size()));
   DG_TRY(dalloc(&h->ylmT, ylmT.size()));
   DG_TRY(dalloc(&h->minvT, minvT.size()));
@@ -2012,7 +1980,7 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   DG_TRY(dalloc(&h->U, (size_t)h->natoms * h->nrad * h->nlm));
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
   DG_TRY(dalloc(&h->blocks, blocks.size()));
-  DG_TRY(dalloc(&h->items, items.size()));
+  DG_TRY(dalloc(&h->items, h->items_cap));
   DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit * h->ngroups, (size_t)h->nsplit_mx * h->ngroups_mx) * 2 * 1024));
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
@@ -2020,8 +1988,9 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
     HIP_TRY(hipMalloc((void**)&h->phh, (size_t)np * h->ldphi * sizeof(__bf16)));
     HIP_TRY(hipMalloc((void**)&h->phl, (size_t)np * h->ldphi * sizeof(__bf16)));
   }
-  DG_TRY(dalloc(&h->J, (size_t)h->nbf * h->nbf));
-  DG_TRY(dalloc(&h->XC, (size_t)h->nbf * h->nbf));
+  DG_TRY(dalloc(&h->J, (size_t)h->packed));
+  DG_TRY(dalloc(&h->XC, (size_t)h->packed));
+  if (bs) DG_TRY(batch_alloc(h, bs, amol));
 
   std::vector<int> shv(h->nshell);
   auto up = [&](void* d, const void* s, size_t bytes) {
@@ -2051,20 +2020,30 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
   if (!mgroups.empty()) HIP_TRY(up(h->mgroups, mgroups.data(), mgroups.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
-  HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)h->natoms * np * 8, h->stream));
+  HIP_TRY(up(h->vbase, vbase.data(), vbase.size() * 8));
+  HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)vtotal * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->rho, 0, (size_t)np * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->vcell, 0, (size_t)np * 8, h->stream));
-  HIP_TRY(hipMemsetAsync(h->scalars, 0, 4 * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->scalars, 0, 4 * 8 * (size_t)std::max(1, h->nmol), h->stream));
 
-  HIP_TRY(allow_cross_lds_l(h->lmax, std::max(h->cross_lds, h->cross_lds_mx)));
-  if ((size_t)h->natoms * 64 * 8 > 160 * 1024)
-    return fail(DG_ERR_UNSUPPORTED, "dg_create: too many atoms for the Becke kernel's LDS");
-  HIP_TRY(hipFuncSetAttribute((const void*)k_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)std::max<size_t>((size_t)h->natoms * 64 * 8, 1024)));
+  HIP_TRY(allow_cross_lds_l(h->lmax, h->cross_lds));
+  if (!bs) {
+    if ((size_t)h->natoms * 64 * 8 > 160 * 1024)
+      return fail(DG_ERR_UNSUPPORTED, "dg_create: too many atoms for the Becke kernel's LDS");
+    HIP_TRY(hipFuncSetAttribute((const void*)k_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)std::max<size_t>((size_t)h->natoms * 64 * 8, 1024)));
+  }
   DG_TRY(run(h, "grid", [&] {
     hipLaunchKernelGGL(k_grid, dim3(nblocks(np, 256)), dim3(256), 0, h->stream, h->atoms, h->rad, h->wrad,
                        h->leb, h->npts, np, h->nrad, h->nang, h->px, h->py, h->pz, h->w);
   }));
+  if (bs) {
+    DG_TRY(batch_setup(h));
+    DG_TRY(sync(h));
+    guard.release();
+    *out = h;
+    return DG_OK;
+  }
   DG_TRY(run(h, "becke", [&] {
     hipLaunchKernelGGL(k_becke, dim3(nblocks(np, 64)), dim3(64), (size_t)h->natoms * 64 * 8, h->stream,
                        h->atoms, h->invR, h->natoms, h->npts, h->cell, h->px, h->py, h->pz, h->w, h->wb);
@@ -2084,6 +2063,10 @@ int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_param
   guard.release();
   *out = h;
   return DG_OK;
+}
+
+int dg_create(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* prm, dg_handle** out) {
+  return create_impl(mol, basis, prm, nullptr, out);
 }
 
 int dg_destroy(dg_handle* h) {
@@ -2123,6 +2106,7 @@ int dg_get_stream(const dg_handle* h, void** stream) {
 
 int dg_set_density(dg_handle* h, const double* P, int nbf) {
   if (!h || !P) return fail(DG_ERR_INVALID, "dg_set_density: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_set_density: not for a batch handle (use dg_batch_build)");
   if (nbf != h->nbf) return fail(DG_ERR_INVALID, "dg_set_density: nbf mismatch");
   for (long i = 0; i < (long)nbf * nbf; ++i)
     if (!std::isfinite(P[i])) return fail(DG_ERR_INVALID, "dg_set_density: P contains a non-finite value");
@@ -2133,6 +2117,7 @@ int dg_set_density(dg_handle* h, const double* P, int nbf) {
 
 int dg_correct_densities(dg_handle* h, double* nelec_before) {
   if (!h) return fail(DG_ERR_INVALID, "dg_correct_densities: null handle");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_correct_densities: not for a batch handle (use dg_batch_build)");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_correct_densities: no density set");
   DG_TRY(stage_correct(h));
   h->has_hartree = h->has_xc = false;
@@ -2175,6 +2160,7 @@ int dg_hartree(dg_handle* h, double* J) {
 
 int dg_xc(dg_handle* h, double* XC, double* exc) {
   if (!h) return fail(DG_ERR_INVALID, "dg_xc: null handle");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_xc: not for a batch handle (use dg_batch_build)");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_xc: no density set");
   DG_TRY(stage_lda(h));
   DG_TRY(stage_contract(h, h->uxc, nullptr, h->XC, nullptr));
@@ -2191,6 +2177,7 @@ int dg_xc(dg_handle* h, double* XC, double* exc) {
 
 int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, double* dscalars) {
   if (!h || !dP) return fail(DG_ERR_INVALID, "dg_build_device: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_build_device: not for a batch handle (use dg_batch_build)");
   DG_TRY(stage_density(h, dP));
   DG_TRY(stage_correct(h));
   DG_TRY(stage_lda(h));
@@ -2259,11 +2246,12 @@ int dg_get_xc_parts(dg_handle* h, double* exd, double* wv) {
 
 int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
   if (!h) return fail(DG_ERR_INVALID, "dg_xc_adjoint: null handle");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_xc_adjoint: not for a batch handle (use dg_batch_build)");
   if (!h->has_xc || !h->corrected || !h->last_P)
     return fail(DG_ERR_STATE, "dg_xc_adjoint: needs a corrected density and dg_xc (or a build) first");
   const long np = h->npts_pad, nblk = np / 64;
   const int na = h->natoms;
-  const size_t becke_lds = (size_t)2 * na * 64 * sizeof(double);
+  const size_t becke_lds = (size_t)na * 64 * (2 * sizeof(double) + sizeof(int));
   if (grad && becke_lds > 150 * 1024)
     return fail(DG_ERR_UNSUPPORTED, "dg_xc_adjoint: too many atoms for the Becke-gradient kernel's LDS");
   if (!h->adj_ub) {
@@ -2445,3 +2433,5 @@ int dg_selftest_mfma(int* ok) {
 }
 
 }  // extern "C"
+
+#include "batch.inl"

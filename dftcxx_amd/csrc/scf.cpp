@@ -299,21 +299,18 @@ void gemm(int n, const double* A, bool ta, const double* B, double* C) {  // C =
     }
 }
 
-// The reference's DFT class (src/dft.h:41-216) with the grid work on the GPU.
-class DFT {
+// Host algebra of the reference's DFT class (src/dft.h:41-216): integrals,
+// orthogonalisation, the density matrix with mixing and the energy.  The grid
+// results (J, XC, scalars) are supplied by a single-molecule handle (DFT) or a
+// batch (batch_scf).
+class HostSCF {
  public:
-  DFT(const MoleculeData& mol, const dg_grid_params& grid)
-      : mol_(mol), grid_(mol), mixed_(grid.precision == DG_PREC_MIXED) {
-    grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision);
-    grid_.create_grid();
-    n_ = grid_.nbf();
+  explicit HostSCF(const MoleculeData& mol, bool mixed = false) : mol_(mol), mixed_(mixed) {
+    n_ = mol.nbf();
     nelec_ = mol.nelec();
   }
-  ~DFT() {
-    if (dP_) (void)hipFree(dP_);
-  }
 
-  // src/dft.cpp:160-227
+  // src/dft.cpp:160-227 without the grid part
   void construct_matrices() {
     std::vector<double> Tm, Vm;
     one_electron(mol_, S_, Tm, Vm);
@@ -339,6 +336,107 @@ class DFT {
     XC_.assign((size_t)n_ * n_, 0.0);
     P_.assign((size_t)n_ * n_, 0.0);
     exc_ = 0.0;  // the reference reads it before the first XC evaluation
+  }
+
+  // src/dft.cpp:330-366
+  void density_matrix() {
+    const size_t nn = (size_t)n_ * n_;
+    std::vector<double> F(nn), tmp(nn), Fp(nn), Cc, C(nn), w;
+    for (size_t i = 0; i < nn; ++i) F[i] = H_[i] + 2.0 * J_[i] + XC_[i];
+    gemm(n_, X_.data(), true, F.data(), tmp.data());
+    gemm(n_, tmp.data(), false, X_.data(), Fp.data());
+    eigh(n_, Fp, w, Cc);
+    // C = X Cc
+    for (int j = 0; j < n_; ++j)
+      for (int i = 0; i < n_; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < n_; ++k) s += X_[i + (size_t)k * n_] * Cc[k + (size_t)j * n_];
+        C[i + (size_t)j * n_] = s;
+      }
+    std::vector<double> Pn(nn, 0.0);
+    const int nocc = nelec_ / 2;
+    for (int j = 0; j < n_; ++j)
+      for (int i = 0; i < n_; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < nocc; ++k) s += C[i + (size_t)k * n_] * C[j + (size_t)k * n_];
+        Pn[i + (size_t)j * n_] = s;
+      }
+    if (first_) {
+      P_ = Pn;
+      first_ = false;
+    } else {
+      for (size_t i = 0; i < nn; ++i) P_[i] = (1.0 - mixing) * Pn[i] + mixing * P_[i];
+    }
+  }
+
+  // results of one grid build: J, XC (taken only with_xc) and {E_xc, N before, N after, E_J}
+  void take_grid(const double* J, const double* XC, const double* sc, bool with_xc) {
+    const size_t nn = (size_t)n_ * n_;
+    std::memcpy(J_.data(), J, nn * 8);
+    nelec_grid_ = sc[2];
+    ej_point_ = sc[3];
+    if (with_xc) {
+      std::memcpy(XC_.data(), XC, nn * 8);
+      exc_ = sc[0];
+    }
+  }
+
+  // src/dft.cpp:441-447
+  void energy() {
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n_; ++i)
+      for (int k = 0; k < n_; ++k) {
+        t1 += P_[i + (size_t)k * n_] * H_[k + (size_t)i * n_];
+        t2 += P_[i + (size_t)k * n_] * J_[k + (size_t)i * n_];
+      }
+    e_one_ = 2.0 * t1;
+    // In reduced precision J carries BF16-product errors that would enter 2 tr(PJ)
+    // at first order; the same quantity evaluated pointwise in FP64 does not.
+    e_j_ = mixed_ ? ej_point_ : 2.0 * t2;
+    et_ = e_one_ + e_j_ + enuc_ + exc_;
+  }
+
+  void fill(dg_scf_result* r, int iterations, double setup, double sec_iter) const {
+    r->iterations = iterations;
+    r->energy = et_;
+    r->e_one = e_one_;
+    r->e_j = e_j_;
+    r->e_xc = exc_;
+    r->e_nuc = enuc_;
+    r->nelec = nelec_grid_;
+    r->seconds_setup = setup;
+    r->seconds_iter = sec_iter;
+  }
+
+  double mixing = 0.5;
+  double et_ = 0.0, e_one_ = 0.0, e_j_ = 0.0, exc_ = 0.0, enuc_ = 0.0, nelec_grid_ = 0.0;
+  std::vector<double> P_;
+  int n_ = 0;
+
+ protected:
+  MoleculeData mol_;
+  bool mixed_ = false;
+  double ej_point_ = 0.0;
+  int nelec_ = 0;
+  bool first_ = true;
+  std::vector<double> S_, H_, X_, J_, XC_;
+};
+
+// The reference's DFT class with the grid work of every iteration on the GPU
+// (dg_build_device on one molecule's handle).
+class DFT : public HostSCF {
+ public:
+  DFT(const MoleculeData& mol, const dg_grid_params& grid)
+      : HostSCF(mol, grid.precision == DG_PREC_MIXED), grid_(mol) {
+    grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision);
+    grid_.create_grid();
+  }
+  ~DFT() {
+    if (dP_) (void)hipFree(dP_);
+  }
+
+  void construct_matrices() {
+    HostSCF::construct_matrices();
     if (hipMalloc((void**)&dP_, sizeof(double) * (3 * (size_t)n_ * n_ + 4)) != hipSuccess)
       throw std::runtime_error("hipMalloc failed");
     density_matrix();
@@ -366,95 +464,81 @@ class DFT {
     return iteration;
   }
 
-  double mixing = 0.5;
-  double et_ = 0.0, e_one_ = 0.0, e_j_ = 0.0, exc_ = 0.0, enuc_ = 0.0, nelec_grid_ = 0.0;
-  std::vector<double> P_;
-
  private:
-  // src/dft.cpp:330-366
-  void density_matrix() {
-    const size_t nn = (size_t)n_ * n_;
-    std::vector<double> F(nn), tmp(nn), Fp(nn), Cc, C(nn), w;
-    for (size_t i = 0; i < nn; ++i) F[i] = H_[i] + 2.0 * J_[i] + XC_[i];
-    gemm(n_, X_.data(), true, F.data(), tmp.data());
-    std::vector<double> Xt(X_);
-    gemm(n_, tmp.data(), false, X_.data(), Fp.data());
-    eigh(n_, Fp, w, Cc);
-    // C = X Cc
-    for (int j = 0; j < n_; ++j)
-      for (int i = 0; i < n_; ++i) {
-        double s = 0.0;
-        for (int k = 0; k < n_; ++k) s += X_[i + (size_t)k * n_] * Cc[k + (size_t)j * n_];
-        C[i + (size_t)j * n_] = s;
-      }
-    std::vector<double> Pn(nn, 0.0);
-    const int nocc = nelec_ / 2;
-    for (int j = 0; j < n_; ++j)
-      for (int i = 0; i < n_; ++i) {
-        double s = 0.0;
-        for (int k = 0; k < nocc; ++k) s += C[i + (size_t)k * n_] * C[j + (size_t)k * n_];
-        Pn[i + (size_t)j * n_] = s;
-      }
-    if (first_) {
-      P_ = Pn;
-      first_ = false;
-    } else {
-      for (size_t i = 0; i < nn; ++i) P_[i] = (1.0 - mixing) * Pn[i] + mixing * P_[i];
-    }
-  }
-
   // set_density + correct_densities + Hartree J (+ XC) on the GPU
   void grid_build(bool with_xc) {
     const size_t nn = (size_t)n_ * n_;
     double* dJ = dP_ + nn;
     double* dXC = dJ + nn;
     double* dS = dXC + nn;
-    hipStream_t st;
     void* sp = nullptr;
     dg_throw(dg_get_stream(grid_.handle(), &sp));
-    st = (hipStream_t)sp;
+    hipStream_t st = (hipStream_t)sp;
     if (hipMemcpyAsync(dP_, P_.data(), nn * 8, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("copy of P failed");
     dg_throw(dg_build_device(grid_.handle(), dP_, dJ, dXC, dS));
     double sc[4];
-    std::vector<double> XC(nn);
-    if (hipMemcpyAsync(J_.data(), dJ, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    std::vector<double> J(nn), XC(nn);
+    if (hipMemcpyAsync(J.data(), dJ, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(XC.data(), dXC, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(sc, dS, 4 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       throw std::runtime_error("copy of grid results failed");
-    nelec_grid_ = sc[2];
-    ej_point_ = sc[3];
-    if (with_xc) {
-      XC_.swap(XC);
-      exc_ = sc[0];
-    }
+    take_grid(J.data(), XC.data(), sc, with_xc);
   }
 
-  // src/dft.cpp:441-447
-  void energy() {
-    double t1 = 0.0, t2 = 0.0;
-    for (int i = 0; i < n_; ++i)
-      for (int k = 0; k < n_; ++k) {
-        t1 += P_[i + (size_t)k * n_] * H_[k + (size_t)i * n_];
-        t2 += P_[i + (size_t)k * n_] * J_[k + (size_t)i * n_];
-      }
-    e_one_ = 2.0 * t1;
-    // In reduced precision J carries BF16-product errors that would enter 2 tr(PJ)
-    // at first order; the same quantity evaluated pointwise in FP64 does not.
-    e_j_ = mixed_ ? ej_point_ : 2.0 * t2;
-    et_ = e_one_ + e_j_ + enuc_ + exc_;
-  }
-
-  MoleculeData mol_;
   MolecularGrid grid_;
-  bool mixed_ = false;
-  double ej_point_ = 0.0;
-  int n_ = 0, nelec_ = 0;
-  bool first_ = true;
-  std::vector<double> S_, H_, X_, J_, XC_;
   double* dP_ = nullptr;
 };
+
+// Batched SCF: every molecule runs the reference's loop (src/dft.cpp:84-142)
+// with its own convergence test; the grid work of one iteration of all
+// molecules is one dg_batch_build.  A converged molecule keeps its density
+// matrix and results (its later builds are discarded), so its energy sequence
+// is the one a single-molecule run produces.
+int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& prm, int* iterations,
+              double* sec_iter) {
+  const int nm = (int)mols.size();
+  std::vector<int> nbf(nm);
+  std::vector<long> off(nm + 1);
+  dg_throw(dg_batch_layout(batch, nbf.data(), off.data(), nullptr));
+  std::vector<double> P(off[nm]), J(off[nm]), XC(off[nm]), sc(4 * (size_t)nm);
+  auto build = [&](bool with_xc, const std::vector<char>& active) {
+    for (int m = 0; m < nm; ++m) std::memcpy(&P[off[m]], mols[m].P_.data(), (size_t)nbf[m] * nbf[m] * 8);
+    dg_throw(dg_batch_build(batch, P.data(), J.data(), XC.data(), sc.data()));
+    for (int m = 0; m < nm; ++m)
+      if (active[m]) {
+        mols[m].take_grid(&J[off[m]], &XC[off[m]], &sc[4 * (size_t)m], with_xc);
+        mols[m].energy();
+      }
+  };
+  std::vector<char> active(nm, 1);
+  for (auto& d : mols) d.density_matrix();
+  build(false, active);
+  std::vector<double> old(nm), diff(nm, 1.0);
+  for (int m = 0; m < nm; ++m) old[m] = mols[m].et_;
+  int iteration = 0, done = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  while (done < nm) {
+    iteration++;
+    for (int m = 0; m < nm; ++m)
+      if (active[m]) mols[m].density_matrix();
+    build(true, active);
+    for (int m = 0; m < nm; ++m) {
+      if (!active[m]) continue;
+      diff[m] = std::fabs(mols[m].et_ - old[m]);
+      old[m] = mols[m].et_;
+      iterations[m] = iteration;
+      if ((diff[m] <= prm.threshold && iteration >= prm.min_iter) || iteration >= prm.max_iter) {
+        active[m] = 0;
+        ++done;
+      }
+    }
+  }
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (sec_iter) *sec_iter = iteration ? dt / iteration : 0.0;
+  return iteration;
+}
 
 }  // namespace
 }  // namespace dftcxx
@@ -527,19 +611,46 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     double sec_iter = 0.0;
     const int it = dft.scf(scf->threshold, scf->min_iter, scf->max_iter, energies, &sec_iter);
-    result->iterations = it;
-    result->energy = dft.et_;
-    result->e_one = dft.e_one_;
-    result->e_j = dft.e_j_;
-    result->e_xc = dft.exc_;
-    result->e_nuc = dft.enuc_;
-    result->nelec = dft.nelec_grid_;
-    result->seconds_setup = setup;
-    result->seconds_iter = sec_iter;
+    dft.fill(result, it, setup, sec_iter);
     if (P_out) std::memcpy(P_out, dft.P_.data(), sizeof(double) * dft.P_.size());
     return DG_OK;
   } catch (const std::exception& e) {
     dg::set_last_error(std::string("dg_scf_run: ") + e.what());
+    return DG_ERR_STATE;
+  }
+}
+
+extern "C" int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basis* bases,
+                                const dg_grid_params* grid, const dg_scf_params* scf, dg_scf_result* results,
+                                int* iterations) {
+  if (nmol < 1 || !mols || !bases || !grid || !scf || !results) {
+    dg::set_last_error("dg_batch_scf_run: null or empty argument");
+    return DG_ERR_INVALID;
+  }
+  dg_batch* batch = nullptr;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<dftcxx::HostSCF> dfts;
+    dfts.reserve(nmol);
+    for (int m = 0; m < nmol; ++m) {
+      dftcxx::MoleculeData md = to_data(&mols[m], &bases[m]);
+      if (md.nelec() % 2 != 0) throw std::runtime_error("closed-shell SCF needs an even electron count");
+      dfts.emplace_back(md);
+      dfts.back().mixing = scf->mixing;
+      dfts.back().construct_matrices();
+    }
+    if (dg_batch_create(nmol, mols, bases, grid, &batch) != DG_OK) throw std::runtime_error(dg_last_error());
+    const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<int> its(nmol, 0);
+    double sec_iter = 0.0;
+    const int total = dftcxx::batch_scf(dfts, batch, *scf, its.data(), &sec_iter);
+    for (int m = 0; m < nmol; ++m) dfts[m].fill(&results[m], its[m], setup, sec_iter);
+    if (iterations) *iterations = total;
+    dg_batch_destroy(batch);
+    return DG_OK;
+  } catch (const std::exception& e) {
+    if (batch) dg_batch_destroy(batch);
+    dg::set_last_error(std::string("dg_batch_scf_run: ") + e.what());
     return DG_ERR_STATE;
   }
 }

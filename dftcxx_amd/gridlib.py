@@ -31,7 +31,9 @@ EXPORTS = (
     "dg_get_positions", "dg_get_densities", "dg_get_amplitudes", "dg_get_hartree_potential",
     "dg_get_hartree_parts", "dg_get_xc_parts", "dg_xc_adjoint", "dg_build_device", "dg_profile_enable",
     "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_mfma_probe_bf16", "dg_scf_run", "dg_one_electron",
-    "dg_eigh",
+    "dg_eigh", "dg_batch_create", "dg_batch_destroy", "dg_batch_info", "dg_batch_layout",
+    "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
+    "dg_batch_autotune", "dg_batch_scf_run",
 )
 
 
@@ -112,6 +114,19 @@ def lib():
                                  C.POINTER(_ScfParams), C.POINTER(_ScfResult), _D, _D]
         L.dg_one_electron.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), _D, _D, _D]
         L.dg_eigh.argtypes = [C.c_int, _D, _D, _D]
+        L.dg_batch_create.argtypes = [C.c_int, C.POINTER(_Molecule), C.POINTER(_Basis),
+                                      C.POINTER(_GridParams), C.POINTER(C.c_void_p)]
+        L.dg_batch_destroy.argtypes = [C.c_void_p]
+        L.dg_batch_info.argtypes = [C.c_void_p, _I, C.POINTER(C.c_long), _I, C.POINTER(C.c_long)]
+        L.dg_batch_layout.argtypes = [C.c_void_p, _I, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.dg_batch_handle.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.dg_batch_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.dg_batch_build_device.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.dg_batch_build.argtypes = [C.c_void_p, _D, _D, _D, _D]
+        L.dg_batch_autotune.argtypes = [C.c_void_p, _I, _I, _D]
+        L.dg_batch_scf_run.argtypes = [C.c_int, C.POINTER(_Molecule), C.POINTER(_Basis),
+                                       C.POINTER(_GridParams), C.POINTER(_ScfParams),
+                                       C.POINTER(_ScfResult), _I]
         _lib = L
     return _lib
 
@@ -344,17 +359,120 @@ class MolecularGrid:
 
     def profile_read(self):
         self._need()
-        maxn = 64
-        names = C.create_string_buffer(48 * maxn)
-        ms = np.zeros(maxn)
-        cnt = np.zeros(maxn, dtype=np.int32)
-        n = C.c_int()
-        _check(lib().dg_profile_read(self.h, maxn, names, _d(ms), _i(cnt), C.byref(n)))
-        out = {}
-        for i in range(min(n.value, maxn)):
-            nm = names.raw[48 * i:48 * i + 48].split(b"\0", 1)[0].decode()
-            out[nm] = (float(ms[i]), int(cnt[i]))
+        return _profile_read(self.h)
+
+
+def _profile_read(h):
+    """{kernel name: (total ms, launches)} accumulated on handle h."""
+    maxn = 64
+    names = C.create_string_buffer(48 * maxn)
+    ms = np.zeros(maxn)
+    cnt = np.zeros(maxn, dtype=np.int32)
+    n = C.c_int()
+    _check(lib().dg_profile_read(h, maxn, names, _d(ms), _i(cnt), C.byref(n)))
+    out = {}
+    for i in range(min(n.value, maxn)):
+        nm = names.raw[48 * i:48 * i + 48].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(cnt[i]))
+    return out
+
+
+class Batch:
+    """Many molecules on one grid specification, built together (dg_batch_*).
+
+    Every SCF iteration is one hipGraph launch for the whole batch; matrices
+    are passed as lists of (nbf_m, nbf_m) arrays, or packed for the device
+    entry point (molecule m's column-major matrix at ``offset[m]``).
+    """
+
+    def __init__(self, mols, radial_points, angular_points, lmax=8):
+        self.mols = list(mols)
+        n = len(self.mols)
+        ins = [_c_inputs(m, radial_points, angular_points, lmax, PREC_FP64) for m in self.mols]
+        self._keep = ins
+        ms = (_Molecule * n)(*[i[0] for i in ins])
+        bs = (_Basis * n)(*[i[1] for i in ins])
+        g = ins[0][2]
+        h = C.c_void_p()
+        self.b = None
+        _check(lib().dg_batch_create(n, ms, bs, C.byref(g), C.byref(h)))
+        self.b = h
+        nm, npts, nat, packed = C.c_int(), C.c_long(), C.c_int(), C.c_long()
+        _check(lib().dg_batch_info(self.b, C.byref(nm), C.byref(npts), C.byref(nat), C.byref(packed)))
+        self.nmol, self.npts, self.natoms, self.packed = nm.value, npts.value, nat.value, packed.value
+        self.nbf = np.empty(n, np.int32)
+        self.offset = np.empty(n + 1, np.int64)
+        self.point0 = np.empty(n + 1, np.int64)
+        L64 = C.POINTER(C.c_long)
+        _check(lib().dg_batch_layout(self.b, _i(self.nbf), self.offset.ctypes.data_as(L64),
+                                     self.point0.ctypes.data_as(L64)))
+        hh = C.c_void_p()
+        _check(lib().dg_batch_handle(self.b, C.byref(hh)))
+        self.h = hh
+
+    def close(self):
+        if self.b is not None:
+            lib().dg_batch_destroy(self.b)
+            self.b = None
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pack(self, mats):
+        """Packed column-major vector of per-molecule matrices."""
+        out = np.empty(self.packed)
+        for m, A in enumerate(mats):
+            out[self.offset[m]:self.offset[m + 1]] = np.asarray(A, dtype=np.float64).T.ravel()
         return out
+
+    def unpack(self, flat):
+        return [flat[self.offset[m]:self.offset[m + 1]].reshape(self.nbf[m], self.nbf[m]).T.copy()
+                for m in range(self.nmol)]
+
+    def build(self, Ps):
+        """(list of J, list of XC, scalars (nmol, 4)) for the density matrices Ps."""
+        P = self.pack(Ps)
+        J, XC = np.empty(self.packed), np.empty(self.packed)
+        sc = np.empty((self.nmol, 4))
+        _check(lib().dg_batch_build(self.b, _d(P), _d(J), _d(XC), _d(sc)))
+        return self.unpack(J), self.unpack(XC), sc
+
+    def build_device(self, dP, dJ, dXC, dscalars):
+        _check(lib().dg_batch_build_device(self.b, C.c_void_p(dP), C.c_void_p(dJ), C.c_void_p(dXC),
+                                           C.c_void_p(dscalars)))
+
+    def set_stream(self, stream_ptr):
+        _check(lib().dg_batch_set_stream(self.b, C.c_void_p(stream_ptr)))
+
+    def autotune(self):
+        """(contraction chunk length, cross item length, their best ms) chosen on this batch."""
+        c, i = C.c_int(), C.c_int()
+        ms = np.zeros(2)
+        _check(lib().dg_batch_autotune(self.b, C.byref(c), C.byref(i), _d(ms)))
+        return c.value, i.value, ms
+
+    def get_weights(self):
+        w = np.empty(self.npts)
+        _check(lib().dg_get_weights(self.h, _d(w)))
+        return w
+
+    def get_densities(self):
+        r = np.empty(self.npts)
+        _check(lib().dg_get_densities(self.h, _d(r)))
+        return r
+
+    def profile_enable(self, on=True):
+        _check(lib().dg_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        _check(lib().dg_profile_reset(self.h))
+
+    def profile_read(self):
+        return _profile_read(self.h)
 
 
 def one_electron(mol):
@@ -389,3 +507,19 @@ def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, m
     _check(lib().dg_scf_run(C.byref(m), C.byref(b), C.byref(g), C.byref(sp), C.byref(res), _d(en), _d(P)))
     r = {k: getattr(res, k) for k, _ in _ScfResult._fields_}
     return en[:res.iterations], r, P.T.copy()
+
+
+def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5):
+    """The reference SCF for every molecule of a batch, one batched grid build per iteration.
+
+    Returns (list of result dicts, number of batch iterations)."""
+    mols = list(mols)
+    n = len(mols)
+    ins = [_c_inputs(m, nrad, nang, lmax, PREC_FP64) for m in mols]
+    ms = (_Molecule * n)(*[i[0] for i in ins])
+    bs = (_Basis * n)(*[i[1] for i in ins])
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing)
+    res = (_ScfResult * n)()
+    it = C.c_int()
+    _check(lib().dg_batch_scf_run(n, ms, bs, C.byref(ins[0][2]), C.byref(sp), res, C.byref(it)))
+    return [{k: getattr(r, k) for k, _ in _ScfResult._fields_} for r in res], it.value

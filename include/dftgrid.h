@@ -180,6 +180,46 @@ typedef struct {
 int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* grid,
                const dg_scf_params* scf, dg_scf_result* result, double* energies, double* P_out);
 
+/* --- batches of molecules ----------------------------------------------------- */
+/* No counterpart in the reference, which runs one molecule per process
+ * (src/dftcxx.cpp:53-54).  A batch holds many independent molecules (each at
+ * most 32 CGFs) on one grid specification (FP64 only) and builds J, XC and the
+ * scalars of all of them in one hipGraph launch per SCF iteration.  P, J and XC
+ * are packed: molecule m's nbf_m x nbf_m column-major matrix starts at
+ * offset[m] (dg_batch_layout).  Each molecule's results equal those of a
+ * single-molecule handle up to summation order. */
+typedef struct dg_batch dg_batch;
+int dg_batch_create(int nmol, const dg_molecule* mols, const dg_basis* bases, const dg_grid_params* prm,
+                    dg_batch** out);
+int dg_batch_destroy(dg_batch* b);
+/* molecules, total grid points, total atoms, packed matrix length (any may be NULL) */
+int dg_batch_info(const dg_batch* b, int* nmol, long* npts, int* natoms, long* packed);
+/* per molecule: nbf [nmol], packed offsets [nmol+1], first grid point [nmol+1] (any may be NULL) */
+int dg_batch_layout(const dg_batch* b, int* nbf, long* offset, long* point0);
+/* The underlying grid handle, for the getters (weights, densities, ...) and
+ * profiling; its single-molecule build calls return DG_ERR_STATE. */
+int dg_batch_handle(dg_batch* b, dg_handle** h);
+int dg_batch_set_stream(dg_batch* b, void* stream);
+/* One build of every molecule on device buffers (packed dP, dJ, dXC; dscalars
+ * 4 doubles per molecule: E_xc, N before correction, N after, E_J), enqueued
+ * as one graph launch on the batch's stream; the graph is captured on the first
+ * call with a given set of pointers and replayed afterwards. */
+int dg_batch_build_device(dg_batch* b, const double* dP, double* dJ, double* dXC, double* dscalars);
+/* The same from and to host buffers (synchronous); any output may be NULL. */
+int dg_batch_build(dg_batch* b, const double* P, double* J, double* XC, double* scalars);
+/* Times the candidate contraction chunk lengths and cross-potential item
+ * lengths on the batch (after one build) and keeps the fastest of each;
+ * best_ms (2 doubles, may be NULL) receives their times. */
+int dg_batch_autotune(dg_batch* b, int* chunk_len, int* item_len, double* best_ms);
+
+/* Runs the reference SCF (dg_scf_run) for every molecule of a batch, all of
+ * them advancing together with one dg_batch_build per iteration; each molecule
+ * stops at its own convergence (its later builds are discarded).  results[m]
+ * per molecule (seconds_iter is the batch's wall time per iteration);
+ * iterations (may be NULL) receives the number of batch iterations. */
+int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basis* bases, const dg_grid_params* grid,
+                     const dg_scf_params* scf, dg_scf_result* results, int* iterations);
+
 /* --- host utilities (no GPU needed) ----------------------------------------- */
 /* Overlap, kinetic and nuclear-attraction matrices (nbf x nbf, column-major)
  * of the basis, as DFT::construct_matrices builds them (src/dft.cpp:184-201,

@@ -74,9 +74,17 @@ def test_adjoint_identities_benzene_full_grid():
     assert np.abs(dP_c - dP - expect * Sg).max() < 1e-10 * np.abs(dP).max()
     for gr in (grad, grad_c):
         assert np.abs(gr.sum(0)).max() < 1e-9 * np.abs(gr).max()
-    # symmetric-equivalent carbons carry gradients of equal length
-    nC = np.linalg.norm(grad[:6], axis=1)
-    assert np.abs(nC - nC.mean()).max() < 1e-6 * nC.mean()
+    # the symmetry operations shared by D6h and the octahedral Lebedev grid map
+    # the gradient onto itself: inversion (atom k -> k+3) and the mirror y -> -y
+    # (k -> -k); the 60-degree rotations are not grid symmetries
+    tol = 1e-10 * np.abs(grad).max()
+    for base in (0, 6):
+        for k in range(6):
+            g, gi = grad[base + k], grad[base + (k + 3) % 6]
+            gm = grad[base + (6 - k) % 6]
+            assert np.abs(gi + g).max() < tol
+            assert np.abs(gm - g * [1, -1, 1]).max() < tol
+    assert np.abs(grad[:, 2]).max() < tol
 
 
 def test_forces_by_finite_differences_on_device():
