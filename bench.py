@@ -128,6 +128,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="benzene_631g")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--screen", type=float, default=0.0,
+                    help="amplitude screening threshold of the timed build (0 = dense, as the reference)")
+    ap.add_argument("--variant-screen", type=float, default=1e-10,
+                    help="screening threshold of the screened comparison builds")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision comparison")
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF energy / s-per-iteration runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,10 +178,10 @@ def main():
     C = X @ Cc
     P = C[:, :mol.nelec // 2] @ C[:, :mol.nelec // 2].T
 
-    def timed_builds(precision):
+    def timed_builds(precision, screen):
         """(grid, ptrs, wall seconds, device ms) for warmup + steps builds on one stream."""
         t0 = time.perf_counter()
-        grid = G.MolecularGrid(mol, nrad, nang, lmax, precision=precision)
+        grid = G.MolecularGrid(mol, nrad, nang, lmax, precision=precision, screen=screen)
         setup = time.perf_counter() - t0
         nbf = grid.nbf
         stream = torch.cuda.Stream(device=dev)
@@ -203,7 +207,7 @@ def main():
         return grid, ptrs, (dP, dJ, dX, dS), wall, e0.elapsed_time(e1), setup
 
     prec = G.PREC_MIXED if args.precision == "mixed" else G.PREC_FP64
-    grid, ptrs, bufs, wall, dev_ms, setup_s = timed_builds(prec)
+    grid, ptrs, bufs, wall, dev_ms, setup_s = timed_builds(prec, args.screen)
     nbf, npts = grid.nbf, grid.npts
     ms_per_step = 1e3 * wall / args.steps
     units = float(npts) * nbf * nbf
@@ -225,7 +229,7 @@ def main():
         "data": "synthetic (core-Hamiltonian guess P)",
         "config": {"workload": args.workload, "molecule": mol.name, "basis": mol.basis,
                    "n_bf": nbf, "n_atoms": mol.natoms, "n_radial": nrad, "n_angular": nang,
-                   "lmax": lmax, "n_pts": npts, "precision": args.precision},
+                   "lmax": lmax, "n_pts": npts, "precision": args.precision, "screen": args.screen},
         "device_ms_per_step": dev_ms / args.steps,
         "grid_setup_s": setup_s,
         "e_xc": float(scal[0]),
@@ -285,29 +289,45 @@ def main():
                           "max_abs_grad": float(np.abs(grad).max()),
                           "max_abs_sum_grad": float(np.abs(grad.sum(0)).max())}
 
-    if not args.no_mixed and prec == G.PREC_FP64:
-        # the same build with BF16-split Fock contractions and FP32 high-l Hartree terms
-        gmx, pmx, bmx, wmx, dmx, _ = timed_builds(G.PREC_MIXED)
-        Jm, Xm = bmx[1].cpu().numpy(), bmx[2].cpu().numpy()
+    if not args.no_mixed and prec == G.PREC_FP64 and args.screen == 0.0:
+        # the same build screened in FP64, and with BF16-split Fock contractions (screened and not)
         J6, X6 = bufs[1].cpu().numpy(), bufs[2].cpu().numpy()
-        out["mixed"] = {
-            "value": units * args.steps / wmx, "ms_per_step": 1e3 * wmx / args.steps,
-            "speedup_vs_fp64": wall / wmx,
-            "J_max_rel_err": float(np.abs(Jm - J6).max() / np.abs(J6).max()),
-            "XC_max_rel_err": float(np.abs(Xm - X6).max() / np.abs(X6).max()),
-        }
-        del gmx, bmx
+        out["variants"] = {}
+        for name, pr, sc in (("fp64_screened", G.PREC_FP64, args.variant_screen),
+                             ("mixed", G.PREC_MIXED, 0.0),
+                             ("mixed_screened", G.PREC_MIXED, args.variant_screen)):
+            gv, pv, bv, wv, dv, _ = timed_builds(pr, sc)
+            Jv, Xv = bv[1].cpu().numpy(), bv[2].cpu().numpy()
+            out["variants"][name] = {
+                "precision": "fp64" if pr == G.PREC_FP64 else "mixed", "screen": sc,
+                "value": units * args.steps / wv, "ms_per_step": 1e3 * wv / args.steps,
+                "speedup_vs_dense_fp64": wall / wv,
+                "J_max_rel_err": float(np.abs(Jv - J6).max() / np.abs(J6).max()),
+                "XC_max_rel_err": float(np.abs(Xv - X6).max() / np.abs(X6).max()),
+                "e_xc_err": float(bv[3].cpu().numpy()[0] - scal[0]),
+            }
+            if not args.no_profile:
+                gv.profile_enable(True)
+                gv.profile_reset()
+                for _ in range(3):
+                    gv.build_device(*pv)
+                torch.cuda.synchronize()
+                out["variants"][name]["kernels_ms_per_step"] = {
+                    k: ms / 3 for k, (ms, _) in sorted(gv.profile_read().items())}
+            del gv, bv
 
     if not args.no_scf:
         scf = {}
-        for name, pr in (("fp64", G.PREC_FP64), ("mixed", G.PREC_MIXED)):
-            if name == "mixed" and args.no_mixed:
+        for name, pr, sc in (("fp64", G.PREC_FP64, 0.0), ("mixed", G.PREC_MIXED, 0.0),
+                             ("mixed_screened", G.PREC_MIXED, args.variant_screen)):
+            if name != "fp64" and args.no_mixed:
                 continue
-            en, res, _ = G.scf_run(mol, nrad, nang, lmax, threshold=1e-6, precision=pr)
+            en, res, _ = G.scf_run(mol, nrad, nang, lmax, threshold=1e-6, precision=pr, screen=sc)
             scf[name] = {"energy_Ha": res["energy"], "iterations": res["iterations"],
                          "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"]}
-        if "mixed" in scf:
-            scf["mixed"]["energy_error_vs_fp64_Ha"] = scf["mixed"]["energy_Ha"] - scf["fp64"]["energy_Ha"]
+        for name in ("mixed", "mixed_screened"):
+            if name in scf:
+                scf[name]["energy_error_vs_fp64_Ha"] = scf[name]["energy_Ha"] - scf["fp64"]["energy_Ha"]
         out["scf"] = scf
 
     if not args.no_cpu_baseline:

@@ -187,6 +187,22 @@ __global__ void k_phi(const double* __restrict__ atoms, int nshell, const int* _
             (int)(g % nshell));
 }
 
+// Screening mask: bit T of mask[b] is set when some |phi| of basis tile T
+// (16 functions) exceeds screen on the 64-point block b.  One wave per block.
+__global__ __launch_bounds__(64) void k_block_mask(const double* __restrict__ phi, int ldphi, int ntb,
+                                                   double screen, unsigned* mask) {
+  const long p = (long)blockIdx.x * 64 + threadIdx.x;
+  const double* row = phi + p * ldphi;
+  unsigned bits = 0u;
+  for (int T = 0; T < ntb; ++T) {
+    double m = 0.0;
+    for (int i = 0; i < 16; ++i) m = fmax(m, fabs(row[16 * T + i]));
+    if (m > screen) bits |= 1u << T;
+  }
+  for (int o = 32; o > 0; o >>= 1) bits |= (unsigned)__shfl_xor((int)bits, o);
+  if (threadIdx.x == 0) mask[blockIdx.x] = bits;
+}
+
 // ---------------------------------------------------------------------------
 // Density: rho_p = 2 phi_p^T P phi_p  (src/gridpoint.cpp:82-84)
 // ---------------------------------------------------------------------------
@@ -220,19 +236,25 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // For output tile b the wave forms T_b[p][16b+4v+g] (result register v) with
 // 4(b+1) MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one
 // tile are staged in LDS and shared by the block's 4 waves.  NT = ceil(nbf/16).
+// With screening, mask (one word per 64-point block, bit T for basis tile T)
+// drops the tiles whose amplitudes are negligible on the whole block; the
+// mask is uniform over the work-group, so every branch on it is too.
 template <int NT>
 __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
-                                                 const double* __restrict__ frag, double* rho) {
+                                                 const double* __restrict__ frag, double* rho,
+                                                 const unsigned* __restrict__ mask) {
   __shared__ double pl[4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long p0 = (long)blockIdx.x * 64 + wave * 16;
   const double* row = phi + (p0 + c) * ldphi;
+  const unsigned mk = mask ? mask[blockIdx.x] : 0xffffffffu;
   double f[4 * NT];
 #pragma unroll
-  for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  for (int s = 0; s < 4 * NT; ++s) f[s] = ((mk >> (s >> 2)) & 1u) ? row[16 * (s >> 2) + 4 * g + (s & 3)] : 0.0;
   double acc = 0.0;
   for (int b = 0; b < NT; ++b) {
+    if (!((mk >> b) & 1u)) continue;
     const int ns = 4 * (b + 1);
     __syncthreads();
     const double* src = frag + 128L * b * (b + 1);
@@ -241,7 +263,7 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
     dbl4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4 * NT; ++s)
-      if (s < ns) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
+      if (s < ns && ((mk >> (s >> 2)) & 1u)) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
     // register v of lane (c, g) holds T_b[p0+c][16b + 4v + g]
     const double* rb = row + 16 * b + g;
     acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
@@ -711,28 +733,35 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // B = u[p_g] phi[p_g][16J + c] and receives C[16I + 4v + g][16J + c] in result
 // register v.  The next step's operands are loaded before the current MFMAs.
 // The 4 waves are summed in LDS in a fixed order; part[split][grp][o][t][v][lane].
+// The points of a group are the 64-point blocks blist[loff[grp] .. loff[grp+1])
+// (every block, or with screening the blocks on which tile I and one of the
+// tiles J are significant); split s of gridDim.y takes an equal share of the
+// list, and wave w the 4-point steps 16j + 4w of each of its blocks.
 template <int NOUT>
 __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi, int ldphi,
                                                   const double* __restrict__ u0,
                                                   const double* __restrict__ u1,
-                                                  const int* __restrict__ groups, long chunk,
-                                                  long npts_pad, int ngroups, double* part) {
+                                                  const int* __restrict__ groups,
+                                                  const int* __restrict__ blist,
+                                                  const int* __restrict__ loff, int ngroups, double* part) {
   __shared__ double red[4 * 4 * 4 * 64];
   const int grp = blockIdx.x, split = blockIdx.y;
   const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const long pb = (long)split * chunk;
-  const long pe = min(pb + chunk, npts_pad);  // chunk and npts_pad are multiples of 16
+  const long l0 = loff[grp], cnt = loff[grp + 1] - l0;
+  const int* bl = blist + l0 + cnt * split / gridDim.y;
+  const int nst = 4 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
+  auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
   const int ia = 16 * I + c, jb = 16 * J0 + c;
   dbl4 acc[NOUT][4];
 #pragma unroll
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
-  long p = pb + 4 * wave + g;
   double a = 0.0, bb[4] = {0.0, 0.0, 0.0, 0.0}, w0 = 0.0, w1 = 0.0;
-  if (p < pe) {
+  if (nst > 0) {
+    const long p = point(0);
     const double* row = phi + p * ldphi;
     a = row[ia];
 #pragma unroll
@@ -740,13 +769,13 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
     w0 = u0[p];
     if (NOUT == 2) w1 = u1[p];
   }
-  for (; p < pe; p += 16) {
+  for (int st = 0; st < nst; ++st) {
     const double ca = a, cw0 = w0, cw1 = w1;
     double cb[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) cb[t] = bb[t];
-    const long pn = p + 16;
-    if (pn < pe) {
+    if (st + 1 < nst) {
+      const long pn = point(st + 1);
       const double* row = phi + pn * ldphi;
       a = row[ia];
 #pragma unroll
@@ -1120,20 +1149,24 @@ __global__ void k_phi_split(const double* __restrict__ phi, int ldphi, long npts
 // Phi[p0+8g+e][16I+c], B[8g+e][c] = u Phi[p0+8g+e][16J+c] and receives
 // C[16I + 4g + v][16J + c] in FP32 register v.  The FP32 sums are flushed into
 // FP64 accumulators every 32 steps (1024 points per wave).
+// The 32-point steps of a group are the halves of the 64-point blocks of its
+// list (as in k_contract); wave w takes the steps w, w+4, ... of its share.
 template <int NOUT>
 __global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ phh,
                                                      const __bf16* __restrict__ phl, int ldphi,
                                                      const double* __restrict__ u0,
                                                      const double* __restrict__ u1,
-                                                     const int* __restrict__ groups, long chunk,
-                                                     long npts_pad, int ngroups, double* part) {
+                                                     const int* __restrict__ groups,
+                                                     const int* __restrict__ blist,
+                                                     const int* __restrict__ loff, int ngroups, double* part) {
   __shared__ double red[4 * 4 * 4 * 64];
   const int grp = blockIdx.x, split = blockIdx.y;
   const int J = groups[3 * grp], I0 = groups[3 * grp + 1], nI = groups[3 * grp + 2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const long kb0 = (long)split * chunk / 32;
-  const long kb1 = min((long)split * chunk + chunk, npts_pad) / 32;  // chunk, npts_pad multiples of 128
+  const long l0 = loff[grp], cnt = loff[grp + 1] - l0;
+  const int* bl = blist + l0 + cnt * split / gridDim.y;
+  const int nst = 2 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
   dbl4 accd[NOUT][4];
   flt4 accf[NOUT][4];
 #pragma unroll
@@ -1144,7 +1177,8 @@ __global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ 
       accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
     }
   int steps = 0;
-  for (long kb = kb0 + wave; kb < kb1; kb += 4) {
+  for (int st = wave; st < nst; st += 4) {
+    const long kb = 2L * bl[st >> 1] + (st & 1);
     const long bb = (kb * ldphi + 16 * J + c) * 32 + 8 * g;
     const bf16x8 bh = *(const bf16x8*)(phh + bb);
     const bf16x8 bl = *(const bf16x8*)(phl + bb);
@@ -1346,6 +1380,11 @@ struct dg_handle {
   int* blocks = nullptr;
   int* items = nullptr;
   int nitems = 0, cross_wgs = 0;
+  // contraction block lists (64-point blocks per tile group) and screening masks
+  double screen = 0.0;
+  unsigned* bmask = nullptr;
+  int *clist = nullptr, *cloff = nullptr, *mlist = nullptr, *mloff = nullptr;
+  long nlisted = 0, nlisted_mx = 0;
   // reduced-precision mode
   __bf16 *phh = nullptr, *phl = nullptr;
   int* mgroups = nullptr;
@@ -1392,7 +1431,8 @@ struct dg_handle {
     for (double* b : bufs)
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups, ash_start, ash_list,
-                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0};
+                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0, clist, cloff, mlist, mloff};
+    if (bmask) (void)hipFree(bmask);
     if (vbase) (void)hipFree(vbase);
     if (mol_poff) (void)hipFree(mol_poff);
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1451,10 +1491,28 @@ int run(dg_handle* h, const char* name, F launch) {
 
 unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Per tile group {a, b0, n} (row groups I = a, J = b0.. for the FP64
+// contraction; column groups J = a, I = b0.. for the reduced-precision one),
+// the 64-point blocks on which tile a and one of the tiles b0..b0+n-1 are
+// significant (all blocks without screening), and the list offsets.
+void group_block_lists(const std::vector<unsigned>& mk, const std::vector<int>& grp, std::vector<int>& list,
+                       std::vector<int>& off) {
+  const int ng = (int)grp.size() / 3;
+  off.assign(ng + 1, 0);
+  list.clear();
+  for (int q = 0; q < ng; ++q) {
+    const int a = grp[3 * q], b0 = grp[3 * q + 1], n = grp[3 * q + 2];
+    const unsigned need = ((1u << n) - 1u) << b0;
+    for (size_t b = 0; b < mk.size(); ++b)
+      if (((mk[b] >> a) & 1u) && (mk[b] & need)) list.push_back((int)b);
+    off[q + 1] = (int)list.size();
+  }
+}
+
 template <int NT>
 void launch_density_nt(dg_handle* h) {
   hipLaunchKernelGGL(k_density<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
-                     h->ldphi, h->pfrag, h->rho);
+                     h->ldphi, h->pfrag, h->rho, h->bmask);
 }
 
 int launch_density(dg_handle* h) {
@@ -1632,18 +1690,18 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
       dim3 grid(h->ngroups_mx, h->nsplit_mx);
       if (nout == 2)
         hipLaunchKernelGGL(k_contract_mx<2>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u1,
-                           h->mgroups, h->chunk_mx, h->npts_pad, h->ngroups_mx, h->cpart);
+                           h->mgroups, h->mlist, h->mloff, h->ngroups_mx, h->cpart);
       else
         hipLaunchKernelGGL(k_contract_mx<1>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u0,
-                           h->mgroups, h->chunk_mx, h->npts_pad, h->ngroups_mx, h->cpart);
+                           h->mgroups, h->mlist, h->mloff, h->ngroups_mx, h->cpart);
     } else {
       dim3 grid(h->ngroups, h->nsplit);
       if (nout == 2)
         hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
-                           h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
+                           h->clist, h->cloff, h->ngroups, h->cpart);
       else
         hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
-                           h->chunk_k, h->npts_pad, h->ngroups, h->cpart);
+                           h->clist, h->cloff, h->ngroups, h->cpart);
     }
   }));
   return run(h, "contract_reduce", [&] {
@@ -1730,6 +1788,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     return fail(DG_ERR_UNSUPPORTED, "dg_create: more than 2^31 grid points");
   if (bs && prm->precision != DG_PREC_FP64)
     return fail(DG_ERR_UNSUPPORTED, "dg_batch_create: batches run in FP64");
+  if (!(prm->screen >= 0.0) || !std::isfinite(prm->screen))
+    return fail(DG_ERR_INVALID, "dg_create: screen must be a finite value >= 0");
+  if (bs && prm->screen != 0.0) return fail(DG_ERR_UNSUPPORTED, "dg_batch_create: batches do not screen");
 
   auto h = new dg_handle();
   std::unique_ptr<dg_handle> guard(h);
@@ -1741,6 +1802,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   h->lmax = prm->lmax;
   h->nlm = (prm->lmax + 1) * (prm->lmax + 1);
   h->precision = prm->precision;
+  h->screen = prm->screen;
   h->cell = (long)h->nrad * h->nang;
   h->npts = h->cell * h->natoms;
   h->npts_pad = round_up(h->npts, 256);
@@ -2058,6 +2120,34 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       hipLaunchKernelGGL(k_phi_split, dim3(nblocks(np * h->ldphi, 256)), dim3(256), 0, h->stream, h->phi,
                          h->ldphi, np, h->phh, h->phl);
     }));
+  }
+  {
+    const long nblk = np / 64;
+    std::vector<unsigned> mk(nblk, 0xffffffffu);
+    if (h->screen > 0.0) {
+      DG_TRY(dalloc(&h->bmask, nblk));
+      DG_TRY(run(h, "block_mask", [&] {
+        hipLaunchKernelGGL(k_block_mask, dim3(nblk), dim3(64), 0, h->stream, h->phi, h->ldphi, h->ntb, h->screen,
+                           h->bmask);
+      }));
+      HIP_TRY(hipMemcpyAsync(mk.data(), h->bmask, nblk * 4, hipMemcpyDeviceToHost, h->stream));
+      DG_TRY(sync(h));
+    }
+    std::vector<int> list, off;
+    group_block_lists(mk, blocks, list, off);
+    h->nlisted = (long)list.size();
+    DG_TRY(dalloc(&h->clist, std::max<size_t>(list.size(), 1)));
+    DG_TRY(dalloc(&h->cloff, off.size()));
+    HIP_TRY(hipMemcpy(h->clist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->cloff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    if (h->precision == DG_PREC_MIXED) {
+      group_block_lists(mk, mgroups, list, off);
+      h->nlisted_mx = (long)list.size();
+      DG_TRY(dalloc(&h->mlist, std::max<size_t>(list.size(), 1)));
+      DG_TRY(dalloc(&h->mloff, off.size()));
+      HIP_TRY(hipMemcpy(h->mlist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(h->mloff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   DG_TRY(sync(h));
   guard.release();

@@ -428,7 +428,7 @@ class DFT : public HostSCF {
  public:
   DFT(const MoleculeData& mol, const dg_grid_params& grid)
       : HostSCF(mol, grid.precision == DG_PREC_MIXED), grid_(mol) {
-    grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision);
+    grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision, grid.screen);
     grid_.create_grid();
   }
   ~DFT() {

@@ -56,7 +56,7 @@ class _Basis(C.Structure):
 
 class _GridParams(C.Structure):
     _fields_ = [("n_radial", C.c_int), ("n_angular", C.c_int), ("lmax", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("screen", C.c_double)]
 
 
 class _ScfParams(C.Structure):
@@ -152,7 +152,7 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
-def _c_inputs(mol, nrad, nang, lmax, precision):
+def _c_inputs(mol, nrad, nang, lmax, precision, screen=0.0):
     s = mol.shells()
     keep = {"Z": _i32(mol.Z), "xyz": _f64(mol.xyz), "atom": _i32(s["atom"]), "L": _i32(s["L"]),
             "nprim": _i32(s["nprim"]), "offset": _i32(s["offset"]), "alpha": _f64(s["alpha"]),
@@ -160,7 +160,7 @@ def _c_inputs(mol, nrad, nang, lmax, precision):
     m = _Molecule(mol.natoms, _i(keep["Z"]), _d(keep["xyz"]))
     b = _Basis(len(keep["atom"]), _i(keep["atom"]), _i(keep["L"]), _i(keep["nprim"]),
                _i(keep["offset"]), _d(keep["alpha"]), _d(keep["coef"]))
-    g = _GridParams(nrad, nang, lmax, precision)
+    g = _GridParams(nrad, nang, lmax, precision, screen)
     return m, b, g, keep
 
 
@@ -198,13 +198,15 @@ class MolecularGrid:
     directly, which builds the grid at once.
     """
 
-    def __init__(self, mol, radial_points=None, angular_points=None, lmax=8, precision=PREC_FP64):
+    def __init__(self, mol, radial_points=None, angular_points=None, lmax=8, precision=PREC_FP64,
+                 screen=0.0):
         self.mol = mol
         self.h = None
         self.radial_points = radial_points
         self.angular_points = angular_points
         self.lmax = lmax
         self.precision = precision
+        self.screen = screen
         if radial_points is not None and angular_points is not None:
             self.create_grid()
 
@@ -218,7 +220,7 @@ class MolecularGrid:
         if self.h is not None:
             raise RuntimeError("grid already created")
         m, b, g, keep = _c_inputs(self.mol, self.radial_points, self.angular_points, self.lmax,
-                                  self.precision)
+                                  self.precision, self.screen)
         h = C.c_void_p()
         _check(lib().dg_create(C.byref(m), C.byref(b), C.byref(g), C.byref(h)))
         self.h = h
@@ -494,11 +496,11 @@ def eigh(A):
 
 
 def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5,
-            precision=PREC_FP64):
+            precision=PREC_FP64, screen=0.0):
     """Reference SCF (src/dft.cpp:84-142) with the grid work on the GPU.
 
     Returns (energies per iteration, result dict, P)."""
-    m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision)
+    m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision, screen)
     sp = _ScfParams(threshold, min_iter, max_iter, mixing)
     res = _ScfResult()
     en = np.zeros(max_iter)

@@ -64,8 +64,8 @@ class MolecularGrid {
   }
   // the same with the angular rule given by its point count (e.g. 302)
   void set_grid_points(unsigned radial_points, unsigned angular_points, unsigned lmax,
-                       int precision = DG_PREC_FP64) {
-    prm_ = dg_grid_params{(int)radial_points, (int)angular_points, (int)lmax, precision};
+                       int precision = DG_PREC_FP64, double screen = 0.0) {
+    prm_ = dg_grid_params{(int)radial_points, (int)angular_points, (int)lmax, precision, screen};
   }
   // src/moleculargrid.h:118
   void create_grid() {
@@ -123,7 +123,7 @@ class MolecularGrid {
     return v;
   }
   MoleculeData mol_;
-  dg_grid_params prm_{15, 110, 8, DG_PREC_FP64};  // reference "medium" grid
+  dg_grid_params prm_{15, 110, 8, DG_PREC_FP64, 0.0};  // reference "medium" grid
   dg_handle* h_ = nullptr;
   int nbf_ = 0;
   long npts_ = 0;

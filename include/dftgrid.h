@@ -66,6 +66,12 @@ typedef struct {
   int n_angular;  /* Lebedev points per shell (a point count, e.g. 110, 302) */
   int lmax;       /* spherical-harmonic cut-off of the Hartree expansion */
   int precision;  /* DG_PREC_FP64 or DG_PREC_MIXED */
+  /* Amplitude screening (no counterpart in the reference, which sums every
+   * basis function at every point): 0 keeps every product; screen > 0 skips,
+   * in the density and in the Fock contractions, each 64-point block's 16-wide
+   * groups of basis functions whose |phi| stays at or below screen on the
+   * whole block.  Single-molecule handles only. */
+  double screen;
 } dg_grid_params;
 
 /* Message of the last failing call on this thread ("" if none). */
