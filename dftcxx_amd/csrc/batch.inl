@@ -257,14 +257,19 @@ int batch_mol_of_atom(const dg_handle* h, int k) {
 }
 
 // Work list of the cross-atom potentials: for every atom k, the points of its
-// molecule outside its own cell, in ranges of at most len points, k ascending.
+// molecule outside its own cell, in ranges of at most len (<= kMaxItem) points,
+// k ascending: {k, p_begin, p_end, offset of the item's point order in perm}.
 std::vector<int> cross_items(const dg_handle* h, int len) {
   std::vector<int> items;
+  long off = 0;
   auto add = [&](int k, long b, long e) {
     for (long p = b; p < e; p += len) {
+      const long q = std::min(e, p + len);
       items.push_back(k);
       items.push_back((int)p);
-      items.push_back((int)std::min(e, p + len));
+      items.push_back((int)q);
+      items.push_back((int)off);
+      off += q - p;
     }
   };
   for (int k = 0; k < h->natoms; ++k) {
@@ -286,7 +291,8 @@ int set_items(dg_handle* h, int len) {
   if (items.size() > h->items_cap) return fail(DG_ERR_INVALID, "cross-potential item list exceeds its buffer");
   if (!items.empty())
     HIP_TRY(hipMemcpy(h->items, items.data(), items.size() * 4, hipMemcpyHostToDevice));
-  h->nitems = (int)items.size() / 3;
+  h->nitems = (int)items.size() / 4;
+  DG_TRY(sort_items(h));
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(h->cross_lds, 1)));
@@ -323,7 +329,7 @@ int set_bchunks(dg_handle* h, int len) {
 }
 
 constexpr int kChunkCandidates[] = {256, 512, 1024, 2048, 4096};
-constexpr int kItemCandidates[] = {256, 512, 1024, 2048, 4096};
+constexpr int kItemCandidates[] = {1024, 2048, 4096};
 
 int batch_alloc(dg_handle* h, const BatchSpec* bs, const std::vector<int>&) {
   const int nm = bs->nmol;

@@ -538,23 +538,74 @@ struct CrossArgs {
   const double* invh;   // 1/(x_t - x_{t-1})
   const double* tab;    // [natoms][nrad][nlm]{y, D}
   const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
-  const int* items;     // {k, p_begin, p_end} point ranges outside cell k, k ascending
+  const int* items;     // {k, p_begin, p_end, perm offset} point ranges outside cell k, k ascending
+  const unsigned short* perm;  // per item, its points in order of distance to atom k
   double* vpart;        // partial potentials, atom k's at vpart + vbase[k] + p
   const long* vbase;
   int nitems, nrad, nlm, l0, l1, accumulate;
 };
 
+// Points of every cross-potential item in order of their distance to the
+// item's atom: perm[off + i] is the offset (from p_begin) of the i-th nearest
+// point.  One work-group per item (at most kMaxItem points), bitonic sort in LDS.
+constexpr int kMaxItem = 4096;
+__global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ items, const double* __restrict__ atoms,
+                                                     const double* __restrict__ px, const double* __restrict__ py,
+                                                     const double* __restrict__ pz, unsigned short* perm) {
+  __shared__ double key[kMaxItem];
+  __shared__ unsigned short idx[kMaxItem];
+  const int* it = items + 4 * blockIdx.x;
+  const int k = it[0];
+  const long pb = it[1];
+  const int len = it[2] - it[1];
+  int n2 = 1;
+  while (n2 < len) n2 <<= 1;
+  const double ax = atoms[4 * k], ay = atoms[4 * k + 1], az = atoms[4 * k + 2];
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+    double r = 1e300;
+    if (i < len) {
+      const double dx = px[pb + i] - ax, dy = py[pb + i] - ay, dz = pz[pb + i] - az;
+      r = dx * dx + dy * dy + dz * dz;
+    }
+    key[i] = r;
+    idx[i] = (unsigned short)i;
+  }
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const double ki = key[i], kj = key[j];
+          if ((ki > kj) == up && ki != kj) {
+            key[i] = kj;
+            key[j] = ki;
+            const unsigned short t = idx[i];
+            idx[i] = idx[j];
+            idx[j] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) perm[it[3] + i] = idx[i];
+}
+
 // Potential of atom k's spherical-harmonic expansion at every point outside
 // cell k: V_k(p) = sum_lm Y_lm(r_p - R_k) S_{k,lm}(|r_p - R_k|) / |r_p - R_k|
 // (src/moleculargrid.cpp:349-376).  Persistent: work-group g walks the items
 // [g n/G, (g+1) n/G) of the (atom, point range) list, reloading atom k's spline
-// table for l in [l0, l1) into LDS when k changes.  Each thread runs the
-// associated-Legendre / Chebyshev recurrences once per (p, k) and forms the
-// four sums sum Y y_{s-1}, sum Y D_{s-1}, sum Y y_s, sum Y D_s, from which the
-// cubic on interval s is evaluated once (linear in the per-lm coefficients).
+// table for l in [l0, l1) into LDS when k changes.  Each thread takes Q
+// consecutive points of the item in order of distance to atom k (perm), runs
+// the associated-Legendre / Chebyshev recurrences for each and forms the four
+// sums sum Y y_{s-1}, sum Y D_{s-1}, sum Y y_s, sum Y D_s per point, from which
+// the cubic on interval s is evaluated once (linear in the per-lm
+// coefficients).  When the Q points share their interval (the usual case for
+// distance-sorted points) each table entry is read from LDS once for all Q.
 // The interval is the reference's first knot with r <= x_s, found by
 // bisection; r outside [x_0, x_max) clamps to the end values (src/cspline.cpp:156-168).
-template <int LMAX>
+template <int LMAX, int Q>
 __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   extern __shared__ double lds[];
   const int n = A.nrad;
@@ -570,114 +621,175 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   const int it1 = (int)((long)(blockIdx.x + 1) * A.nitems / gridDim.x);
   int cur = -1;
   for (int it = it0; it < it1; ++it) {
-  const int k = A.items[3 * it];
-  if (k != cur) {
-    __syncthreads();
-    const double* src = A.tab + (long)k * n * A.nlm * 2;
-    for (int i = threadIdx.x; i < n * nlmc * 2; i += blockDim.x) {
-      const int t = i / (nlmc * 2), rem = i % (nlmc * 2);
-      tab[i] = src[(long)t * A.nlm * 2 + lmb * 2 + rem];
-    }
-    __syncthreads();
-    cur = k;
-  }
-  const long pb = A.items[3 * it + 1], pe = A.items[3 * it + 2];
-  const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
-  const double xlo = kn[0], xhi = kn[n - 1];
-  for (long p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-    const double dx = A.px[p] - Ax, dy = A.py[p] - Ay, dz = A.pz[p] - Az;
-    const double rxy2 = dx * dx + dy * dy;
-    const double r = sqrt(rxy2 + dz * dz);
-    const double rinv = 1.0 / r;
-    const double rxy = sqrt(rxy2);
-    const double ct = dz * rinv, st = rxy * rinv;
-    double cp = 1.0, sp = 0.0;
-    if (rxy > 0.0) {
-      const double ir = 1.0 / rxy;
-      cp = dx * ir;
-      sp = dy * ir;
-    }
-    int s0, s1;
-    double t = 0.0;
-    bool clamp = true;
-    if (r < xlo) {
-      s0 = s1 = 0;
-    } else if (r >= xhi) {
-      s0 = s1 = n - 1;
-    } else {
-      int lo = 1, hi = n - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (r <= kn[mid])
-          hi = mid;
-        else
-          lo = mid + 1;
+    const int k = A.items[4 * it];
+    if (k != cur) {
+      __syncthreads();
+      const double* src = A.tab + (long)k * n * A.nlm * 2;
+      for (int i = threadIdx.x; i < n * nlmc * 2; i += blockDim.x) {
+        const int t = i / (nlmc * 2), rem = i % (nlmc * 2);
+        tab[i] = src[(long)t * A.nlm * 2 + lmb * 2 + rem];
       }
-      s1 = lo;
-      s0 = lo - 1;
-      t = r - kn[s0];
-      clamp = false;
+      __syncthreads();
+      cur = k;
     }
-    const double* T0 = tab + s0 * nlmc * 2;
-    const double* T1 = tab + s1 * nlmc * 2;
-    double sy0 = 0.0, sd0 = 0.0, sy1 = 0.0, sd1 = 0.0;
-    double pmm = 1.0, cm = 1.0, sm = 0.0;
+    const long pb = A.items[4 * it + 1];
+    const int len = A.items[4 * it + 2] - A.items[4 * it + 1];
+    const unsigned short* pm = A.perm + A.items[4 * it + 3];
+    const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
+    const double xlo = kn[0], xhi = kn[n - 1];
+    for (int q0 = threadIdx.x * Q; q0 < len; q0 += blockDim.x * Q) {
+      long pq[Q];
+      double rinv[Q], ct[Q], st[Q], cp[Q], sp[Q], tq[Q];
+      int s0[Q], s1[Q];
+      bool clamp[Q];
 #pragma unroll
-    for (int m = 0; m <= LMAX; ++m) {
-      if (m > 0) {
-        pmm *= -(double)(2 * m - 1) * st;
-        const double c2 = cm * cp - sm * sp;
-        sm = sm * cp + cm * sp;
-        cm = c2;
+      for (int q = 0; q < Q; ++q) {
+        const int pos = (q0 + q < len) ? q0 + q : q0;  // a missing tail point repeats the first
+        pq[q] = pb + pm[pos];
+        const double dx = A.px[pq[q]] - Ax, dy = A.py[pq[q]] - Ay, dz = A.pz[pq[q]] - Az;
+        const double rxy2 = dx * dx + dy * dy;
+        const double r = sqrt(rxy2 + dz * dz);
+        rinv[q] = 1.0 / r;
+        const double rxy = sqrt(rxy2);
+        ct[q] = dz * rinv[q];
+        st[q] = rxy * rinv[q];
+        cp[q] = 1.0;
+        sp[q] = 0.0;
+        if (rxy > 0.0) {
+          const double ir = 1.0 / rxy;
+          cp[q] = dx * ir;
+          sp[q] = dy * ir;
+        }
+        tq[q] = 0.0;
+        clamp[q] = true;
+        if (r < xlo) {
+          s0[q] = s1[q] = 0;
+        } else if (r >= xhi) {
+          s0[q] = s1[q] = n - 1;
+        } else {
+          int lo = 1, hi = n - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (r <= kn[mid])
+              hi = mid;
+            else
+              lo = mid + 1;
+          }
+          s1[q] = lo;
+          s0[q] = lo - 1;
+          tq[q] = r - kn[s0[q]];
+          clamp[q] = false;
+        }
       }
-      double p1 = 0.0, p2 = 0.0;
+      bool same = true;
 #pragma unroll
-      for (int l = m; l <= LMAX; ++l) {
-        const int ci = 3 * (l * (l + 1) / 2 + m);
-        double pl;
-        if (l == m)
-          pl = pmm;
-        else if (l == m + 1)
-          pl = ct * (double)(2 * m + 1) * pmm;
-        else
-          pl = A.shc[ci + 1] * ct * p1 - A.shc[ci + 2] * p2;
-        p2 = p1;
-        p1 = pl;
-        if (l >= A.l0 && l < A.l1) {
-          const double base = A.shc[ci] * pl;
-          const int ip = (l * l + l + m - lmb) * 2;
-          const double yp = (m == 0) ? base : base * cm;
-          sy0 += yp * T0[ip];
-          sd0 += yp * T0[ip + 1];
-          sy1 += yp * T1[ip];
-          sd1 += yp * T1[ip + 1];
-          if (m > 0) {
+      for (int q = 1; q < Q; ++q) same = same && s0[q] == s0[0] && s1[q] == s1[0];
+      double sy0[Q], sd0[Q], sy1[Q], sd1[Q], pmm[Q], cm[Q], sm[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        sy0[q] = sd0[q] = sy1[q] = sd1[q] = 0.0;
+        pmm[q] = 1.0;
+        cm[q] = 1.0;
+        sm[q] = 0.0;
+      }
+#pragma unroll
+      for (int m = 0; m <= LMAX; ++m) {
+        if (m > 0) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            pmm[q] *= -(double)(2 * m - 1) * st[q];
+            const double c2 = cm[q] * cp[q] - sm[q] * sp[q];
+            sm[q] = sm[q] * cp[q] + cm[q] * sp[q];
+            cm[q] = c2;
+          }
+        }
+        double p1[Q], p2[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) p1[q] = p2[q] = 0.0;
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l) {
+          const int ci = 3 * (l * (l + 1) / 2 + m);
+          double yp[Q], yn[Q];
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            double pl;
+            if (l == m)
+              pl = pmm[q];
+            else if (l == m + 1)
+              pl = ct[q] * (double)(2 * m + 1) * pmm[q];
+            else
+              pl = A.shc[ci + 1] * ct[q] * p1[q] - A.shc[ci + 2] * p2[q];
+            p2[q] = p1[q];
+            p1[q] = pl;
+            const double base = A.shc[ci] * pl;
+            yp[q] = (m == 0) ? base : base * cm[q];
+            yn[q] = base * sm[q];
+          }
+          if (l >= A.l0 && l < A.l1) {
+            const int ip = (l * l + l + m - lmb) * 2;
             const int in = (l * l + l - m - lmb) * 2;
-            const double yn = base * sm;
-            sy0 += yn * T0[in];
-            sd0 += yn * T0[in + 1];
-            sy1 += yn * T1[in];
-            sd1 += yn * T1[in + 1];
+            if (same) {
+              const double* T0 = tab + s0[0] * nlmc * 2;
+              const double* T1 = tab + s1[0] * nlmc * 2;
+              const double a0 = T0[ip], a1 = T0[ip + 1], b0 = T1[ip], b1 = T1[ip + 1];
+#pragma unroll
+              for (int q = 0; q < Q; ++q) {
+                sy0[q] += yp[q] * a0;
+                sd0[q] += yp[q] * a1;
+                sy1[q] += yp[q] * b0;
+                sd1[q] += yp[q] * b1;
+              }
+              if (m > 0) {
+                const double c0 = T0[in], c1 = T0[in + 1], e0 = T1[in], e1 = T1[in + 1];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                  sy0[q] += yn[q] * c0;
+                  sd0[q] += yn[q] * c1;
+                  sy1[q] += yn[q] * e0;
+                  sd1[q] += yn[q] * e1;
+                }
+              }
+            } else {
+#pragma unroll
+              for (int q = 0; q < Q; ++q) {
+                const double* T0 = tab + s0[q] * nlmc * 2;
+                const double* T1 = tab + s1[q] * nlmc * 2;
+                sy0[q] += yp[q] * T0[ip];
+                sd0[q] += yp[q] * T0[ip + 1];
+                sy1[q] += yp[q] * T1[ip];
+                sd1[q] += yp[q] * T1[ip + 1];
+                if (m > 0) {
+                  sy0[q] += yn[q] * T0[in];
+                  sd0[q] += yn[q] * T0[in + 1];
+                  sy1[q] += yn[q] * T1[in];
+                  sd1[q] += yn[q] * T1[in + 1];
+                }
+              }
+            }
           }
         }
       }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (q0 + q >= len) break;
+        double val;
+        if (clamp[q]) {
+          val = sy0[q];
+        } else {
+          const double hi_ = ih[s1[q]];
+          const double t = tq[q];
+          const double dy = (sy1[q] - sy0[q]) * hi_;
+          const double c = hi_ * (3.0 * dy - 2.0 * sd0[q] - sd1[q]);
+          const double d = hi_ * hi_ * (-2.0 * dy + sd0[q] + sd1[q]);
+          val = sy0[q] + sd0[q] * t + c * t * t + d * t * t * t;
+        }
+        double* out = A.vpart + A.vbase[k] + pq[q];
+        if (A.accumulate)
+          *out += val * rinv[q];
+        else
+          *out = val * rinv[q];
+      }
     }
-    double val;
-    if (clamp) {
-      val = sy0;
-    } else {
-      const double hi_ = ih[s1];
-      const double dy = (sy1 - sy0) * hi_;
-      const double c = hi_ * (3.0 * dy - 2.0 * sd0 - sd1);
-      const double d = hi_ * hi_ * (-2.0 * dy + sd0 + sd1);
-      val = sy0 + sd0 * t + c * t * t + d * t * t * t;
-    }
-    double* out = A.vpart + A.vbase[k] + p;
-    if (A.accumulate)
-      *out += val * rinv;
-    else
-      *out = val * rinv;
-  }
   }
 }
 
@@ -1379,6 +1491,8 @@ struct dg_handle {
   double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
   int* blocks = nullptr;
   int* items = nullptr;
+  unsigned short* perm = nullptr;  // per item, point order by distance to the item's atom
+  long npairs = 0;
   int nitems = 0, cross_wgs = 0;
   // contraction block lists (64-point blocks per tile group) and screening masks
   double screen = 0.0;
@@ -1400,7 +1514,7 @@ struct dg_handle {
   double *mol_z = nullptr, *mol_fac = nullptr, *mol_qraw = nullptr, *atom_exc = nullptr, *atom_ej = nullptr;
   int* bchunks = nullptr;   // contraction chunks {m, p_begin, p_end}
   int* mol_chunk0 = nullptr;
-  int nbchunks = 0, max_mol_shells = 0, max_mol_atoms = 0, bchunk_len = 2048, item_len = 1024;
+  int nbchunks = 0, max_mol_shells = 0, max_mol_atoms = 0, bchunk_len = 2048, item_len = 2048;
   long packed = 0;  // sum of nbf_m^2
   std::vector<int> h_mol_atom0, h_mol_nbf;
   std::vector<long> h_mol_poff;
@@ -1434,6 +1548,7 @@ struct dg_handle {
                     atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0, clist, cloff, mlist, mloff};
     if (bmask) (void)hipFree(bmask);
     if (vbase) (void)hipFree(vbase);
+    if (perm) (void)hipFree(perm);
     if (mol_poff) (void)hipFree(mol_poff);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1507,6 +1622,15 @@ void group_block_lists(const std::vector<unsigned>& mk, const std::vector<int>& 
       if (((mk[b] >> a) & 1u) && (mk[b] & need)) list.push_back((int)b);
     off[q + 1] = (int)list.size();
   }
+}
+
+// Orders the points of every cross-potential item by distance to its atom.
+int sort_items(dg_handle* h) {
+  if (h->nitems == 0) return DG_OK;
+  return run(h, "sort_items", [&] {
+    hipLaunchKernelGGL(k_sort_items, dim3(h->nitems), dim3(1024), 0, h->stream, h->items, h->atoms, h->px, h->py,
+                       h->pz, h->perm);
+  });
 }
 
 template <int NT>
@@ -1589,15 +1713,18 @@ int stage_correct(dg_handle* h) {
   return DG_OK;
 }
 
+// points per thread of the cross-potential kernel (items hold up to 1024 kCrossQ points)
+constexpr int kCrossQ = 2;
+
 template <int L>
 void launch_cross_l(dg_handle* h, CrossArgs& a, dim3 grid) {
-  hipLaunchKernelGGL(k_hartree_cross<L>, grid, dim3(1024), h->cross_lds, h->stream, a);
+  hipLaunchKernelGGL((k_hartree_cross<L, kCrossQ>), grid, dim3(1024), h->cross_lds, h->stream, a);
 }
 
 template <int L>
 hipError_t allow_cross_lds(size_t bytes) {
-  return hipFuncSetAttribute((const void*)k_hartree_cross<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes);
+  return hipFuncSetAttribute((const void*)k_hartree_cross<L, kCrossQ>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t allow_cross_lds_l(int lmax, size_t bytes) {
@@ -1652,6 +1779,7 @@ int stage_hartree(dg_handle* h) {
     a.tab = h->tab;
     a.shc = h->shc;
     a.items = h->items;
+    a.perm = h->perm;
     a.vpart = h->vpart;
     a.vbase = h->vbase;
     a.nitems = h->nitems;
@@ -1987,9 +2115,10 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     for (int k = 0; k < h->natoms; ++k) vbase[k] = (long)k * h->npts_pad;
     vtotal = (long)h->natoms * h->npts_pad;
   }
-  std::vector<int> items = cross_items(h, 1024);
-  h->nitems = (int)items.size() / 3;
-  h->items_cap = std::max<size_t>(bs ? cross_items(h, 256).size() : items.size(), 3);
+  std::vector<int> items = cross_items(h, 1024 * kCrossQ);
+  h->nitems = (int)items.size() / 4;
+  h->items_cap = std::max<size_t>(bs ? cross_items(h, 1024).size() : items.size(), 4);
+  h->npairs = items.empty() ? 0 : (long)items[items.size() - 1] + items[items.size() - 2] - items[items.size() - 3];
   {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -2043,6 +2172,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, h->items_cap));
+  HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
   DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit * h->ngroups, (size_t)h->nsplit_mx * h->ngroups_mx) * 2 * 1024));
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
@@ -2121,6 +2251,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
                          h->ldphi, np, h->phh, h->phl);
     }));
   }
+  DG_TRY(sort_items(h));
   {
     const long nblk = np / 64;
     std::vector<unsigned> mk(nblk, 0xffffffffu);

@@ -111,7 +111,10 @@ class Grid:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_grid_destroy(self.h)
+            try:
+                lib().or_grid_destroy(self.h)
+            except Exception:  # interpreter shutdown: the library is already gone
+                pass
             self.h = None
 
     def points(self):
