@@ -225,7 +225,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == G.PREC_FP64 else "f64+bf16",
+        "dtype": "f64" if prec == G.PREC_FP64 else "f64+f32",
         "data": "synthetic (core-Hamiltonian guess P)",
         "config": {"workload": args.workload, "molecule": mol.name, "basis": mol.basis,
                    "n_bf": nbf, "n_atoms": mol.natoms, "n_radial": nrad, "n_angular": nang,

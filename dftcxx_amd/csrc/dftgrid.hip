@@ -49,7 +49,6 @@ namespace {
   } while (0)
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
@@ -1111,15 +1110,27 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
     const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
     dist[m * 64 + lane] = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
   }
-  double Z = 0.0;
-  for (int n = 0; n < natoms; ++n) {
+  // Cell functions, the nearest atom first: its P is a lower bound of Z, and
+  // the product of another atom stops once it is below 1e-4 kBeckeSkip of
+  // that bound (its factor for the nearest atom comes first, so a far atom
+  // stops at once); such an atom is below the skip level and its remaining
+  // product (at most the truncated one) changes Z by less than 1e-24 Z per atom.
+  int near = 0;
+  for (int m = 1; m < natoms; ++m)
+    if (dist[m * 64 + lane] < dist[near * 64 + lane]) near = m;
+  double Z = 0.0, cut = 0.0;
+  for (int r = 0; r < natoms; ++r) {
+    const int n = (r == 0) ? near : (r <= near ? r - 1 : r);
+    const double dn = dist[n * 64 + lane];
     double prod = 1.0;
-    for (int m = 0; m < natoms; ++m) {
-      if (m == n) continue;
-      prod *= becke_s((dist[n * 64 + lane] - dist[m * 64 + lane]) * invR[n * natoms + m]);
+    if (n != near) prod = becke_s((dn - dist[near * 64 + lane]) * invR[n * natoms + near]);
+    for (int m = 0; m < natoms && prod >= cut; ++m) {
+      if (m == n || m == near) continue;
+      prod *= becke_s((dn - dist[m * 64 + lane]) * invR[n * natoms + m]);
     }
     Pn[n * 64 + lane] = prod;
     Z += prod;
+    if (r == 0) cut = 1e-4 * kBeckeSkip * prod;
   }
   const double skip = kBeckeSkip * Z;
   int nact = 0;
@@ -1236,49 +1247,41 @@ __global__ void k_adjoint_weights(const double* __restrict__ uxc, const double* 
   udp[p] = 2.0 * c * wv;
 }
 
-// Phi split for the reduced-precision contractions: hi = bf16(phi),
-// lo = bf16(phi - hi), stored point-blocked as [p/32][col][32] so that the 8
-// consecutive points a lane feeds to one BF16 MFMA are one 16-byte load.
-__global__ void k_phi_split(const double* __restrict__ phi, int ldphi, long npts_pad, __bf16* hi,
-                            __bf16* lo) {
+// FP32 copy of the amplitudes for the reduced-precision contractions.
+__global__ void k_phi_f32(const double* __restrict__ phi, long n, float* phf) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= npts_pad * ldphi) return;
-  const long p = e / ldphi;
-  const int col = (int)(e % ldphi);
-  const double v = phi[e];
-  const __bf16 h = (__bf16)(float)v;
-  const __bf16 l = (__bf16)(float)(v - (double)(float)h);
-  const long o = ((p >> 5) * ldphi + col) * 32 + (p & 31);
-  hi[o] = h;
-  lo[o] = l;
+  if (e < n) phf[e] = (float)phi[e];
 }
 
-// Reduced-precision contraction C_o = Phi^T diag(u_o) Phi with the BF16
-// 16x16x32 MFMA (FP32 accumulation).  Column groups of up to 4 tiles
-// (I0..I0+nI-1; J) share the B operand B = u Phi_J, which is formed in FP32
-// from hi + lo and split again into BF16 hi/lo in registers; each tile takes
-// A_hi B_hi + A_hi B_lo + A_lo B_hi.  Lane (c, g) feeds A[c][8g+e] =
-// Phi[p0+8g+e][16I+c], B[8g+e][c] = u Phi[p0+8g+e][16J+c] and receives
-// C[16I + 4g + v][16J + c] in FP32 register v.  The FP32 sums are flushed into
-// FP64 accumulators every 32 steps (1024 points per wave).
-// The 32-point steps of a group are the halves of the 64-point blocks of its
-// list (as in k_contract); wave w takes the steps w, w+4, ... of its share.
+__device__ __forceinline__ flt4 mfma16x16x4f32(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Reduced-precision contraction C_o = Phi^T diag(u_o) Phi: the k_contract
+// scheme with FP32 operands (Phi in FP32, u_o Phi formed in FP32) on the FP32
+// 16x16x4 MFMA.  The FP32 accumulators are added into FP64 ones every
+// kFlushSteps steps (64 points per wave), so the FP32 sums stay short.  The
+// FP32 result register v of lane (c, g) holds C[16I + 4g + v][16J + c]; it is
+// stored in the FP64 kernel's order (row 4v + g of register v) so that
+// k_contract_reduce serves both.
+constexpr int kFlushSteps = 16;
 template <int NOUT>
-__global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ phh,
-                                                     const __bf16* __restrict__ phl, int ldphi,
-                                                     const double* __restrict__ u0,
-                                                     const double* __restrict__ u1,
-                                                     const int* __restrict__ groups,
-                                                     const int* __restrict__ blist,
-                                                     const int* __restrict__ loff, int ngroups, double* part) {
+__global__ __launch_bounds__(256) void k_contract_f32(const float* __restrict__ phf, int ldphi,
+                                                      const double* __restrict__ u0,
+                                                      const double* __restrict__ u1,
+                                                      const int* __restrict__ groups,
+                                                      const int* __restrict__ blist,
+                                                      const int* __restrict__ loff, int ngroups, double* part) {
   __shared__ double red[4 * 4 * 4 * 64];
   const int grp = blockIdx.x, split = blockIdx.y;
-  const int J = groups[3 * grp], I0 = groups[3 * grp + 1], nI = groups[3 * grp + 2];
+  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long l0 = loff[grp], cnt = loff[grp + 1] - l0;
   const int* bl = blist + l0 + cnt * split / gridDim.y;
-  const int nst = 2 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
+  const int nst = 4 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
+  auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
+  const int ia = 16 * I + c, jb = 16 * J0 + c;
   dbl4 accd[NOUT][4];
   flt4 accf[NOUT][4];
 #pragma unroll
@@ -1288,45 +1291,38 @@ __global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ 
       accd[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
       accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
     }
-  int steps = 0;
-  for (int st = wave; st < nst; st += 4) {
-    const long kb = 2L * bl[st >> 1] + (st & 1);
-    const long bb = (kb * ldphi + 16 * J + c) * 32 + 8 * g;
-    const bf16x8 bh = *(const bf16x8*)(phh + bb);
-    const bf16x8 bl = *(const bf16x8*)(phl + bb);
-    const double* up0 = u0 + kb * 32 + 8 * g;
-    const double* up1 = u1 + kb * 32 + 8 * g;
-    bf16x8 b0h, b0l, b1h, b1l;
+  float a = 0.f, bb[4] = {0.f, 0.f, 0.f, 0.f}, w0 = 0.f, w1 = 0.f;
+  if (nst > 0) {
+    const long p = point(0);
+    const float* row = phf + p * ldphi;
+    a = row[ia];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const float f = (float)bh[e] + (float)bl[e];
-      const float x0 = (float)up0[e] * f;
-      b0h[e] = (__bf16)x0;
-      b0l[e] = (__bf16)(x0 - (float)b0h[e]);
-      if (NOUT == 2) {
-        const float x1 = (float)up1[e] * f;
-        b1h[e] = (__bf16)x1;
-        b1l[e] = (__bf16)(x1 - (float)b1h[e]);
-      }
+    for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.f;
+    w0 = (float)u0[p];
+    if (NOUT == 2) w1 = (float)u1[p];
+  }
+  for (int st = 0; st < nst; ++st) {
+    const float ca = a, cw0 = w0, cw1 = w1;
+    float cb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) cb[t] = bb[t];
+    if (st + 1 < nst) {
+      const long pn = point(st + 1);
+      const float* row = phf + pn * ldphi;
+      a = row[ia];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.f;
+      w0 = (float)u0[pn];
+      if (NOUT == 2) w1 = (float)u1[pn];
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      if (t < nI) {
-        const long ab = (kb * ldphi + 16 * (I0 + t) + c) * 32 + 8 * g;
-        const bf16x8 ah = *(const bf16x8*)(phh + ab);
-        const bf16x8 al = *(const bf16x8*)(phl + ab);
-        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b0h, accf[0][t], 0, 0, 0);
-        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b0l, accf[0][t], 0, 0, 0);
-        accf[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b0h, accf[0][t], 0, 0, 0);
-        if (NOUT == 2) {
-          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1h, accf[NOUT - 1][t], 0, 0, 0);
-          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1l, accf[NOUT - 1][t], 0, 0, 0);
-          accf[NOUT - 1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1h, accf[NOUT - 1][t], 0, 0, 0);
-        }
+      if (t < nJ) {
+        accf[0][t] = mfma16x16x4f32(ca, cw0 * cb[t], accf[0][t]);
+        if (NOUT == 2) accf[NOUT - 1][t] = mfma16x16x4f32(ca, cw1 * cb[t], accf[NOUT - 1][t]);
       }
     }
-    if (++steps == 32) {
-      steps = 0;
+    if ((st + 1) % kFlushSteps == 0 || st + 1 == nst) {
 #pragma unroll
       for (int o = 0; o < NOUT; ++o)
 #pragma unroll
@@ -1338,21 +1334,15 @@ __global__ __launch_bounds__(256) void k_contract_mx(const __bf16* __restrict__ 
     }
   }
 #pragma unroll
-  for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) accd[o][t][v] += (double)accf[o][t][v];
-#pragma unroll
   for (int o = 0; o < NOUT; ++o) {
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + v) * 64 + lane] = accd[o][t][v];
+      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + g) * 64 + c + 16 * v] = accd[o][t][v];
     __syncthreads();
     double* dst = part + (((long)split * ngroups + grp) * NOUT + o) * 1024;
-    for (int e = threadIdx.x; e < nI * 256; e += 256)
+    for (int e = threadIdx.x; e < nJ * 256; e += 256)
       dst[e] = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
   }
 }
@@ -1408,14 +1398,10 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restric
 
 // One wave of the BF16 16x16x32 MFMA: lane i supplies a[8i..8i+7], b[8i..8i+7]
 // (FP32 values rounded to BF16); d[v*64 + lane] = FP32 result register v.
-__global__ void k_mfma_raw_bf16(const float* a, const float* b, float* d) {
+// One wave: d[v*64 + lane] = FP32 register v of lane after the FP32 16x16x4 MFMA.
+__global__ void k_mfma_raw_f32(const float* a, const float* b, float* d) {
   const int lane = threadIdx.x;
-  bf16x8 av, bv;
-  for (int e = 0; e < 8; ++e) {
-    av[e] = (__bf16)a[8 * lane + e];
-    bv[e] = (__bf16)b[8 * lane + e];
-  }
-  flt4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  flt4 r = mfma16x16x4f32(a[lane], b[lane], flt4{0.f, 0.f, 0.f, 0.f});
   for (int v = 0; v < 4; ++v) d[v * 64 + lane] = r[v];
 }
 
@@ -1497,13 +1483,10 @@ struct dg_handle {
   // contraction block lists (64-point blocks per tile group) and screening masks
   double screen = 0.0;
   unsigned* bmask = nullptr;
-  int *clist = nullptr, *cloff = nullptr, *mlist = nullptr, *mloff = nullptr;
-  long nlisted = 0, nlisted_mx = 0;
+  int *clist = nullptr, *cloff = nullptr;
+  long nlisted = 0;
   // reduced-precision mode
-  __bf16 *phh = nullptr, *phl = nullptr;
-  int* mgroups = nullptr;
-  int ngroups_mx = 0, nsplit_mx = 0;
-  long chunk_mx = 0;
+  float* phf = nullptr;  // FP32 amplitudes
   double* ej_part = nullptr;
   double *cpart = nullptr, *J = nullptr, *XC = nullptr;
   long* vbase = nullptr;  // [natoms] offset of atom k's partial potentials in vpart (minus first point)
@@ -1544,16 +1527,15 @@ struct dg_handle {
                       mol_z, mol_fac, mol_qraw, atom_exc, atom_ej};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, mgroups, ash_start, ash_list,
-                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0, clist, cloff, mlist, mloff};
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, ash_start, ash_list,
+                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0, clist, cloff};
     if (bmask) (void)hipFree(bmask);
     if (vbase) (void)hipFree(vbase);
     if (perm) (void)hipFree(perm);
     if (mol_poff) (void)hipFree(mol_poff);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (phh) (void)hipFree(phh);
-    if (phl) (void)hipFree(phl);
+    if (phf) (void)hipFree(phf);
     if (ej_part) (void)hipFree(ej_part);
     for (int* b : ibufs)
       if (b) (void)hipFree(b);
@@ -1811,19 +1793,16 @@ int stage_lda(dg_handle* h) {
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
-  const int ng = mx ? h->ngroups_mx : h->ngroups, ns = mx ? h->nsplit_mx : h->nsplit;
-  const int* grp = mx ? h->mgroups : h->blocks;
   DG_TRY(run(h, "contract", [&] {
+    dim3 grid(h->ngroups, h->nsplit);
     if (mx) {
-      dim3 grid(h->ngroups_mx, h->nsplit_mx);
       if (nout == 2)
-        hipLaunchKernelGGL(k_contract_mx<2>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u1,
-                           h->mgroups, h->mlist, h->mloff, h->ngroups_mx, h->cpart);
+        hipLaunchKernelGGL(k_contract_f32<2>, grid, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->blocks,
+                           h->clist, h->cloff, h->ngroups, h->cpart);
       else
-        hipLaunchKernelGGL(k_contract_mx<1>, grid, dim3(256), 0, h->stream, h->phh, h->phl, h->ldphi, u0, u0,
-                           h->mgroups, h->mlist, h->mloff, h->ngroups_mx, h->cpart);
+        hipLaunchKernelGGL(k_contract_f32<1>, grid, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, h->blocks,
+                           h->clist, h->cloff, h->ngroups, h->cpart);
     } else {
-      dim3 grid(h->ngroups, h->nsplit);
       if (nout == 2)
         hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
                            h->clist, h->cloff, h->ngroups, h->cpart);
@@ -1833,8 +1812,8 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
     }
   }));
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)ng * nout * 1024, 16)), dim3(256), 0, h->stream,
-                       h->cpart, ns, ng, nout, grp, h->nbf, mx ? 1 : 0, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 16)), dim3(256), 0,
+                       h->stream, h->cpart, h->nsplit, h->ngroups, nout, h->blocks, h->nbf, 0, out0, out1);
   });
 }
 
@@ -2064,21 +2043,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
   }
   // reduced-precision contraction: column groups (J; I0..I0+nI-1) sharing B
-  std::vector<int> mgroups;
-  if (h->precision == DG_PREC_MIXED) {
-    for (int J = 0; J < h->ntb; ++J)
-      for (int I0 = 0; I0 <= J; I0 += 4) {
-        mgroups.push_back(J);
-        mgroups.push_back(I0);
-        mgroups.push_back(std::min(4, J + 1 - I0));
-      }
-    h->ngroups_mx = (int)mgroups.size() / 3;
-    const long target_wg = 1024;
-    long ns = std::max(1L, target_wg / h->ngroups_mx);
-    ns = std::min(std::min(ns, 256L), h->npts_pad / 128);
-    h->chunk_mx = round_up((h->npts_pad + ns - 1) / ns, 128);
-    h->nsplit_mx = (int)((h->npts_pad + h->chunk_mx - 1) / h->chunk_mx);
-  }
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
   {
@@ -2173,12 +2137,10 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, h->items_cap));
   HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
-  DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit * h->ngroups, (size_t)h->nsplit_mx * h->ngroups_mx) * 2 * 1024));
+  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->ngroups * 2 * 1024));
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
-    DG_TRY(dalloc(&h->mgroups, mgroups.size()));
-    HIP_TRY(hipMalloc((void**)&h->phh, (size_t)np * h->ldphi * sizeof(__bf16)));
-    HIP_TRY(hipMalloc((void**)&h->phl, (size_t)np * h->ldphi * sizeof(__bf16)));
+    HIP_TRY(hipMalloc((void**)&h->phf, (size_t)np * h->ldphi * sizeof(float)));
   }
   DG_TRY(dalloc(&h->J, (size_t)h->packed));
   DG_TRY(dalloc(&h->XC, (size_t)h->packed));
@@ -2210,7 +2172,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
   if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
-  if (!mgroups.empty()) HIP_TRY(up(h->mgroups, mgroups.data(), mgroups.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
   HIP_TRY(up(h->vbase, vbase.data(), vbase.size() * 8));
   HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)vtotal * 8, h->stream));
@@ -2246,9 +2207,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
                        h->pr_cn0, h->pr_cn1, h->npts, h->px, h->py, h->pz, h->ldphi, h->phi);
   }));
   if (h->precision == DG_PREC_MIXED) {
-    DG_TRY(run(h, "phi_split", [&] {
-      hipLaunchKernelGGL(k_phi_split, dim3(nblocks(np * h->ldphi, 256)), dim3(256), 0, h->stream, h->phi,
-                         h->ldphi, np, h->phh, h->phl);
+    DG_TRY(run(h, "phi_f32", [&] {
+      hipLaunchKernelGGL(k_phi_f32, dim3(nblocks(np * h->ldphi, 256)), dim3(256), 0, h->stream, h->phi,
+                         np * h->ldphi, h->phf);
     }));
   }
   DG_TRY(sort_items(h));
@@ -2271,14 +2232,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     DG_TRY(dalloc(&h->cloff, off.size()));
     HIP_TRY(hipMemcpy(h->clist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(h->cloff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
-    if (h->precision == DG_PREC_MIXED) {
-      group_block_lists(mk, mgroups, list, off);
-      h->nlisted_mx = (long)list.size();
-      DG_TRY(dalloc(&h->mlist, std::max<size_t>(list.size(), 1)));
-      DG_TRY(dalloc(&h->mloff, off.size()));
-      HIP_TRY(hipMemcpy(h->mlist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(h->mloff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
-    }
   }
   DG_TRY(sync(h));
   guard.release();
@@ -2614,20 +2567,20 @@ int dg_mfma_probe(const double* a, const double* b, double* d) {
   return DG_OK;
 }
 
-int dg_mfma_probe_bf16(const float* a, const float* b, float* d) {
-  if (!a || !b || !d) return fail(DG_ERR_INVALID, "dg_mfma_probe_bf16: null argument");
+int dg_mfma_probe_f32(const float* a, const float* b, float* d) {
+  if (!a || !b || !d) return fail(DG_ERR_INVALID, "dg_mfma_probe_f32: null argument");
   float* buf = nullptr;
-  hipError_t e = hipMalloc((void**)&buf, (512 + 512 + 256) * sizeof(float));
-  if (e != hipSuccess) return fail(DG_ERR_ALLOC, "dg_mfma_probe_bf16: hipMalloc failed");
-  e = hipMemcpy(buf, a, 512 * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(buf + 512, b, 512 * 4, hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc((void**)&buf, (64 + 64 + 256) * sizeof(float));
+  if (e != hipSuccess) return fail(DG_ERR_ALLOC, "dg_mfma_probe_f32: hipMalloc failed");
+  e = hipMemcpy(buf, a, 64 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(buf + 64, b, 64 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mfma_raw_bf16, dim3(1), dim3(64), 0, 0, buf, buf + 512, buf + 1024);
+    hipLaunchKernelGGL(k_mfma_raw_f32, dim3(1), dim3(64), 0, 0, buf, buf + 64, buf + 128);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpy(d, buf + 1024, 256 * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(d, buf + 128, 256 * 4, hipMemcpyDeviceToHost);
   (void)hipFree(buf);
-  if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_mfma_probe_bf16: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return fail(DG_ERR_HIP, std::string("dg_mfma_probe_f32: ") + hipGetErrorString(e));
   return DG_OK;
 }
 

@@ -30,7 +30,7 @@ EXPORTS = (
     "dg_calculate_density", "dg_hartree", "dg_xc", "dg_get_weights", "dg_get_becke_weights",
     "dg_get_positions", "dg_get_densities", "dg_get_amplitudes", "dg_get_hartree_potential",
     "dg_get_hartree_parts", "dg_get_xc_parts", "dg_xc_adjoint", "dg_build_device", "dg_profile_enable",
-    "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_mfma_probe_bf16", "dg_scf_run", "dg_one_electron",
+    "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_mfma_probe_f32", "dg_scf_run", "dg_one_electron",
     "dg_eigh", "dg_batch_create", "dg_batch_destroy", "dg_batch_info", "dg_batch_layout",
     "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
     "dg_batch_autotune", "dg_batch_scf_run",
@@ -109,7 +109,7 @@ def lib():
         L.dg_profile_reset.argtypes = [C.c_void_p]
         L.dg_selftest_mfma.argtypes = [_I]
         L.dg_mfma_probe.argtypes = [_D, _D, _D]
-        L.dg_mfma_probe_bf16.argtypes = [C.POINTER(C.c_float)] * 3
+        L.dg_mfma_probe_f32.argtypes = [C.POINTER(C.c_float)] * 3
         L.dg_scf_run.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(_GridParams),
                                  C.POINTER(_ScfParams), C.POINTER(_ScfResult), _D, _D]
         L.dg_one_electron.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), _D, _D, _D]
@@ -178,13 +178,13 @@ def mfma_probe(a, b):
     return d.reshape(4, 64)
 
 
-def mfma_probe_bf16(a, b):
-    """FP32 register file (4 x 64) of one BF16 16x16x32 MFMA; a, b are (64, 8) lane inputs."""
+def mfma_probe_f32(a, b):
+    """FP32 register file (4 x 64) of one FP32 16x16x4 MFMA with lane inputs a, b (64 each)."""
     F = C.POINTER(C.c_float)
     a = np.ascontiguousarray(a, dtype=np.float32)
     b = np.ascontiguousarray(b, dtype=np.float32)
     d = np.empty(256, dtype=np.float32)
-    _check(lib().dg_mfma_probe_bf16(a.ctypes.data_as(F), b.ctypes.data_as(F), d.ctypes.data_as(F)))
+    _check(lib().dg_mfma_probe_f32(a.ctypes.data_as(F), b.ctypes.data_as(F), d.ctypes.data_as(F)))
     return d.reshape(4, 64)
 
 

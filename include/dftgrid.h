@@ -41,7 +41,7 @@ extern "C" {
 #define DG_ERR_UNSUPPORTED 5
 
 #define DG_PREC_FP64 0  /* every stage in FP64 */
-#define DG_PREC_MIXED 1 /* FP64 density/energies, reduced-precision Fock contractions */
+#define DG_PREC_MIXED 1 /* FP64 density/potentials/energies, FP32-MFMA Fock contractions */
 
 typedef struct dg_handle dg_handle;
 
@@ -156,9 +156,9 @@ int dg_selftest_mfma(int* ok);
 /* Runs the FP64 16x16x4 MFMA once on one wave: lane i supplies a[i] and b[i];
  * d[v*64 + i] receives result register v of lane i (diagnostics). */
 int dg_mfma_probe(const double* a, const double* b, double* d);
-/* The same for the BF16 16x16x32 MFMA: lane i supplies a[8i..8i+7] and
- * b[8i..8i+7] (rounded to BF16); d[v*64 + i] receives FP32 result register v. */
-int dg_mfma_probe_bf16(const float* a, const float* b, float* d);
+/* The same for the FP32 16x16x4 MFMA of the reduced-precision contractions:
+ * lane i supplies a[i] and b[i]; d[v*64 + i] receives result register v. */
+int dg_mfma_probe_f32(const float* a, const float* b, float* d);
 
 /* --- SCF driver (host C++, src/dft.cpp:84-142, 160-227, 300-366, 441-447) --- */
 typedef struct {

@@ -1,11 +1,11 @@
 """Reduced-precision (DG_PREC_MIXED) grid build against the FP64 build.
 
-Mixed mode keeps rho, the functional, the Poisson solve and every energy in
-FP64; the Fock contractions J and XC run as split-BF16 MFMA products with FP32
-accumulation, the l >= 1 part of the cross-atom Hartree sum in FP32.  Matrix
-errors therefore enter the SCF energy at second order only (E_J is evaluated
-pointwise in FP64).  Tolerances: J and XC within 2e-5 of their scale; total
-SCF energies within 1e-6 Ha of the FP64 build.
+Mixed mode keeps rho, the functional, the Poisson solve, the Hartree
+potential and every energy in FP64; the Fock contractions J and XC run on the
+FP32 16x16x4 MFMA (FP32 amplitudes, FP32 sums added into FP64 every 64 points
+per wave).  Matrix errors enter the SCF energy at second order only (E_J is
+evaluated pointwise in FP64).  Tolerances: J and XC within 1e-6 of their
+scale; total SCF energies within 1e-6 Ha of the FP64 build.
 """
 import numpy as np
 import pytest
@@ -51,11 +51,11 @@ def test_mixed_build_matches_fp64(case):
     Jmx, Xmx, smx = _build(gmx, P)
     # identity 2 tr(P J) = sum w V rho / (2c) in FP64
     assert abs(s64[3] - 2 * np.trace(P @ J64)) < 1e-10 * abs(s64[3])
-    assert rel_err(Jmx, J64) < 2e-5
-    assert rel_err(Xmx, X64) < 2e-5
+    assert rel_err(Jmx, J64) < 1e-6
+    assert rel_err(Xmx, X64) < 1e-6
     assert np.abs(Jmx - Jmx.T).max() == 0.0 and np.abs(Xmx - Xmx.T).max() == 0.0
     assert abs(smx[0] - s64[0]) < 1e-12 * abs(s64[0])      # E_xc: same FP64 path
-    assert abs(smx[3] - s64[3]) < 1e-9 * abs(s64[3])       # E_J pointwise: FP64 rho, V to ~1e-9
+    assert abs(smx[3] - s64[3]) < 1e-12 * abs(s64[3])      # E_J pointwise: the same FP64 rho and V
     assert abs(smx[2] - mol.nelec) < 1e-10
 
 
@@ -68,3 +68,19 @@ def test_mixed_scf_energy_within_1e6(make, nrad, nang):
     print(f"E64 {r64['energy']:.10f} Emixed {rmx['energy']:.10f} diff {rmx['energy'] - r64['energy']:.2e}")
     assert abs(rmx["energy"] - r64["energy"]) < 1e-6
     assert np.abs(Pmx - P64).max() < 1e-4
+
+
+def test_fp32_mfma_layout():
+    # FP32 16x16x4 MFMA: A[i][k] on lane i + 16k, B[k][j] on lane j + 16k,
+    # D[4g + v][c] in register v of lane c + 16g (the layout k_contract_f32 assumes)
+    rng = np.random.default_rng(4)
+    a = rng.integers(-8, 8, 64).astype(np.float32)
+    b = rng.integers(-8, 8, 64).astype(np.float32)
+    A = np.array([[a[i + 16 * k] for k in range(4)] for i in range(16)], dtype=np.float64)
+    B = np.array([[b[j + 16 * k] for j in range(16)] for k in range(4)], dtype=np.float64)
+    D = A @ B
+    d = G.mfma_probe_f32(a, b)
+    for lane in range(64):
+        c, g = lane % 16, lane // 16
+        for v in range(4):
+            assert d[v, lane] == D[4 * g + v, c], (lane, v)
