@@ -1087,6 +1087,31 @@ __device__ __forceinline__ double becke_s(double mu) {
 // 10.5 (kBeckeSkip Z)^{7/8} ~ 3e-17 Z^{7/8}.
 constexpr double kBeckeSkip = 1e-20;
 
+// At most kBeckeActive cell functions per point may lie above the skip level
+// (about 15 on a linear alkane); a point with more sets *overflow and the
+// caller reports the gradient as unsupported.
+constexpr int kBeckeActive = 24;
+
+__device__ __forceinline__ double atom_dist(const double* __restrict__ atoms, int m, double x, double y, double z) {
+  const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
+  return fmax(sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
+}
+
+// Cell function P_n at the point, its factors taken with the nearest atom
+// first and stopped once the product is at most cut (then it is below the
+// level that matters and is returned as is).
+__device__ __forceinline__ double cell_function(const double* __restrict__ atoms,
+                                               const double* __restrict__ invR, int natoms, int n, int near,
+                                               double dn, double dnear, double x, double y, double z,
+                                               double cut) {
+  double prod = (n == near) ? 1.0 : becke_s((dn - dnear) * invR[n * natoms + near]);
+  for (int m = 0; m < natoms && prod > cut; ++m) {
+    if (m == n || m == near) continue;
+    prod *= becke_s((dn - atom_dist(atoms, m, x, y, z)) * invR[n * natoms + m]);
+  }
+  return prod;
+}
+
 __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ atoms,
                                                     const double* __restrict__ invR, int natoms,
                                                     long npts, long cell, int nrad, int nang,
@@ -1097,70 +1122,83 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
                                                     const double* __restrict__ py,
                                                     const double* __restrict__ pz,
                                                     const double* __restrict__ ab, long nblk,
-                                                    double* part, double* mot) {
-  extern __shared__ double sh[];
-  double* dist = sh;                          // natoms x 64
-  double* Pn = sh + natoms * 64;              // natoms x 64
-  int* act = (int*)(sh + 2 * natoms * 64);    // natoms x 64: the point's atoms above the skip level
+                                                    double* part, double* mot, int* overflow) {
+  // the point's cell functions above the skip level: atom, P_n, |r - R_n|
+  __shared__ int act_n[kBeckeActive * 64];
+  __shared__ double act_P[kBeckeActive * 64];
+  __shared__ double act_d[kBeckeActive * 64];
   const int lane = threadIdx.x;
   const long p = (long)blockIdx.x * 64 + lane;
-  const bool valid = p < npts;
-  const double x = valid ? px[p] : 0.0, y = valid ? py[p] : 0.0, z = valid ? pz[p] : 0.0;
-  for (int m = 0; m < natoms; ++m) {
-    const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
-    dist[m * 64 + lane] = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
-  }
-  // Cell functions, the nearest atom first: its P is a lower bound of Z, and
-  // the product of another atom stops once it is below 1e-4 kBeckeSkip of
-  // that bound (its factor for the nearest atom comes first, so a far atom
-  // stops at once); such an atom is below the skip level and its remaining
-  // product (at most the truncated one) changes Z by less than 1e-24 Z per atom.
-  int near = 0;
-  for (int m = 1; m < natoms; ++m)
-    if (dist[m * 64 + lane] < dist[near * 64 + lane]) near = m;
-  double Z = 0.0, cut = 0.0;
-  for (int r = 0; r < natoms; ++r) {
-    const int n = (r == 0) ? near : (r <= near ? r - 1 : r);
-    const double dn = dist[n * 64 + lane];
-    double prod = 1.0;
-    if (n != near) prod = becke_s((dn - dist[near * 64 + lane]) * invR[n * natoms + near]);
-    for (int m = 0; m < natoms && prod >= cut; ++m) {
-      if (m == n || m == near) continue;
-      prod *= becke_s((dn - dist[m * 64 + lane]) * invR[n * natoms + m]);
-    }
-    Pn[n * 64 + lane] = prod;
-    Z += prod;
-    if (r == 0) cut = 1e-4 * kBeckeSkip * prod;
-  }
-  const double skip = kBeckeSkip * Z;
-  int nact = 0;
-  for (int n = 0; n < natoms; ++n)
-    if (Pn[n * 64 + lane] > skip) act[(nact++) * 64 + lane] = n;
-  const int own = valid ? (int)(p / cell) : 0;
-  double pref = 0.0;  // w0 a_p / Z
-  if (valid && Z > 0.0) {
+  double w0a = 0.0;  // weight without the Becke factor, times a_p
+  if (p < npts) {
     const int j = (int)(p % nang);
     const int i = (int)((p / nang) % nrad);
     const double ox = leb[4 * j] * rad[i], oy = leb[4 * j + 1] * rad[i], oz = leb[4 * j + 2] * rad[i];
-    const double w0 = wrad[i] * leb[4 * j + 3] * (ox * ox + oy * oy + oz * oz) * 4.0 * kPiD;
-    pref = w0 * ab[p] / Z;
+    w0a = wrad[i] * leb[4 * j + 3] * (ox * ox + oy * oy + oz * oz) * 4.0 * kPiD * ab[p];
   }
-  const double wbv = (valid && Z > 0.0) ? Pn[own * 64 + lane] / Z : 0.0;
+  // points with |w0 a_p| <= 1e-30 (far out, where the density has vanished and
+  // many cells overlap) add nothing that survives rounding and are skipped
+  const bool valid = p < npts && fabs(w0a) > 1e-30;
+  const double x = valid ? px[p] : 0.0, y = valid ? py[p] : 0.0, z = valid ? pz[p] : 0.0;
+  int near = 0;
+  double dnear = atom_dist(atoms, 0, x, y, z);
+  for (int m = 1; m < natoms; ++m) {
+    const double d = atom_dist(atoms, m, x, y, z);
+    if (d < dnear) {
+      dnear = d;
+      near = m;
+    }
+  }
+  // Z: the nearest atom's P is a lower bound, and a product that drops below
+  // 1e-4 kBeckeSkip of it (below the skip level) is kept as truncated, which
+  // changes Z by less than 1e-24 Z per atom
+  double pnear = 1.0, Z = 1.0;
+  if (valid) {
+    pnear = cell_function(atoms, invR, natoms, near, near, dnear, dnear, x, y, z, 0.0);
+    const double cut0 = 1e-4 * kBeckeSkip * pnear;
+    Z = pnear;
+    for (int n = 0; n < natoms; ++n)
+      if (n != near)
+        Z += cell_function(atoms, invR, natoms, n, near, atom_dist(atoms, n, x, y, z), dnear, x, y, z, cut0);
+  }
+  const double skip = kBeckeSkip * Z;
+  int nact = 0;
+  double Pown = 0.0;
+  const int own = valid ? (int)(p / cell) : 0;
+  for (int n = 0; n < natoms && valid; ++n) {
+    const double dn = (n == near) ? dnear : atom_dist(atoms, n, x, y, z);
+    const double P = (n == near) ? pnear : cell_function(atoms, invR, natoms, n, near, dn, dnear, x, y, z, skip);
+    if (n == own) Pown = P;
+    if (P > skip) {
+      if (nact < kBeckeActive) {
+        act_n[nact * 64 + lane] = n;
+        act_P[nact * 64 + lane] = P;
+        act_d[nact * 64 + lane] = dn;
+        ++nact;
+      } else {
+        *overflow = 1;
+      }
+    }
+  }
+  const double pref = (valid && Z > 0.0) ? w0a / Z : 0.0;
+  const double wbv = (valid && Z > 0.0) ? Pown / Z : 0.0;
   double own_acc[3] = {0.0, 0.0, 0.0};
   for (int B = 0; B < natoms; ++B) {
     double gB[3] = {0.0, 0.0, 0.0};
     if (valid && B != own && pref != 0.0) {
-      const double dB = dist[B * 64 + lane];
+      const double dB = atom_dist(atoms, B, x, y, z);
       const double uB[3] = {(x - atoms[4 * B]) / dB, (y - atoms[4 * B + 1]) / dB, (z - atoms[4 * B + 2]) / dB};
       double sumdP[3] = {0.0, 0.0, 0.0}, dPa[3] = {0.0, 0.0, 0.0};
-      const double PB = Pn[B * 64 + lane];
+      double PB = 0.0;
+      for (int t = 0; t < nact; ++t)
+        if (act_n[t * 64 + lane] == B) PB = act_P[t * 64 + lane];
       if (PB > skip) {
         // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) (-(u_B + mu_Bm e_Bm)/R_Bm)
         double acc[3] = {0.0, 0.0, 0.0};
         for (int m = 0; m < natoms; ++m) {
           if (m == B) continue;
           const double iR = invR[B * natoms + m];
-          const double mu = (dB - dist[m * 64 + lane]) * iR;
+          const double mu = (dB - atom_dist(atoms, m, x, y, z)) * iR;
           double sv, rt;
           cutoff_and_ratio(mu, sv, rt);
           const double e[3] = {(atoms[4 * B] - atoms[4 * m]) * iR, (atoms[4 * B + 1] - atoms[4 * m + 1]) * iR,
@@ -1171,15 +1209,15 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
       }
       // n != B: dP_n/dR_B = P_n ratio(mu_nB) (u_B + mu_nB e_nB)/R_nB
       for (int t = 0; t < nact; ++t) {
-        const int n = act[t * 64 + lane];
+        const int n = act_n[t * 64 + lane];
         if (n == B) continue;
         const double iR = invR[n * natoms + B];
-        const double mu = (dist[n * 64 + lane] - dB) * iR;
+        const double mu = (act_d[t * 64 + lane] - dB) * iR;
         double sv, rt;
         cutoff_and_ratio(mu, sv, rt);
         const double e[3] = {(atoms[4 * n] - atoms[4 * B]) * iR, (atoms[4 * n + 1] - atoms[4 * B + 1]) * iR,
                              (atoms[4 * n + 2] - atoms[4 * B + 2]) * iR};
-        const double f = Pn[n * 64 + lane] * rt * iR;
+        const double f = act_P[t * 64 + lane] * rt * iR;
         for (int d = 0; d < 3; ++d) {
           const double tv = f * (uB[d] + mu * e[d]);
           sumdP[d] += tv;
@@ -2425,14 +2463,11 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
     return fail(DG_ERR_STATE, "dg_xc_adjoint: needs a corrected density and dg_xc (or a build) first");
   const long np = h->npts_pad, nblk = np / 64;
   const int na = h->natoms;
-  const size_t becke_lds = (size_t)na * 64 * (2 * sizeof(double) + sizeof(int));
-  if (grad && becke_lds > 150 * 1024)
-    return fail(DG_ERR_UNSUPPORTED, "dg_xc_adjoint: too many atoms for the Becke-gradient kernel's LDS");
   if (!h->adj_ub) {
     for (double** b : {&h->adj_ub, &h->adj_ab, &h->adj_udp}) DG_TRY(dalloc(b, np));
     DG_TRY(dalloc(&h->adj_lam, np / 256 + 1));
     DG_TRY(dalloc(&h->adj_dP, (size_t)h->nbf * h->nbf));
-    DG_TRY(dalloc(&h->adj_out, (size_t)6 * na));
+    DG_TRY(dalloc(&h->adj_out, (size_t)6 * na + 1));
   }
   if (grad && !h->adj_T) {
     DG_TRY(dalloc(&h->adj_T, (size_t)np * h->ldphi));
@@ -2453,9 +2488,6 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
     DG_TRY(dalloc(&h->ash_list, std::max<size_t>(list.size(), 1)));
     HIP_TRY(hipMemcpyAsync(h->ash_start, start.data(), (na + 1) * 4, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->ash_list, list.data(), list.size() * 4, hipMemcpyHostToDevice, h->stream));
-    if (becke_lds > 64 * 1024)
-      HIP_TRY(hipFuncSetAttribute((const void*)k_force_becke, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)becke_lds));
     DG_TRY(sync(h));
   }
   // Lambda = sum w v rho (uxc = w v), then the per-point factors
@@ -2485,18 +2517,24 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
                          h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->pr_cn0, h->pr_cn1, h->npts,
                          h->px, h->py, h->pz, h->ldphi, h->adj_T, h->adj_ub, nblk, h->adj_part, h->adj_mot);
     }));
+    HIP_TRY(hipMemsetAsync(h->adj_out + 6 * na, 0, sizeof(double), h->stream));
     DG_TRY(run(h, "adj_force_becke", [&] {
-      hipLaunchKernelGGL(k_force_becke, dim3(nblk), dim3(64), becke_lds, h->stream, h->atoms, h->invR, na, h->npts,
+      hipLaunchKernelGGL(k_force_becke, dim3(nblk), dim3(64), 0, h->stream, h->atoms, h->invR, na, h->npts,
                          h->cell, h->nrad, h->nang, h->rad, h->wrad, h->leb, h->px, h->py, h->pz, h->adj_ab, nblk,
-                         h->adj_part, h->adj_mot);
+                         h->adj_part, h->adj_mot, (int*)(h->adj_out + 6 * na));
     }));
     DG_TRY(run(h, "adj_force_sum", [&] {
       hipLaunchKernelGGL(k_row_sum, dim3(3 * na), dim3(1024), 0, h->stream, h->adj_part, nblk, h->adj_out);
       hipLaunchKernelGGL(k_row_sum, dim3(3 * na), dim3(1024), 0, h->stream, h->adj_mot, h->cell,
                          h->adj_out + 3 * na);
     }));
-    std::vector<double> o((size_t)6 * na);
+    std::vector<double> o((size_t)6 * na + 1);
     DG_TRY(copy_out(h, o.data(), h->adj_out, o.size()));
+    int over = 0;
+    std::memcpy(&over, &o[6 * na], sizeof(int));
+    if (over)
+      return fail(DG_ERR_UNSUPPORTED, "dg_xc_adjoint: a grid point has more than " + std::to_string(kBeckeActive) +
+                                          " significant Becke cell functions");
     for (int a = 0; a < na; ++a)
       for (int d = 0; d < 3; ++d) grad[3 * a + d] = o[3 * a + d] + o[3 * na + d * na + a];
   }
