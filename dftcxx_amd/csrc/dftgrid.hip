@@ -272,6 +272,59 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
   if (g == 0) rho[p0 + c] = 2.0 * acc;
 }
 
+// k_density for small bases (NT <= 7): the whole fragment set (128 NT(NT+1)
+// doubles) is staged in LDS once per work-group, which then walks the 64-point
+// blocks b = blockIdx.x, blockIdx.x + G, ... with no further barriers; each
+// wave loads the next block's amplitude row while it multiplies the current.
+// The products and their order are those of k_density.
+template <int NT>
+__global__ __launch_bounds__(256) void k_density_pl(const double* __restrict__ phi, int ldphi,
+                                                    const double* __restrict__ frag, long nblk, double* rho,
+                                                    const unsigned* __restrict__ mask) {
+  __shared__ double pl[128 * NT * (NT + 1)];
+  for (int e = threadIdx.x; e < 128 * NT * (NT + 1); e += 256) pl[e] = frag[e];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  double f[4 * NT];
+  long blk = blockIdx.x;
+  if (blk < nblk) {
+    const double* row = phi + (blk * 64 + wave * 16 + c) * ldphi;
+#pragma unroll
+    for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  }
+  for (; blk < nblk; blk += gridDim.x) {
+    const long p0 = blk * 64 + wave * 16;
+    const unsigned mk = mask ? mask[blk] : 0xffffffffu;
+    double cf[4 * NT];
+#pragma unroll
+    for (int s = 0; s < 4 * NT; ++s) cf[s] = ((mk >> (s >> 2)) & 1u) ? f[s] : 0.0;
+    const long nb = blk + gridDim.x;
+    if (nb < nblk) {
+      const double* row = phi + (nb * 64 + wave * 16 + c) * ldphi;
+#pragma unroll
+      for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+    }
+    double acc = 0.0;
+    const double* row = phi + (p0 + c) * ldphi;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      if (!((mk >> b) & 1u)) continue;
+      const double* src = pl + 128 * b * (b + 1);
+      dbl4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4 * (b + 1); ++s)
+        if ((mk >> (s >> 2)) & 1u) d = mfma16x16x4(src[s * 64 + lane], cf[s], d);
+      // register v of lane (c, g) holds T_b[p0+c][16b + 4v + g]
+      const double* rb = row + 16 * b + g;
+      acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
+    }
+    acc += __shfl_xor(acc, 16);
+    acc += __shfl_xor(acc, 32);
+    if (g == 0) rho[p0 + c] = 2.0 * acc;
+  }
+}
+
 // out[a] = sum over the points of cell a of x*y, fixed order (deterministic).
 __global__ __launch_bounds__(1024) void k_cell_sum(const double* __restrict__ x,
                                                    const double* __restrict__ y, long cell,
@@ -916,6 +969,91 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi
   }
 }
 
+// Contraction for small bases (NT = ceil(nbf/16) <= 5 tiles, dense): a wave
+// accumulates all NT(NT+1)/2 upper tiles of one output, so each amplitude is
+// read from HBM once per build and serves as both operands.  Work-group s
+// takes the 64-point blocks [nblk s/G, nblk (s+1)/G); its 4 waves are split
+// over the outputs (4/NOUT per output) and interleave the 16 four-point steps
+// of each block (next step's operands prefetched).  The waves of an output
+// are summed in LDS in wave order and written in the k_contract_reduce layout
+// (group of row I and columns J0 = I + 4k.., tile t = (J - I) % 4, register v
+// holding row 4v + g).
+template <int NT, int NOUT>
+__global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restrict__ phi, int ldphi,
+                                                        const double* __restrict__ u0,
+                                                        const double* __restrict__ u1, long nblk,
+                                                        int ngroups, double* part) {
+  constexpr int NP = NT * (NT + 1) / 2;
+  constexpr int NW = 4 / NOUT;  // waves per output
+  __shared__ double red[NP * 256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int o = wave / NW, sw = wave % NW;
+  const double* u = (o == 0) ? u0 : u1;
+  const long b0 = nblk * blockIdx.x / gridDim.x, b1 = nblk * (blockIdx.x + 1) / gridDim.x;
+  constexpr int SPB = 16 / NW;  // this wave's steps per block
+  const int nst = SPB * (int)(b1 - b0);
+  auto point = [&](int t) { return 64L * (b0 + t / SPB) + 4 * (sw + NW * (t % SPB)) + g; };
+  dbl4 acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+  double f[NT], w = 0.0;
+  if (nst > 0) {
+    const long p = point(0);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) f[T] = phi[p * ldphi + 16 * T + c];
+    w = u[p];
+  }
+  for (int st = 0; st < nst; ++st) {
+    double cf[NT];
+#pragma unroll
+    for (int T = 0; T < NT; ++T) cf[T] = f[T];
+    const double cw = w;
+    if (st + 1 < nst) {
+      const long p = point(st + 1);
+#pragma unroll
+      for (int T = 0; T < NT; ++T) f[T] = phi[p * ldphi + 16 * T + c];
+      w = u[p];
+    }
+    double bw[NT];
+#pragma unroll
+    for (int T = 0; T < NT; ++T) bw[T] = cw * cf[T];
+    int q = 0;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = I; J < NT; ++J, ++q) acc[q] = mfma16x16x4(cf[I], bw[J], acc[q]);
+  }
+  for (int oo = 0; oo < NOUT; ++oo) {
+    for (int s2 = 0; s2 < NW; ++s2) {
+      __syncthreads();
+      if (o == oo && sw == s2) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double* r = red + q * 256 + v * 64 + lane;
+            *r = (s2 == 0) ? acc[q][v] : *r + acc[q][v];
+          }
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NP * 256; e += 256) {
+      const int q = e >> 8, rem = e & 255;
+      int I = 0, qq = q;
+      while (qq >= NT - I) {
+        qq -= NT - I;
+        ++I;
+      }
+      const int J = I + qq;
+      int grp = 0;
+      for (int I2 = 0; I2 < I; ++I2) grp += (NT - I2 + 3) / 4;
+      grp += (J - I) / 4;
+      part[(((long)blockIdx.x * ngroups + grp) * NOUT + oo) * 1024 + ((J - I) & 3) * 256 + rem] = red[e];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint of E_xc (no counterpart in the reference)
 // ---------------------------------------------------------------------------
@@ -1397,11 +1535,12 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restric
   __shared__ double red[16][17];
   const int el = threadIdx.x & 15, r = threadIdx.x >> 4;
   const long e = (long)blockIdx.x * 16 + el;
-  const bool live = e < (long)ngroups * nout * 1024;
+  bool live = e < (long)ngroups * nout * 1024;
   double s = 0.0;
   const int idx = (int)(e & 1023);
   const int o = live ? (int)((e >> 10) % nout) : 0;
   const int grp = live ? (int)((e >> 10) / nout) : 0;
+  if (live && (idx >> 8) >= groups[3 * grp + 2]) live = false;  // tile beyond the group: not read
   if (live) {
     const long stride = (long)ngroups * nout * 1024;
     const double* src = part + ((long)grp * nout + o) * 1024 + idx;
@@ -1495,7 +1634,7 @@ struct dg_handle {
   bool own_stream = true;
   int natoms = 0, nbf = 0, nshell = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0, precision = 0;
   long npts = 0, npts_pad = 0, cell = 0;
-  int ldphi = 0, ntb = 0, ngroups = 0, nsplit = 0;
+  int ldphi = 0, ntb = 0, ngroups = 0, nsplit = 0, nsplit_small = 0;
   long chunk_k = 0, nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
   VwnConst vwn{};
@@ -1659,7 +1798,28 @@ void launch_density_nt(dg_handle* h) {
                      h->ldphi, h->pfrag, h->rho, h->bmask);
 }
 
+template <int NT>
+void launch_density_pl_nt(dg_handle* h) {
+  const long nblk = h->npts_pad / 64;
+  const unsigned G = (unsigned)std::min<long>(nblk, 1024);
+  hipLaunchKernelGGL(k_density_pl<NT>, dim3(G), dim3(256), 0, h->stream, h->phi, h->ldphi, h->pfrag, nblk, h->rho,
+                     h->bmask);
+}
+
 int launch_density(dg_handle* h) {
+  if (h->ntb <= 7)  // fragments within 64 KB of LDS
+    return run(h, "density", [&] {
+      switch (h->ntb) {
+        case 1: launch_density_pl_nt<1>(h); break;
+        case 2: launch_density_pl_nt<2>(h); break;
+        case 3: launch_density_pl_nt<3>(h); break;
+        case 4: launch_density_pl_nt<4>(h); break;
+        case 5: launch_density_pl_nt<5>(h); break;
+        case 6: launch_density_pl_nt<6>(h); break;
+        case 7: launch_density_pl_nt<7>(h); break;
+        default: break;
+      }
+    });
   return run(h, "density", [&] {
     switch (h->ntb) {
 #define DG_NT(N) \
@@ -1828,12 +1988,36 @@ int stage_lda(dg_handle* h) {
   });
 }
 
+template <int N>
+void launch_contract_small_nt(dg_handle* h, const double* u0, const double* u1, int nout) {
+  const long nblk = h->npts_pad / 64;
+  if (nout == 2)
+    hipLaunchKernelGGL((k_contract_small<N, 2>), dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi,
+                       u0, u1, nblk, h->ngroups, h->cpart);
+  else
+    hipLaunchKernelGGL((k_contract_small<N, 1>), dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi,
+                       u0, u0, nblk, h->ngroups, h->cpart);
+}
+
+void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int nout) {
+  switch (h->ntb) {
+    case 1: launch_contract_small_nt<1>(h, u0, u1, nout); break;
+    case 2: launch_contract_small_nt<2>(h, u0, u1, nout); break;
+    case 3: launch_contract_small_nt<3>(h, u0, u1, nout); break;
+    case 4: launch_contract_small_nt<4>(h, u0, u1, nout); break;
+    case 5: launch_contract_small_nt<5>(h, u0, u1, nout); break;
+    default: break;
+  }
+}
+
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
+  const bool small = !mx && h->screen == 0.0 && h->ntb <= 5;
   DG_TRY(run(h, "contract", [&] {
     dim3 grid(h->ngroups, h->nsplit);
-    if (mx) {
+    if (small) {
+      launch_contract_small(h, u0, u1, nout);    } else if (mx) {
       if (nout == 2)
         hipLaunchKernelGGL(k_contract_f32<2>, grid, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->blocks,
                            h->clist, h->cloff, h->ngroups, h->cpart);
@@ -1851,7 +2035,8 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
   }));
   return run(h, "contract_reduce", [&] {
     hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 16)), dim3(256), 0,
-                       h->stream, h->cpart, h->nsplit, h->ngroups, nout, h->blocks, h->nbf, 0, out0, out1);
+                       h->stream, h->cpart, small ? h->nsplit_small : h->nsplit, h->ngroups, nout, h->blocks, h->nbf,
+                       0, out0, out1);
   });
 }
 
@@ -2079,6 +2264,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     ns = std::min(std::min(ns, 256L), h->npts_pad / 64);
     h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
     h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
+    h->nsplit_small = (int)std::max(1L, std::min(256L, h->npts_pad / 256));
   }
   // reduced-precision contraction: column groups (J; I0..I0+nI-1) sharing B
   h->nlda_blocks = h->npts_pad / 256;
@@ -2175,7 +2361,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, h->items_cap));
   HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
-  DG_TRY(dalloc(&h->cpart, (size_t)h->nsplit * h->ngroups * 2 * 1024));
+  DG_TRY(dalloc(&h->cpart, (size_t)std::max(h->nsplit, h->nsplit_small) * h->ngroups * 2 * 1024));
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
     HIP_TRY(hipMalloc((void**)&h->phf, (size_t)np * h->ldphi * sizeof(float)));
