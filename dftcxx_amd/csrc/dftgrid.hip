@@ -55,6 +55,12 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Wave-level sum in a fixed butterfly order (all 64 lanes end with the total).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 constexpr int kMaxL = 12;  // largest lmax compiled
 constexpr double kPiD = 3.14159265358979323846;
 
@@ -273,16 +279,16 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
 }
 
 // k_density for small bases (NT <= 7): the whole fragment set (128 NT(NT+1)
-// doubles) is staged in LDS once per work-group, which then walks the 64-point
+// doubles, formed from P as k_pfrag does) is staged in LDS once per work-group, which then walks the 64-point
 // blocks b = blockIdx.x, blockIdx.x + G, ... with no further barriers; each
 // wave loads the next block's amplitude row while it multiplies the current.
 // The products and their order are those of k_density.
 template <int NT>
 __global__ __launch_bounds__(256) void k_density_pl(const double* __restrict__ phi, int ldphi,
-                                                    const double* __restrict__ frag, long nblk, double* rho,
+                                                    const double* __restrict__ P, int nbf, long nblk, double* rho,
                                                     const unsigned* __restrict__ mask) {
   __shared__ double pl[128 * NT * (NT + 1)];
-  for (int e = threadIdx.x; e < 128 * NT * (NT + 1); e += 256) pl[e] = frag[e];
+  for (int e = threadIdx.x; e < 128 * NT * (NT + 1); e += 256) pl[e] = pfrag_value(P, nbf, e);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
@@ -340,6 +346,54 @@ __global__ __launch_bounds__(1024) void k_cell_sum(const double* __restrict__ x,
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// The same sums split S ways per cell (block b: cell b / S, part b % S), for
+// molecules with few atoms, whose cells would otherwise occupy few CUs.
+__global__ __launch_bounds__(256) void k_cell_sum_split(const double* __restrict__ x,
+                                                        const double* __restrict__ y, long cell, int S,
+                                                        double* part) {
+  __shared__ double red[256];
+  const int a = blockIdx.x / S, sp = blockIdx.x % S;
+  const long base = (long)a * cell, lo = cell * sp / S, hi = cell * (sp + 1) / S;
+  double s = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += 256) s += x[base + i] * y[base + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// k_scale from the split populations: q_a = sum_s part[a S + s], in order;
+// the total is summed by the first wave of every block in one fixed order
+// (lane strides, then a butterfly), so every block gets the same factor.
+__global__ __launch_bounds__(256) void k_scale_parts(double* rho, long n, const double* __restrict__ part, int S,
+                                                     int natoms, double zsum, double* q, double* qc,
+                                                     double* scalars) {
+  __shared__ double qsum;
+  if (threadIdx.x < 64) {
+    double t = 0.0;
+    for (int e = threadIdx.x; e < natoms * S; e += 64) t += part[e];
+    t = wave_sum(t);
+    if (threadIdx.x == 0) qsum = t;
+  }
+  __syncthreads();
+  const double qs = qsum;
+  const double f = zsum / qs;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) rho[p] *= f;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0 && scalars) scalars[1] = qs;
+    for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
+      double qa = 0.0;
+      for (int sp = 0; sp < S; ++sp) qa += part[a * S + sp];
+      q[a] = qa;
+      qc[a] = f * qa;
+    }
+  }
 }
 
 // rho *= sum(Z) / sum_a q_a  (src/moleculargrid.cpp:132-146); scalars[1] = sum q
@@ -455,10 +509,11 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
 // ---------------------------------------------------------------------------
 
 // rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p.  One block per
-// (a, 4 consecutive radial shells): the shells' rho w_j wb_p go to LDS, wave w
-// sums the points j = w (mod 4) for lm = lane, lane+64, lane+128 of all four
-// shells (each Y row is read once for 4 shells), the four wave sums are added in order.
-constexpr int kShells = 4;
+// (a, kShells consecutive radial shells): the shells' rho w_j wb_p go to LDS,
+// wave w sums the points j = w (mod 4) for lm = lane, lane+64, lane+128 of all
+// the block's shells (each Y row is read once for them), the four wave sums are
+// added in order.
+constexpr int kShells = 2;
 __global__ __launch_bounds__(256) void k_rho_lm(const double* __restrict__ rho,
                                                 const double* __restrict__ wb,
                                                 const double* __restrict__ leb,
@@ -509,11 +564,12 @@ __global__ __launch_bounds__(256) void k_rho_lm(const double* __restrict__ rho,
 // g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm], g_{N+1} = 0, KU_l = rows 1..N of the
 // inverse FD matrix; the not-a-knot slopes D of U over the ascending knots are
 // a fixed linear map of U, so D_t = sum_k KD_l[k][t] g_k with KD_l = S KU_l.
-// tab[a][t][lm] = {U at knot t, D_t}; knot t is radial node N-1-t.
+// tab[a][t][lm] = N_lm {U at knot t, D_t}; knot t is radial node N-1-t, and
+// N_lm (the harmonic's normalisation, shc) is folded in for the cross kernel.
 __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
                           const double* __restrict__ rad, const double* __restrict__ KU,
-                          const double* __restrict__ KD, int nrad, int nlm, double* U,
-                          double* tab) {
+                          const double* __restrict__ KD, const double* __restrict__ shc, int nrad, int nlm,
+                          double* U, double* tab) {
   extern __shared__ double gv[];
   const int a = blockIdx.x / nlm, lm = blockIdx.x % nlm;
   int l = 0;
@@ -539,15 +595,18 @@ __global__ void k_poisson(const double* __restrict__ rho_lm, const double* __res
     d += Md[(long)k * nrad + i] * gk;
   }
   U[((long)a * nrad + i) * nlm + lm] = u;
+  const int m = lm - l * l - l;
+  const double nl = shc[3 * (l * (l + 1) / 2 + (m < 0 ? -m : m))];
   double* T = tab + (long)a * nrad * nlm * 2 + lm * 2;
-  T[(long)(nrad - 1 - i) * nlm * 2] = u;
-  T[(long)i * nlm * 2 + 1] = d;
+  T[(long)(nrad - 1 - i) * nlm * 2] = nl * u;
+  T[(long)i * nlm * 2 + 1] = nl * d;
 }
 
 // Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
 // (src/atomicgrid.cpp:434-462).  One block per (a, i) with U[a][i][:] in LDS;
 // thread (j, h) sums half h of the lm range (ylmT is [lm][j], coalesced over j)
-// and the two halves are added in order.
+// and the two halves are added in order.  The block has 2 jw threads, jw the
+// angular count rounded up to 64 (at most 512).
 __global__ __launch_bounds__(1024) void k_vcell(const double* __restrict__ U,
                                                 const double* __restrict__ ylmT,
                                                 const double* __restrict__ atoms,
@@ -562,10 +621,11 @@ __global__ __launch_bounds__(1024) void k_vcell(const double* __restrict__ U,
   const int a = (int)(ai / nrad);
   for (int lm = threadIdx.x; lm < nlm; lm += blockDim.x) u[lm] = U[ai * nlm + lm];
   __syncthreads();
-  const int hsel = threadIdx.x >> 9, jt = threadIdx.x & 511;
+  const int jw = blockDim.x >> 1;  // angular points per pass (a multiple of 64)
+  const int hsel = threadIdx.x >= jw, jt = threadIdx.x - (hsel ? jw : 0);
   const int lmid = nlm / 2;
   const int lb = hsel ? lmid : 0, le = hsel ? nlm : lmid;
-  for (int j0 = 0; j0 < nang; j0 += 512) {
+  for (int j0 = 0; j0 < nang; j0 += jw) {
     const int j = j0 + jt;
     double acc = 0.0;
     if (j < nang)
@@ -773,7 +833,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
               pl = A.shc[ci + 1] * ct[q] * p1[q] - A.shc[ci + 2] * p2[q];
             p2[q] = p1[q];
             p1[q] = pl;
-            const double base = A.shc[ci] * pl;
+            const double base = pl;  // N_lm is folded into the table
             yp[q] = (m == 0) ? base : base * cm[q];
             yn[q] = base * sm[q];
           }
@@ -1130,11 +1190,6 @@ __device__ __forceinline__ void shell_grad(int L, int comp, double r0, double d0
   gr[2] = px * py * dpz * R + z * poly * Rp;
 }
 
-// Wave-level sum in a fixed butterfly order (all 64 lanes end with the total).
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 
 // Basis-centre and grid-motion terms of dE_xc/dR, one wave per 64 points.
 // For point p and atom A: g_A(p) = ub_p sum_{i on A} T_pi grad phi_i(r_p).
@@ -1649,6 +1704,8 @@ struct dg_handle {
   double *P = nullptr, *pfrag = nullptr, *rho = nullptr, *uxc = nullptr, *exd = nullptr;
   double *uJ = nullptr, *V = nullptr, *vcell = nullptr, *vpart = nullptr;
   double *q = nullptr, *qc = nullptr, *exc_part = nullptr, *scalars = nullptr;
+  double* qpart = nullptr;  // split cell populations (cell_split > 1)
+  int cell_split = 1;
   double *ylm = nullptr, *ylmT = nullptr, *minvT = nullptr, *shc = nullptr;
   double *knots = nullptr, *invh = nullptr, *KD = nullptr;
   double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
@@ -1701,7 +1758,7 @@ struct dg_handle {
                       V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, ylmT, minvT, shc,
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC,
                       adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP,
-                      mol_z, mol_fac, mol_qraw, atom_exc, atom_ej};
+                      mol_z, mol_fac, mol_qraw, atom_exc, atom_ej, qpart};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, ash_start, ash_list,
@@ -1799,24 +1856,24 @@ void launch_density_nt(dg_handle* h) {
 }
 
 template <int NT>
-void launch_density_pl_nt(dg_handle* h) {
+void launch_density_pl_nt(dg_handle* h, const double* dP) {
   const long nblk = h->npts_pad / 64;
   const unsigned G = (unsigned)std::min<long>(nblk, 1024);
-  hipLaunchKernelGGL(k_density_pl<NT>, dim3(G), dim3(256), 0, h->stream, h->phi, h->ldphi, h->pfrag, nblk, h->rho,
+  hipLaunchKernelGGL(k_density_pl<NT>, dim3(G), dim3(256), 0, h->stream, h->phi, h->ldphi, dP, h->nbf, nblk, h->rho,
                      h->bmask);
 }
 
-int launch_density(dg_handle* h) {
+int launch_density(dg_handle* h, const double* dP) {
   if (h->ntb <= 7)  // fragments within 64 KB of LDS
     return run(h, "density", [&] {
       switch (h->ntb) {
-        case 1: launch_density_pl_nt<1>(h); break;
-        case 2: launch_density_pl_nt<2>(h); break;
-        case 3: launch_density_pl_nt<3>(h); break;
-        case 4: launch_density_pl_nt<4>(h); break;
-        case 5: launch_density_pl_nt<5>(h); break;
-        case 6: launch_density_pl_nt<6>(h); break;
-        case 7: launch_density_pl_nt<7>(h); break;
+        case 1: launch_density_pl_nt<1>(h, dP); break;
+        case 2: launch_density_pl_nt<2>(h, dP); break;
+        case 3: launch_density_pl_nt<3>(h, dP); break;
+        case 4: launch_density_pl_nt<4>(h, dP); break;
+        case 5: launch_density_pl_nt<5>(h, dP); break;
+        case 6: launch_density_pl_nt<6>(h, dP); break;
+        case 7: launch_density_pl_nt<7>(h, dP); break;
         default: break;
       }
     });
@@ -1869,11 +1926,12 @@ int launch_tmat(dg_handle* h) {
 
 int stage_density(dg_handle* h, const double* dP) {
   const long nfrag = 128L * h->ntb * (h->ntb + 1);
-  DG_TRY(run(h, "pfrag", [&] {
-    hipLaunchKernelGGL(k_pfrag, dim3(nblocks(nfrag, 256)), dim3(256), 0, h->stream, dP, h->nbf, h->ntb,
-                       h->pfrag);
-  }));
-  DG_TRY(launch_density(h));
+  if (h->ntb > 7)  // the persistent kernel forms its fragments itself
+    DG_TRY(run(h, "pfrag", [&] {
+      hipLaunchKernelGGL(k_pfrag, dim3(nblocks(nfrag, 256)), dim3(256), 0, h->stream, dP, h->nbf, h->ntb,
+                         h->pfrag);
+    }));
+  DG_TRY(launch_density(h, dP));
   h->last_P = dP;
   h->corrected = false;
   h->has_density = true;
@@ -1882,13 +1940,25 @@ int stage_density(dg_handle* h, const double* dP) {
 }
 
 int stage_correct(dg_handle* h) {
-  DG_TRY(run(h, "cell_sum", [&] {
-    hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
-  }));
-  DG_TRY(run(h, "scale", [&] {
-    hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
-                       h->npts_pad, h->q, h->natoms, h->zsum, h->qc, h->scalars);
-  }));
+  const int S = h->cell_split;
+  if (S > 1) {
+    DG_TRY(run(h, "cell_sum", [&] {
+      hipLaunchKernelGGL(k_cell_sum_split, dim3(h->natoms * S), dim3(256), 0, h->stream, h->rho, h->w, h->cell, S,
+                         h->qpart);
+    }));
+    DG_TRY(run(h, "scale", [&] {
+      hipLaunchKernelGGL(k_scale_parts, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
+                         h->npts_pad, h->qpart, S, h->natoms, h->zsum, h->q, h->qc, h->scalars);
+    }));
+  } else {
+    DG_TRY(run(h, "cell_sum", [&] {
+      hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    }));
+    DG_TRY(run(h, "scale", [&] {
+      hipLaunchKernelGGL(k_scale, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->rho,
+                         h->npts_pad, h->q, h->natoms, h->zsum, h->qc, h->scalars);
+    }));
+  }
   h->corrected = true;
   return DG_OK;
 }
@@ -1942,10 +2012,11 @@ int stage_hartree(dg_handle* h) {
   DG_TRY(run(h, "poisson", [&] {
     hipLaunchKernelGGL(k_poisson, dim3(h->natoms * h->nlm), dim3(round_up(h->nrad + 2, 64)),
                        (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->qc, h->rad, h->minvT, h->KD,
-                       h->nrad, h->nlm, h->U, h->tab);
+                       h->shc, h->nrad, h->nlm, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(1024), (h->nlm + 512) * sizeof(double), h->stream,
+    const int jw = (int)std::min<long>(512, round_up(h->nang, 64));
+    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(2 * jw), (h->nlm + 512) * sizeof(double), h->stream,
                        h->U, h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
   }));
   if (h->natoms > 1) {
@@ -2345,6 +2416,8 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->vpart, (size_t)vtotal));
   DG_TRY(dalloc(&h->vbase, (size_t)h->natoms));
   DG_TRY(dalloc(&h->q, h->natoms));
+  h->cell_split = (int)std::max(1, std::min(32, 256 / h->natoms));
+  DG_TRY(dalloc(&h->qpart, (size_t)h->natoms * h->cell_split));
   DG_TRY(dalloc(&h->qc, h->natoms));
   DG_TRY(dalloc(&h->exc_part, h->nlda_blocks));
   DG_TRY(dalloc(&h->scalars, 4 * (size_t)std::max(1, h->nmol)));

@@ -49,14 +49,16 @@ def test_batch_matches_single_molecule_handles():
     Ps = [_guess(m) for m in mols]
     J, XC, sc = b.build(Ps)
     _compare(b, mols, grid, Ps, J, XC, sc, range(12))
-    # the Becke weights and densities are those of the single handles, point for point
+    # the Becke weights are those of the single handles bit for bit; the densities
+    # to rounding (the renormalisation sums its cells in a different order)
     for m in (0, 5, 11):
         g = G.MolecularGrid(mols[m], *grid)
         g.set_density(Ps[m])
         g.correct_densities()
         sl = slice(b.point0[m], b.point0[m + 1])
         assert np.array_equal(b.get_weights()[sl], g.get_weights())
-        assert np.array_equal(b.get_densities()[sl], g.get_densities())
+        r = g.get_densities()
+        assert np.abs(b.get_densities()[sl] - r).max() < 1e-14 * r.max()
 
 
 def test_batch_graph_replays_and_autotune():
