@@ -950,82 +950,122 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // ---------------------------------------------------------------------------
 // Contractions  C_o = Phi^T diag(u_o) Phi  (src/dft.cpp:424-432, src/atomicgrid.cpp:471-488)
 // ---------------------------------------------------------------------------
-// Output tiles of 16x16 on and above the diagonal, gathered in groups of up to
-// four tiles of one tile row (I; J0..J0+nJ-1) that share the A operand.  A
-// work-group takes one group and a range of points (split-K); its 4 waves
-// interleave 4-point steps.  Lane (c, g) feeds A = phi[p_g][16I + c] and
-// B = u[p_g] phi[p_g][16J + c] and receives C[16I + 4v + g][16J + c] in result
-// register v.  The next step's operands are loaded before the current MFMAs.
-// The 4 waves are summed in LDS in a fixed order; part[split][grp][o][t][v][lane].
-// The points of a group are the 64-point blocks blist[loff[grp] .. loff[grp+1])
-// (every block, or with screening the blocks on which tile I and one of the
-// tiles J are significant); split s of gridDim.y takes an equal share of the
-// list, and wave w the 4-point steps 16j + 4w of each of its blocks.
-template <int NOUT>
-__global__ __launch_bounds__(256) void k_contract(const double* __restrict__ phi, int ldphi,
-                                                  const double* __restrict__ u0,
-                                                  const double* __restrict__ u1,
-                                                  const int* __restrict__ groups,
-                                                  const int* __restrict__ blist,
-                                                  const int* __restrict__ loff, int ngroups, double* part) {
-  __shared__ double red[4 * 4 * 4 * 64];
-  const int grp = blockIdx.x, split = blockIdx.y;
-  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
+// Contraction for larger bases with 4 x 2 tile blocks: block {I0, nI, J0, nJ}
+// covers the tiles (I0 + r, J0 + t), r < nI, t < nJ, I <= J, of both outputs,
+// so a wave loads 4 + 2 amplitude tiles per step for 16 MFMAs (a row group of
+// k_contract loads 5 for 8).  Points, splits and prefetch as in k_contract; the
+// 4 waves are added in LDS in wave order and written as 4 reduction groups
+// (I0 + r, J0, nJ) in the k_contract_reduce layout.  T = float (reduced
+// precision): FP32 amplitudes and u Phi on the FP32 16x16x4 MFMA, whose sums
+// are added into FP64 accumulators every 16 steps (64 points per wave); its
+// register v holds row 4g + v and is stored in the FP64 order.
+template <typename T, int NOUT>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 2) void k_contract2d(const T* __restrict__ phi, int ldphi,
+                                                    const double* __restrict__ u0,
+                                                    const double* __restrict__ u1,
+                                                    const int* __restrict__ cblocks,
+                                                    const int* __restrict__ blist,
+                                                    const int* __restrict__ loff, int ngroups, double* part) {
+  __shared__ double red[8 * 256];
+  const int cbi = blockIdx.x, split = blockIdx.y;
+  const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const long l0 = loff[grp], cnt = loff[grp + 1] - l0;
+  const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
   const int* bl = blist + l0 + cnt * split / gridDim.y;
   const int nst = 4 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
   auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
-  const int ia = 16 * I + c, jb = 16 * J0 + c;
-  dbl4 acc[NOUT][4];
+  const int ia = 16 * I0 + c, jb = 16 * J0 + c;
+  constexpr bool f32 = sizeof(T) == 4;
+  typedef typename std::conditional<f32, flt4, dbl4>::type accT;
+  accT acc[NOUT][4][2];
+  dbl4 accd[f32 ? NOUT : 1][4][2];
 #pragma unroll
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
-  double a = 0.0, bb[4] = {0.0, 0.0, 0.0, 0.0}, w0 = 0.0, w1 = 0.0;
-  if (nst > 0) {
-    const long p = point(0);
-    const double* row = phi + p * ldphi;
-    a = row[ia];
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.0;
-    w0 = u0[p];
-    if (NOUT == 2) w1 = u1[p];
-  }
+      for (int t = 0; t < 2; ++t) acc[o][r][t] = accT{0, 0, 0, 0};
+#pragma unroll
+  for (int o = 0; o < (f32 ? NOUT : 1); ++o)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) accd[o][r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  T a[4] = {0, 0, 0, 0}, bb[2] = {0, 0}, w0 = 0, w1 = 0;
+  auto load = [&](long p) {
+    const T* row = phi + p * ldphi;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = (r < nI) ? row[ia + 16 * r] : (T)0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : (T)0;
+    w0 = (T)u0[p];
+    if (NOUT == 2) w1 = (T)u1[p];
+  };
+  if (nst > 0) load(point(0));
   for (int st = 0; st < nst; ++st) {
-    const double ca = a, cw0 = w0, cw1 = w1;
-    double cb[4];
+    T ca[4], cb[2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) cb[t] = bb[t];
-    if (st + 1 < nst) {
-      const long pn = point(st + 1);
-      const double* row = phi + pn * ldphi;
-      a = row[ia];
+    for (int r = 0; r < 4; ++r) ca[r] = a[r];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.0;
-      w0 = u0[pn];
-      if (NOUT == 2) w1 = u1[pn];
+    for (int t = 0; t < 2; ++t) cb[t] = bb[t];
+    const T cw0 = w0, cw1 = w1;
+    if (st + 1 < nst) load(point(st + 1));
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const T b0 = cw0 * cb[t], b1 = cw1 * cb[t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r < nI && t < nJ && I0 + r <= J0 + t) {
+          if constexpr (f32) {
+            acc[0][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b0, acc[0][r][t], 0, 0, 0);
+            if (NOUT == 2)
+              acc[NOUT - 1][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b1, acc[NOUT - 1][r][t], 0, 0, 0);
+          } else {
+            acc[0][r][t] = mfma16x16x4(ca[r], b0, acc[0][r][t]);
+            if (NOUT == 2) acc[NOUT - 1][r][t] = mfma16x16x4(ca[r], b1, acc[NOUT - 1][r][t]);
+          }
+        }
+      }
     }
+    if constexpr (f32) {
+      if ((st & 15) == 15 || st + 1 == nst) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (t < nJ) {
-        acc[0][t] = mfma16x16x4(ca, cw0 * cb[t], acc[0][t]);
-        if (NOUT == 2) acc[NOUT - 1][t] = mfma16x16x4(ca, cw1 * cb[t], acc[NOUT - 1][t]);
+        for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) accd[o][r][t][v] += (double)acc[o][r][t][v];
+              acc[o][r][t] = accT{0, 0, 0, 0};
+            }
       }
     }
   }
 #pragma unroll
   for (int o = 0; o < NOUT; ++o) {
-    __syncthreads();
+    for (int wv = 0; wv < 4; ++wv) {
+      __syncthreads();
+      if (wave == wv) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + v) * 64 + lane] = acc[o][t][v];
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              // FP64 order: register v of lane (c, g) = row 4v + g; FP32: row 4g + v
+              double* q = red + (r * 2 + t) * 256 + (f32 ? g * 64 + c + 16 * v : v * 64 + lane);
+              const double x = f32 ? accd[o][r][t][v] : (double)acc[o][r][t][v];
+              *q = (wv == 0) ? x : *q + x;
+            }
+      }
+    }
     __syncthreads();
-    double* dst = part + (((long)split * ngroups + grp) * NOUT + o) * 1024;
-    for (int e = threadIdx.x; e < nJ * 256; e += 256)
-      dst[e] = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
+    for (int e = threadIdx.x; e < 8 * 256; e += 256) {
+      const int r = e >> 9, t = (e >> 8) & 1, rem = e & 255;
+      part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[e];
+    }
   }
 }
 
@@ -1488,96 +1528,6 @@ __device__ __forceinline__ flt4 mfma16x16x4f32(float a, float b, flt4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Reduced-precision contraction C_o = Phi^T diag(u_o) Phi: the k_contract
-// scheme with FP32 operands (Phi in FP32, u_o Phi formed in FP32) on the FP32
-// 16x16x4 MFMA.  The FP32 accumulators are added into FP64 ones every
-// kFlushSteps steps (64 points per wave), so the FP32 sums stay short.  The
-// FP32 result register v of lane (c, g) holds C[16I + 4g + v][16J + c]; it is
-// stored in the FP64 kernel's order (row 4v + g of register v) so that
-// k_contract_reduce serves both.
-constexpr int kFlushSteps = 16;
-template <int NOUT>
-__global__ __launch_bounds__(256) void k_contract_f32(const float* __restrict__ phf, int ldphi,
-                                                      const double* __restrict__ u0,
-                                                      const double* __restrict__ u1,
-                                                      const int* __restrict__ groups,
-                                                      const int* __restrict__ blist,
-                                                      const int* __restrict__ loff, int ngroups, double* part) {
-  __shared__ double red[4 * 4 * 4 * 64];
-  const int grp = blockIdx.x, split = blockIdx.y;
-  const int I = groups[3 * grp], J0 = groups[3 * grp + 1], nJ = groups[3 * grp + 2];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, c = lane & 15;
-  const long l0 = loff[grp], cnt = loff[grp + 1] - l0;
-  const int* bl = blist + l0 + cnt * split / gridDim.y;
-  const int nst = 4 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
-  auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
-  const int ia = 16 * I + c, jb = 16 * J0 + c;
-  dbl4 accd[NOUT][4];
-  flt4 accf[NOUT][4];
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      accd[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
-      accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
-    }
-  float a = 0.f, bb[4] = {0.f, 0.f, 0.f, 0.f}, w0 = 0.f, w1 = 0.f;
-  if (nst > 0) {
-    const long p = point(0);
-    const float* row = phf + p * ldphi;
-    a = row[ia];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.f;
-    w0 = (float)u0[p];
-    if (NOUT == 2) w1 = (float)u1[p];
-  }
-  for (int st = 0; st < nst; ++st) {
-    const float ca = a, cw0 = w0, cw1 = w1;
-    float cb[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) cb[t] = bb[t];
-    if (st + 1 < nst) {
-      const long pn = point(st + 1);
-      const float* row = phf + pn * ldphi;
-      a = row[ia];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : 0.f;
-      w0 = (float)u0[pn];
-      if (NOUT == 2) w1 = (float)u1[pn];
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (t < nJ) {
-        accf[0][t] = mfma16x16x4f32(ca, cw0 * cb[t], accf[0][t]);
-        if (NOUT == 2) accf[NOUT - 1][t] = mfma16x16x4f32(ca, cw1 * cb[t], accf[NOUT - 1][t]);
-      }
-    }
-    if ((st + 1) % kFlushSteps == 0 || st + 1 == nst) {
-#pragma unroll
-      for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) accd[o][t][v] += (double)accf[o][t][v];
-          accf[o][t] = flt4{0.f, 0.f, 0.f, 0.f};
-        }
-    }
-  }
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o) {
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) red[((wave * 4 + t) * 4 + g) * 64 + c + 16 * v] = accd[o][t][v];
-    __syncthreads();
-    double* dst = part + (((long)split * ngroups + grp) * NOUT + o) * 1024;
-    for (int e = threadIdx.x; e < nJ * 256; e += 256)
-      dst[e] = red[e] + red[1024 + e] + red[2048 + e] + red[3072 + e];
-  }
-}
-
 // out_o = sum over splits of the partial tiles.  A block of 256 threads takes
 // 16 tile elements; thread (element, r) sums the splits r, r+16, ... and the 16
 // partial sums are added in order, so the result does not depend on timing.
@@ -1586,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float* __restrict__ 
 __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restrict__ part, int nsplit,
                                                          int ngroups, int nout,
                                                          const int* __restrict__ groups, int nbf,
-                                                         int colgroups, double* out0, double* out1) {
+                                                         double* out0, double* out1) {
   __shared__ double red[16][17];
   const int el = threadIdx.x & 15, r = threadIdx.x >> 4;
   const long e = (long)blockIdx.x * 16 + el;
@@ -1607,29 +1557,18 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restric
   double tot = 0.0;
   for (int q = 0; q < 16; ++q) tot += red[el][q];
   const int t = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
-  int ti, tj, il;
-  if (colgroups) {  // BF16 layout: tile (I0+t, J), result register v holds row 4g + v
-    if (t >= groups[3 * grp + 2]) return;
-    ti = groups[3 * grp + 1] + t;
-    tj = groups[3 * grp];
-    il = 4 * (ln >> 4) + v;
-  } else {  // FP64 layout: tile (I, J0+t), result register v holds row 4v + g
-    if (t >= groups[3 * grp + 2]) return;
-    ti = groups[3 * grp];
-    tj = groups[3 * grp + 1] + t;
-    il = 4 * v + (ln >> 4);
-  }
+  // tile (I, J0 + t); result register v of lane (c, g) holds row 4v + g
+  if (t >= groups[3 * grp + 2]) return;
+  const int ti = groups[3 * grp], tj = groups[3 * grp + 1] + t, il = 4 * v + (ln >> 4);
   const int jl = ln & 15;
   const int i = 16 * ti + il, j = 16 * tj + jl;
-  if (i >= nbf || j >= nbf || (ti == tj && il > jl)) return;
+  if (i >= nbf || j >= nbf || ti > tj || (ti == tj && il > jl)) return;
   double* out = (o == 0) ? out0 : out1;
   if (!out) return;
   out[i + (long)j * nbf] = tot;
   out[j + (long)i * nbf] = tot;
 }
 
-// One wave of the BF16 16x16x32 MFMA: lane i supplies a[8i..8i+7], b[8i..8i+7]
-// (FP32 values rounded to BF16); d[v*64 + lane] = FP32 result register v.
 // One wave: d[v*64 + lane] = FP32 register v of lane after the FP32 16x16x4 MFMA.
 __global__ void k_mfma_raw_f32(const float* a, const float* b, float* d) {
   const int lane = threadIdx.x;
@@ -1689,8 +1628,11 @@ struct dg_handle {
   bool own_stream = true;
   int natoms = 0, nbf = 0, nshell = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0, precision = 0;
   long npts = 0, npts_pad = 0, cell = 0;
-  int ldphi = 0, ntb = 0, ngroups = 0, nsplit = 0, nsplit_small = 0;
-  long chunk_k = 0, nlda_blocks = 0;
+  int ldphi = 0, ntb = 0, ngroups = 0, nsplit_small = 0;
+  // 4 x 2 tile blocks of k_contract2d: {I0, nI, J0, nJ}, their reduction groups and point lists
+  int *cb = nullptr, *cb_groups = nullptr, *cblist = nullptr, *cbloff = nullptr;
+  int ncb = 0, nsplit2d = 0;
+  long nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
   VwnConst vwn{};
   std::vector<std::pair<int, int>> lchunks;  // l ranges of the cross kernel
@@ -1714,11 +1656,9 @@ struct dg_handle {
   unsigned short* perm = nullptr;  // per item, point order by distance to the item's atom
   long npairs = 0;
   int nitems = 0, cross_wgs = 0;
-  // contraction block lists (64-point blocks per tile group) and screening masks
+  // screening masks
   double screen = 0.0;
   unsigned* bmask = nullptr;
-  int *clist = nullptr, *cloff = nullptr;
-  long nlisted = 0;
   // reduced-precision mode
   float* phf = nullptr;  // FP32 amplitudes
   double* ej_part = nullptr;
@@ -1762,7 +1702,8 @@ struct dg_handle {
     for (double* b : bufs)
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, ash_start, ash_list,
-                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0, clist, cloff};
+                    atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0,
+                    cb, cb_groups, cblist, cbloff};
     if (bmask) (void)hipFree(bmask);
     if (vbase) (void)hipFree(vbase);
     if (perm) (void)hipFree(perm);
@@ -1822,20 +1763,19 @@ int run(dg_handle* h, const char* name, F launch) {
 
 unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Per tile group {a, b0, n} (row groups I = a, J = b0.. for the FP64
-// contraction; column groups J = a, I = b0.. for the reduced-precision one),
-// the 64-point blocks on which tile a and one of the tiles b0..b0+n-1 are
-// significant (all blocks without screening), and the list offsets.
-void group_block_lists(const std::vector<unsigned>& mk, const std::vector<int>& grp, std::vector<int>& list,
-                       std::vector<int>& off) {
-  const int ng = (int)grp.size() / 3;
-  off.assign(ng + 1, 0);
+// Per 4 x 2 tile block {I0, nI, J0, nJ}, the 64-point blocks on which one of its
+// row tiles and one of its column tiles are significant (all blocks without
+// screening), and the list offsets.
+void tile_block_lists(const std::vector<unsigned>& mk, const std::vector<int>& cbs, std::vector<int>& list,
+                      std::vector<int>& off) {
+  const int n = (int)cbs.size() / 4;
+  off.assign(n + 1, 0);
   list.clear();
-  for (int q = 0; q < ng; ++q) {
-    const int a = grp[3 * q], b0 = grp[3 * q + 1], n = grp[3 * q + 2];
-    const unsigned need = ((1u << n) - 1u) << b0;
+  for (int q = 0; q < n; ++q) {
+    const unsigned ri = ((1u << cbs[4 * q + 1]) - 1u) << cbs[4 * q];
+    const unsigned cj = ((1u << cbs[4 * q + 3]) - 1u) << cbs[4 * q + 2];
     for (size_t b = 0; b < mk.size(); ++b)
-      if (((mk[b] >> a) & 1u) && (mk[b] & need)) list.push_back((int)b);
+      if ((mk[b] & ri) && (mk[b] & cj)) list.push_back((int)b);
     off[q + 1] = (int)list.size();
   }
 }
@@ -2086,28 +2026,32 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
   const bool mx = h->precision == DG_PREC_MIXED;
   const bool small = !mx && h->screen == 0.0 && h->ntb <= 5;
   DG_TRY(run(h, "contract", [&] {
-    dim3 grid(h->ngroups, h->nsplit);
     if (small) {
-      launch_contract_small(h, u0, u1, nout);    } else if (mx) {
+      launch_contract_small(h, u0, u1, nout);
+    } else if (mx) {
+      dim3 g2(h->ncb, h->nsplit2d);
       if (nout == 2)
-        hipLaunchKernelGGL(k_contract_f32<2>, grid, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->blocks,
-                           h->clist, h->cloff, h->ngroups, h->cpart);
+        hipLaunchKernelGGL((k_contract2d<float, 2>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->cb,
+                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
       else
-        hipLaunchKernelGGL(k_contract_f32<1>, grid, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, h->blocks,
-                           h->clist, h->cloff, h->ngroups, h->cpart);
+        hipLaunchKernelGGL((k_contract2d<float, 1>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, h->cb,
+                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
     } else {
+      dim3 g2(h->ncb, h->nsplit2d);
       if (nout == 2)
-        hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->blocks,
-                           h->clist, h->cloff, h->ngroups, h->cpart);
+        hipLaunchKernelGGL((k_contract2d<double, 2>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->cb,
+                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
       else
-        hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->blocks,
-                           h->clist, h->cloff, h->ngroups, h->cpart);
+        hipLaunchKernelGGL((k_contract2d<double, 1>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->cb,
+                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
     }
   }));
+  const bool two_d = !small;
+  const int ng = two_d ? 4 * h->ncb : h->ngroups;
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)h->ngroups * nout * 1024, 16)), dim3(256), 0,
-                       h->stream, h->cpart, small ? h->nsplit_small : h->nsplit, h->ngroups, nout, h->blocks, h->nbf,
-                       0, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)ng * nout * 1024, 16)), dim3(256), 0, h->stream,
+                       h->cpart, small ? h->nsplit_small : h->nsplit2d, ng, nout,
+                       two_d ? h->cb_groups : h->blocks, h->nbf, out0, out1);
   });
 }
 
@@ -2329,15 +2273,16 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       blocks.push_back(std::min(4, h->ntb - J0));
     }
   h->ngroups = (int)blocks.size() / 3;
-  {
-    const long target_wg = 1024;
-    long ns = std::max(1L, target_wg / h->ngroups);
-    ns = std::min(std::min(ns, 256L), h->npts_pad / 64);
-    h->chunk_k = round_up((h->npts_pad + ns - 1) / ns, 16);
-    h->nsplit = (int)((h->npts_pad + h->chunk_k - 1) / h->chunk_k);
-    h->nsplit_small = (int)std::max(1L, std::min(256L, h->npts_pad / 256));
-  }
-  // reduced-precision contraction: column groups (J; I0..I0+nI-1) sharing B
+  std::vector<int> cbs, cbg;
+  for (int I0 = 0; I0 < h->ntb; I0 += 4)
+    for (int J0 = I0; J0 < h->ntb; J0 += 2) {
+      const int nI = std::min(4, h->ntb - I0), nJ = std::min(2, h->ntb - J0);
+      cbs.insert(cbs.end(), {I0, nI, J0, nJ});
+      for (int r = 0; r < 4; ++r) cbg.insert(cbg.end(), {I0 + r, J0, r < nI ? nJ : 0});
+    }
+  h->ncb = (int)cbs.size() / 4;
+  h->nsplit2d = (int)std::max(1L, std::min(std::min(256L, 1024L / h->ncb + 1), h->npts_pad / 64));
+  h->nsplit_small = (int)std::max(1L, std::min(256L, h->npts_pad / 256));
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
   {
@@ -2434,7 +2379,10 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, h->items_cap));
   HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
-  DG_TRY(dalloc(&h->cpart, (size_t)std::max(h->nsplit, h->nsplit_small) * h->ngroups * 2 * 1024));
+  DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit_small * h->ngroups,
+                                     (size_t)h->nsplit2d * h->ncb * 4) * 2 * 1024));
+  DG_TRY(dalloc(&h->cb, cbs.size()));
+  DG_TRY(dalloc(&h->cb_groups, cbg.size()));
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
     HIP_TRY(hipMalloc((void**)&h->phf, (size_t)np * h->ldphi * sizeof(float)));
@@ -2468,6 +2416,8 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   HIP_TRY(up(h->invh, invh.data(), invh.size() * 8));
   HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
+  HIP_TRY(up(h->cb, cbs.data(), cbs.size() * 4));
+  HIP_TRY(up(h->cb_groups, cbg.data(), cbg.size() * 4));
   if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
   HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
   HIP_TRY(up(h->vbase, vbase.data(), vbase.size() * 8));
@@ -2522,13 +2472,14 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       HIP_TRY(hipMemcpyAsync(mk.data(), h->bmask, nblk * 4, hipMemcpyDeviceToHost, h->stream));
       DG_TRY(sync(h));
     }
-    std::vector<int> list, off;
-    group_block_lists(mk, blocks, list, off);
-    h->nlisted = (long)list.size();
-    DG_TRY(dalloc(&h->clist, std::max<size_t>(list.size(), 1)));
-    DG_TRY(dalloc(&h->cloff, off.size()));
-    HIP_TRY(hipMemcpy(h->clist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->cloff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    {
+      std::vector<int> l2, o2;
+      tile_block_lists(mk, cbs, l2, o2);
+      DG_TRY(dalloc(&h->cblist, std::max<size_t>(l2.size(), 1)));
+      DG_TRY(dalloc(&h->cbloff, o2.size()));
+      HIP_TRY(hipMemcpy(h->cblist, l2.data(), l2.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(h->cbloff, o2.data(), o2.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   DG_TRY(sync(h));
   guard.release();

@@ -390,7 +390,7 @@ class HostSCF {
         t2 += P_[i + (size_t)k * n_] * J_[k + (size_t)i * n_];
       }
     e_one_ = 2.0 * t1;
-    // In reduced precision J carries BF16-product errors that would enter 2 tr(PJ)
+    // In reduced precision J carries FP32-product errors that would enter 2 tr(PJ)
     // at first order; the same quantity evaluated pointwise in FP64 does not.
     e_j_ = mixed_ ? ej_point_ : 2.0 * t2;
     et_ = e_one_ + e_j_ + enuc_ + exc_;
