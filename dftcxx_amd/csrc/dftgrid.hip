@@ -2139,8 +2139,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
 
   auto h = new dg_handle();
   std::unique_ptr<dg_handle> guard(h);
-  HIP_TRY(hipGetDevice(&h->device));
-  HIP_TRY(hipStreamCreate(&h->stream));
   h->natoms = mol->natoms;
   h->nrad = prm->n_radial;
   h->nang = prm->n_angular;
@@ -2323,6 +2321,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   h->nitems = (int)items.size() / 4;
   h->items_cap = std::max<size_t>(bs ? cross_items(h, 1024).size() : items.size(), 4);
   h->npairs = items.empty() ? 0 : (long)items[items.size() - 1] + items[items.size() - 2] - items[items.size() - 3];
+  // host-side checks and tables are done: from here on the device is used
+  HIP_TRY(hipGetDevice(&h->device));
+  HIP_TRY(hipStreamCreate(&h->stream));
   {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));

@@ -202,3 +202,44 @@ def test_cpp_facade_compiles(tmp_path):
                         "-Wl,-rpath," + os.path.dirname(G.LIB_PATH)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert subprocess.run([str(exe)]).returncode == 0
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(nrad=4), "n_radial"),
+    (dict(nang=7), "Lebedev"),
+    (dict(lmax=13), "lmax"),
+    (dict(Z=19), "Z outside"),
+    (dict(coincide=True), "coincide"),
+    (dict(L=3), "L must be"),
+    (dict(alpha=-1.0), "exponent"),
+    (dict(screen=-1.0), "screen"),
+    (dict(precision=7), "precision"),
+])
+def test_create_rejects_invalid_input(change, msg):
+    # argument checks run before any device call, so they hold without a GPU
+    import ctypes as C
+    mol = M.h2o()
+    Z = np.array(mol.Z, np.int32)
+    xyz = np.array(mol.xyz)
+    if "Z" in change:
+        Z[0] = change["Z"]
+    if change.get("coincide"):
+        xyz[1] = xyz[0]
+    s = mol.shells()
+    L = np.array(s["L"], np.int32)
+    alpha = np.array(s["alpha"])
+    if "L" in change:
+        L[0] = change["L"]
+    if "alpha" in change:
+        alpha[0] = change["alpha"]
+    keep = [Z, xyz, np.ascontiguousarray(s["atom"], np.int32), L, np.ascontiguousarray(s["nprim"], np.int32),
+            np.ascontiguousarray(s["offset"], np.int32), alpha, np.ascontiguousarray(s["coef"])]
+    m = G._Molecule(mol.natoms, G._i(keep[0]), G._d(keep[1]))
+    b = G._Basis(len(keep[2]), G._i(keep[2]), G._i(keep[3]), G._i(keep[4]), G._i(keep[5]), G._d(keep[6]),
+                 G._d(keep[7]))
+    prm = G._GridParams(change.get("nrad", 20), change.get("nang", 26), change.get("lmax", 4),
+                        change.get("precision", 0), change.get("screen", 0.0))
+    h = C.c_void_p()
+    rc = G.lib().dg_create(C.byref(m), C.byref(b), C.byref(prm), C.byref(h))
+    assert rc == 1 and not h.value
+    assert msg in G.lib().dg_last_error().decode()
