@@ -2280,7 +2280,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     }
   h->ncb = (int)cbs.size() / 4;
   h->nsplit2d = (int)std::max(1L, std::min(std::min(256L, 1024L / h->ncb + 1), h->npts_pad / 64));
-  h->nsplit_small = (int)std::max(1L, std::min(256L, h->npts_pad / 256));
+  h->nsplit_small = (int)std::max(1L, std::min(512L, h->npts_pad / 256));
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
   {
