@@ -259,7 +259,9 @@ int batch_mol_of_atom(const dg_handle* h, int k) {
 // Work list of the cross-atom potentials: for every atom k, the points of its
 // molecule outside its own cell, in ranges of at most len (<= kMaxItem) points,
 // k ascending: {k, p_begin, p_end, offset of the item's point order in perm}.
-std::vector<int> cross_items(const dg_handle* h, int len) {
+// An item's order holds its points and at most one pad slot per spline
+// interval group (nrad + 1 groups); *perm_len receives the total.
+std::vector<int> cross_items(const dg_handle* h, int len, long* perm_len) {
   std::vector<int> items;
   long off = 0;
   auto add = [&](int k, long b, long e) {
@@ -269,7 +271,7 @@ std::vector<int> cross_items(const dg_handle* h, int len) {
       items.push_back((int)p);
       items.push_back((int)q);
       items.push_back((int)off);
-      off += q - p;
+      off += (q - p) + std::min<long>(q - p, h->nrad + 1);
     }
   };
   for (int k = 0; k < h->natoms; ++k) {
@@ -283,12 +285,15 @@ std::vector<int> cross_items(const dg_handle* h, int len) {
     add(k, p0, (long)k * h->cell);
     add(k, (long)(k + 1) * h->cell, p1);
   }
+  if (perm_len) *perm_len = off;
   return items;
 }
 
 int set_items(dg_handle* h, int len) {
-  std::vector<int> items = cross_items(h, len);
-  if (items.size() > h->items_cap) return fail(DG_ERR_INVALID, "cross-potential item list exceeds its buffer");
+  long plen = 0;
+  std::vector<int> items = cross_items(h, len, &plen);
+  if (items.size() > h->items_cap || plen > h->npairs)
+    return fail(DG_ERR_INVALID, "cross-potential item list exceeds its buffer");
   if (!items.empty())
     HIP_TRY(hipMemcpy(h->items, items.data(), items.size() * 4, hipMemcpyHostToDevice));
   h->nitems = (int)items.size() / 4;

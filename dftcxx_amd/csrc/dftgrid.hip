@@ -508,136 +508,160 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
 // Hartree potential (src/atomicgrid.cpp:241-466, src/moleculargrid.cpp:336-389)
 // ---------------------------------------------------------------------------
 
-// rho_lm[a][i][lm] = 4 pi sum_j rho_p Y_lm(Omega_j) w_j wb_p.  One block per
-// (a, kShells consecutive radial shells): the shells' rho w_j wb_p go to LDS,
-// wave w sums the points j = w (mod 4) for lm = lane, lane+64, lane+128 of all
-// the block's shells (each Y row is read once for them), the four wave sums are
-// added in order.
-constexpr int kShells = 2;
-__global__ __launch_bounds__(256) void k_rho_lm(const double* __restrict__ rho,
-                                                const double* __restrict__ wb,
-                                                const double* __restrict__ leb,
-                                                const double* __restrict__ ylm, int nrad, int nang,
-                                                int nlm, double* rho_lm) {
-  extern __shared__ double sh[];
-  double* f = sh;                      // kShells x nang
-  double* part = sh + kShells * nang;  // 4 waves x kShells x 192
-  const int groups = (nrad + kShells - 1) / kShells;
-  const int a = blockIdx.x / groups, i0 = (blockIdx.x % groups) * kShells;
-  const int ns = min(kShells, nrad - i0);
-  const long base = ((long)a * nrad + i0) * nang;
-  for (int e = threadIdx.x; e < ns * nang; e += 256) {
-    const int j = e % nang;
-    f[e] = rho[base + e] * leb[4 * j + 3] * wb[base + e];
-  }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double acc[kShells][3] = {};
-  for (int j = wave; j < nang; j += 4) {
-    const double* y = ylm + (long)j * nlm;
-    double yv[3];
+// The three small per-atom products of the Hartree stage run as FP64 MFMA
+// tiles (lane (c, g) feeds A[c][g] and B[g][c]; result register v holds row
+// 4v + g, column c), summing the inner index in steps of 4 with zero padding.
+// Each wave loads the operands of kSteps steps before it issues their MFMAs,
+// so that one memory latency covers kSteps steps.
+constexpr int kSteps = 4;
+
+// rho_lm[a][i][lm] = 4 pi sum_j F[i][j] Y[j][lm] with F[i][j] = rho_p w_j wb_p,
+// p = (a nrad + i) nang + j (src/atomicgrid.cpp:241-296).  Block (a, 16 shells
+// i0.., group of NT lm tiles): the 8 waves split the angular points (wave w
+// takes j = 4 (w + 8 t) + 0..3), each A fragment feeds the NT tiles, and the
+// wave partials are added in wave order through LDS.
+constexpr int kRhoWaves = 8;
+template <int NT>
+__global__ __launch_bounds__(64 * kRhoWaves) void k_rho_lm(const double* __restrict__ rho,
+                                                           const double* __restrict__ wb,
+                                                           const double* __restrict__ leb,
+                                                           const double* __restrict__ ylm, int nrad, int nang,
+                                                           int nlm, double* rho_lm) {
+  __shared__ double part[kRhoWaves][NT * 256];
+  const int ti = (nrad + 15) / 16;
+  const int a = blockIdx.x / ti, i0 = 16 * (blockIdx.x % ti), l0 = 16 * NT * blockIdx.y;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const int i = i0 + c;
+  dbl4 d[NT];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) yv[q] = (lane + 64 * q < nlm) ? y[lane + 64 * q] : 0.0;
+  for (int t = 0; t < NT; ++t) d[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const long prow = ((long)a * nrad + i) * nang;
+  for (int jb = 4 * wave; jb < nang; jb += 4 * kRhoWaves * kSteps) {
+    double av[kSteps], bv[kSteps][NT];
 #pragma unroll
-    for (int sh_ = 0; sh_ < kShells; ++sh_) {
-      const double fj = (sh_ < ns) ? f[sh_ * nang + j] : 0.0;
+    for (int u = 0; u < kSteps; ++u) {
+      const int j = jb + 4 * kRhoWaves * u + g;
+      av[u] = (j < nang && i < nrad) ? rho[prow + j] * leb[4 * j + 3] * wb[prow + j] : 0.0;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) acc[sh_][q] += fj * yv[q];
+      for (int t = 0; t < NT; ++t) {
+        const int lm = l0 + 16 * t + c;
+        bv[u][t] = (j < nang && lm < nlm) ? ylm[(long)j * nlm + lm] : 0.0;
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) d[t] = mfma16x16x4(av[u], bv[u][t], d[t]);
   }
 #pragma unroll
-  for (int sh_ = 0; sh_ < kShells; ++sh_)
+  for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) part[(wave * kShells + sh_) * 192 + lane + 64 * q] = acc[sh_][q];
+    for (int v = 0; v < 4; ++v) part[wave][t * 256 + v * 64 + lane] = d[t][v];
   __syncthreads();
-  for (int e = threadIdx.x; e < ns * nlm; e += 256) {
-    const int sh_ = e / nlm, lm = e % nlm;
-    const double v = part[(0 * kShells + sh_) * 192 + lm] + part[(1 * kShells + sh_) * 192 + lm] +
-                     part[(2 * kShells + sh_) * 192 + lm] + part[(3 * kShells + sh_) * 192 + lm];
-    rho_lm[(((long)a * nrad + i0 + sh_) * nlm) + lm] = v * (4.0 * kPiD);
+  for (int e = threadIdx.x; e < NT * 256; e += 64 * kRhoWaves) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kRhoWaves; ++w) sum += part[w][e];
+    const int t = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+    const int row = i0 + 4 * v + (ln >> 4), lm = l0 + 16 * t + (ln & 15);
+    if (row < nrad && lm < nlm) rho_lm[((long)a * nrad + row) * nlm + lm] = sum * (4.0 * kPiD);
   }
 }
 
-// Radial Poisson solve and spline slopes for one (a, lm) per block
-// (src/atomicgrid.cpp:395-432, src/cspline.cpp:66-127, src/atomicgrid.cpp:535-556):
-// U_i = sum_k KU_l[k][i] g_k  with g_0 = sqrt(4 pi) q_a [lm = 0],
-// g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm], g_{N+1} = 0, KU_l = rows 1..N of the
-// inverse FD matrix; the not-a-knot slopes D of U over the ascending knots are
-// a fixed linear map of U, so D_t = sum_k KD_l[k][t] g_k with KD_l = S KU_l.
+// Radial Poisson solve and spline slopes (src/atomicgrid.cpp:395-432,
+// src/cspline.cpp:66-127, src/atomicgrid.cpp:535-556): per l, with the columns
+// (a, m) of all atoms, U_i = sum_k KU_l[k][i] g_k and D_t = sum_k KD_l[k][t] g_k,
+// where g_0 = sqrt(4 pi) q_a [lm = 0], g_k = -4 pi r_{k-1} rho_lm[a][k-1][lm],
+// g_{N+1} = 0, KU_l = rows 1..N of the inverse FD matrix and KD_l = S KU_l (the
+// not-a-knot slope map S over the ascending knots).  Block x = (l, row tile,
+// column tile, output U or D); tiles past the l's columns exit.
 // tab[a][t][lm] = N_lm {U at knot t, D_t}; knot t is radial node N-1-t, and
 // N_lm (the harmonic's normalisation, shc) is folded in for the cross kernel.
-__global__ void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
-                          const double* __restrict__ rad, const double* __restrict__ KU,
-                          const double* __restrict__ KD, const double* __restrict__ shc, int nrad, int nlm,
-                          double* U, double* tab) {
-  extern __shared__ double gv[];
-  const int a = blockIdx.x / nlm, lm = blockIdx.x % nlm;
-  int l = 0;
-  while ((l + 1) * (l + 1) <= lm) ++l;
-  const int n2 = nrad + 2;
-  for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-    double v = 0.0;
-    if (k == 0)
-      v = (lm == 0) ? sqrt(4.0 * kPiD) * q[a] : 0.0;
-    else if (k <= nrad)
-      v = -4.0 * kPiD * rad[k - 1] * rho_lm[((long)a * nrad + (k - 1)) * nlm + lm];
-    gv[k] = v;
+__global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
+                                                const double* __restrict__ rad, const double* __restrict__ KU,
+                                                const double* __restrict__ KD, const double* __restrict__ shc,
+                                                int natoms, int nrad, int lmax, int tcmax, double* U,
+                                                double* tab) {
+  const int nlm = (lmax + 1) * (lmax + 1), n2 = nrad + 2, ti = (nrad + 15) / 16;
+  int b = blockIdx.x;
+  const int which = b & 1;
+  b >>= 1;
+  const int tc = b % tcmax;
+  b /= tcmax;
+  const int it = b % ti, l = b / ti;
+  const int w = 2 * l + 1, ncol = natoms * w;
+  if (16 * tc >= ncol) return;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int i = 16 * it + c, col = 16 * tc + c;
+  const int ca = col / w, clm = l * l + col % w;
+  const double* M = (which ? KD : KU) + (long)l * n2 * nrad;
+  dbl4 d = {0.0, 0.0, 0.0, 0.0};
+  for (int kb = 0; kb < n2; kb += 4 * kSteps) {
+    double av[kSteps], bv[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+      const int k = kb + 4 * u + g;
+      av[u] = bv[u] = 0.0;
+      if (k < n2) {
+        if (i < nrad) av[u] = M[(long)k * nrad + i];
+        if (col < ncol) {
+          if (k == 0)
+            bv[u] = (clm == 0) ? sqrt(4.0 * kPiD) * q[ca] : 0.0;
+          else if (k <= nrad)
+            bv[u] = -4.0 * kPiD * rad[k - 1] * rho_lm[((long)ca * nrad + (k - 1)) * nlm + clm];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) d = mfma16x16x4(av[u], bv[u], d);
   }
-  __syncthreads();
-  const int i = threadIdx.x;
-  if (i >= nrad) return;
-  const double* Mu = KU + (long)l * n2 * nrad;
-  const double* Md = KD + (long)l * n2 * nrad;
-  double u = 0.0, d = 0.0;
-  for (int k = 0; k < n2; ++k) {
-    const double gk = gv[k];
-    u += Mu[(long)k * nrad + i] * gk;
-    d += Md[(long)k * nrad + i] * gk;
-  }
-  U[((long)a * nrad + i) * nlm + lm] = u;
-  const int m = lm - l * l - l;
+  if (col >= ncol) return;
+  const int m = clm - l * l - l;
   const double nl = shc[3 * (l * (l + 1) / 2 + (m < 0 ? -m : m))];
-  double* T = tab + (long)a * nrad * nlm * 2 + lm * 2;
-  T[(long)(nrad - 1 - i) * nlm * 2] = nl * u;
-  T[(long)i * nlm * 2 + 1] = nl * d;
+  double* T = tab + (long)ca * nrad * nlm * 2 + clm * 2;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = 16 * it + 4 * v + g;
+    if (row >= nrad) continue;
+    if (which == 0) {
+      U[((long)ca * nrad + row) * nlm + clm] = d[v];
+      T[(long)(nrad - 1 - row) * nlm * 2] = nl * d[v];
+    } else {
+      T[(long)row * nlm * 2 + 1] = nl * d[v];
+    }
+  }
 }
 
 // Own-cell potential V_cell(p) = sum_lm U[a][i][lm] Y_lm(Omega_j) / |r_p - R_a|
-// (src/atomicgrid.cpp:434-462).  One block per (a, i) with U[a][i][:] in LDS;
-// thread (j, h) sums half h of the lm range (ylmT is [lm][j], coalesced over j)
-// and the two halves are added in order.  The block has 2 jw threads, jw the
-// angular count rounded up to 64 (at most 512).
-__global__ __launch_bounds__(1024) void k_vcell(const double* __restrict__ U,
-                                                const double* __restrict__ ylmT,
-                                                const double* __restrict__ atoms,
-                                                const double* __restrict__ px,
-                                                const double* __restrict__ py,
-                                                const double* __restrict__ pz, int nrad, int nang,
-                                                int nlm, double* vcell) {
-  extern __shared__ double sh[];
-  double* u = sh;           // nlm
-  double* half = sh + nlm;  // 512
-  const long ai = blockIdx.x;
-  const int a = (int)(ai / nrad);
-  for (int lm = threadIdx.x; lm < nlm; lm += blockDim.x) u[lm] = U[ai * nlm + lm];
-  __syncthreads();
-  const int jw = blockDim.x >> 1;  // angular points per pass (a multiple of 64)
-  const int hsel = threadIdx.x >= jw, jt = threadIdx.x - (hsel ? jw : 0);
-  const int lmid = nlm / 2;
-  const int lb = hsel ? lmid : 0, le = hsel ? nlm : lmid;
-  for (int j0 = 0; j0 < nang; j0 += jw) {
-    const int j = j0 + jt;
-    double acc = 0.0;
-    if (j < nang)
-      for (int lm = lb; lm < le; ++lm) acc += ylmT[(long)lm * nang + j] * u[lm];
-    __syncthreads();
-    if (hsel) half[jt] = acc;
-    __syncthreads();
-    if (!hsel && j < nang) {
-      const long p = ai * nang + j;
-      const double dx = px[p] - atoms[4 * a], dy = py[p] - atoms[4 * a + 1], dz = pz[p] - atoms[4 * a + 2];
-      vcell[p] = (acc + half[jt]) / sqrt(dx * dx + dy * dy + dz * dz);
+// (src/atomicgrid.cpp:434-462): tile (shells i0.., angular points j0..) of atom a.
+// |r_p - R_a| is the radial node r_i (the point is R_a + r_i Omega_j with a unit
+// Omega_j), which spares the coordinate reads.
+__global__ __launch_bounds__(64) void k_vcell(const double* __restrict__ U, const double* __restrict__ ylm,
+                                              const double* __restrict__ rad, int nrad, int nang, int nlm,
+                                              double* vcell) {
+  const int ti = (nrad + 15) / 16, tj = (nang + 15) / 16;
+  const int a = blockIdx.x / (ti * tj), rem = blockIdx.x % (ti * tj);
+  const int i0 = 16 * (rem / tj), j0 = 16 * (rem % tj);
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int i = i0 + c, j = j0 + c;
+  dbl4 d = {0.0, 0.0, 0.0, 0.0};
+  for (int kb = 0; kb < nlm; kb += 4 * kSteps) {
+    double av[kSteps], bv[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+      const int k = kb + 4 * u + g;
+      av[u] = (k < nlm && i < nrad) ? U[((long)a * nrad + i) * nlm + k] : 0.0;
+      bv[u] = (k < nlm && j < nang) ? ylm[(long)j * nlm + k] : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) d = mfma16x16x4(av[u], bv[u], d);
+  }
+  if (j >= nang) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = i0 + 4 * v + g;
+    if (row >= nrad) continue;
+    vcell[((long)a * nrad + row) * nang + j] = d[v] / rad[row];
   }
 }
 
@@ -651,21 +675,74 @@ struct CrossArgs {
   const double* tab;    // [natoms][nrad][nlm]{y, D}
   const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
   const int* items;     // {k, p_begin, p_end, perm offset} point ranges outside cell k, k ascending
-  const unsigned short* perm;  // per item, its points in order of distance to atom k
+  const unsigned short* perm;  // per item, its points by distance to atom k, interval groups padded to even
+  const unsigned short* pkey;  // interval key (spline_key) of every perm slot
+  const int* plen;             // per item, the padded length of its perm range
   double* vpart;        // partial potentials, atom k's at vpart + vbase[k] + p
   const long* vbase;
   int nitems, nrad, nlm, l0, l1, accumulate;
 };
 
+// Distance r of a point from an atom centre and its distance rxy from the
+// atom's z axis, given the offset (dx, dy, dz).  One definition for the item
+// sort and the potential kernel, so that both see the same r.
+__device__ __forceinline__ void point_polar(double dx, double dy, double dz, double& rxy, double& r) {
+  const double rxy2 = dx * dx + dy * dy;
+  r = sqrt(rxy2 + dz * dz);
+  rxy = sqrt(rxy2);
+}
+
+// Spline interval of r over the ascending knots kn[0..n): the reference's first
+// knot s >= 1 with r <= kn[s] (the interval is [s-1, s]), or 0 / n for r below
+// kn[0] / at or beyond kn[n-1], where the end values are used (src/cspline.cpp:156-168).
+__device__ __forceinline__ int spline_key(double r, const double* kn, int n) {
+  if (r < kn[0]) return 0;
+  if (r >= kn[n - 1]) return n;
+  int lo = 1, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r <= kn[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// Inclusive scan (maximum or sum) of a[0..len) by the whole work-group, ping-ponging
+// between a and b; returns the buffer that holds the result.
+template <bool MAX>
+__device__ int* block_scan(int* a, int* b, int len) {
+  for (int d = 1; d < len; d <<= 1) {
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      const int v = a[i];
+      b[i] = (i >= d) ? (MAX ? max(v, a[i - d]) : v + a[i - d]) : v;
+    }
+    __syncthreads();
+    int* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
 // Points of every cross-potential item in order of their distance to the
-// item's atom: perm[off + i] is the offset (from p_begin) of the i-th nearest
-// point.  One work-group per item (at most kMaxItem points), bitonic sort in LDS.
+// item's atom, grouped by spline interval, each group padded to an even count:
+// perm[off + i] is the offset (from p_begin) of the i-th point, 0xFFFF a pad
+// slot (always at an odd position), pkey[off + i] the slot's interval key
+// (spline_key) and plen[item] the padded length.  Two consecutive slots from an
+// even position then always share their interval.  One work-group per item (at
+// most kMaxItem points), bitonic sort in LDS.
 constexpr int kMaxItem = 4096;
+constexpr unsigned short kPadSlot = 0xFFFF;
 __global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ items, const double* __restrict__ atoms,
                                                      const double* __restrict__ px, const double* __restrict__ py,
-                                                     const double* __restrict__ pz, unsigned short* perm) {
+                                                     const double* __restrict__ pz, const double* __restrict__ knots,
+                                                     int nrad, unsigned short* perm, unsigned short* pkey,
+                                                     int* plen) {
   __shared__ double key[kMaxItem];
   __shared__ unsigned short idx[kMaxItem];
+  __shared__ int grp[kMaxItem];
   const int* it = items + 4 * blockIdx.x;
   const int k = it[0];
   const long pb = it[1];
@@ -676,8 +753,8 @@ __global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ ite
   for (int i = threadIdx.x; i < n2; i += blockDim.x) {
     double r = 1e300;
     if (i < len) {
-      const double dx = px[pb + i] - ax, dy = py[pb + i] - ay, dz = pz[pb + i] - az;
-      r = dx * dx + dy * dy + dz * dz;
+      double rxy;
+      point_polar(px[pb + i] - ax, py[pb + i] - ay, pz[pb + i] - az, rxy, r);
     }
     key[i] = r;
     idx[i] = (unsigned short)i;
@@ -701,7 +778,48 @@ __global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ ite
       }
       __syncthreads();
     }
-  for (int i = threadIdx.x; i < len; i += blockDim.x) perm[it[3] + i] = idx[i];
+  for (int i = threadIdx.x; i < len; i += blockDim.x) grp[i] = spline_key(key[i], knots, nrad);
+  __syncthreads();
+  // the keys are done with: their space holds two scan buffers
+  int* sa = reinterpret_cast<int*>(key);
+  int* sb = sa + kMaxItem;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) sa[i] = (i == 0 || grp[i] != grp[i - 1]) ? i : 0;
+  __syncthreads();
+  const int* gs = block_scan<true>(sa, sb, len);  // first sorted position of each point's group
+  int flag[kMaxItem / 1024];
+#pragma unroll
+  for (int r = 0; r < kMaxItem / 1024; ++r) {
+    const int i = threadIdx.x + 1024 * r;
+    flag[r] = 0;
+    if (i < len) {
+      const bool last = i == len - 1 || grp[i + 1] != grp[i];
+      flag[r] = (last && ((i - gs[i] + 1) & 1)) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kMaxItem / 1024; ++r) {
+    const int i = threadIdx.x + 1024 * r;
+    if (i < len) sa[i] = flag[r];
+  }
+  __syncthreads();
+  const int* pads = block_scan<false>(sa, sb, len);  // pad slots up to and including i's group end
+  unsigned short* out = perm + it[3];
+  unsigned short* okey = pkey + it[3];
+#pragma unroll
+  for (int r = 0; r < kMaxItem / 1024; ++r) {
+    const int i = threadIdx.x + 1024 * r;
+    if (i < len) {
+      const int pos = i + pads[i] - flag[r];
+      out[pos] = idx[i];
+      okey[pos] = (unsigned short)grp[i];
+      if (flag[r]) {
+        out[pos + 1] = kPadSlot;
+        okey[pos + 1] = (unsigned short)grp[i];
+      }
+    }
+  }
+  if (threadIdx.x == 0) plen[blockIdx.x] = len > 0 ? len + pads[len - 1] : 0;
 }
 
 // Potential of atom k's spherical-harmonic expansion at every point outside
@@ -709,16 +827,17 @@ __global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ ite
 // (src/moleculargrid.cpp:349-376).  Persistent: work-group g walks the items
 // [g n/G, (g+1) n/G) of the (atom, point range) list, reloading atom k's spline
 // table for l in [l0, l1) into LDS when k changes.  Each thread takes Q
-// consecutive points of the item in order of distance to atom k (perm), runs
-// the associated-Legendre / Chebyshev recurrences for each and forms the four
-// sums sum Y y_{s-1}, sum Y D_{s-1}, sum Y y_s, sum Y D_s per point, from which
-// the cubic on interval s is evaluated once (linear in the per-lm
-// coefficients).  When the Q points share their interval (the usual case for
-// distance-sorted points) each table entry is read from LDS once for all Q.
-// The interval is the reference's first knot with r <= x_s, found by
-// bisection; r outside [x_0, x_max) clamps to the end values (src/cspline.cpp:156-168).
+// consecutive slots of the item's padded order (perm: by distance to atom k,
+// interval groups padded to even), runs the associated-Legendre / Chebyshev
+// recurrences for each and forms the four sums sum Y y_{s-1}, sum Y D_{s-1},
+// sum Y y_s, sum Y D_s per point, from which the cubic on interval s is
+// evaluated once (linear in the per-lm coefficients).  The Q = 2 slots of a
+// thread share their interval (a pad slot repeats its partner's point and is
+// not written), taken from the sort (pkey), so each table entry is read from
+// LDS once for both.
 template <int LMAX, int Q>
 __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
+  static_assert(Q == 2, "slots are paired by the item sort");
   extern __shared__ double lds[];
   const int n = A.nrad;
   const int lmb = A.l0 * A.l0, nlmc = A.l1 * A.l1 - lmb;
@@ -745,24 +864,30 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       cur = k;
     }
     const long pb = A.items[4 * it + 1];
-    const int len = A.items[4 * it + 2] - A.items[4 * it + 1];
+    const int len = A.plen[it];
     const unsigned short* pm = A.perm + A.items[4 * it + 3];
+    const unsigned short* pk = A.pkey + A.items[4 * it + 3];
     const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
-    const double xlo = kn[0], xhi = kn[n - 1];
     for (int q0 = threadIdx.x * Q; q0 < len; q0 += blockDim.x * Q) {
       long pq[Q];
       double rinv[Q], ct[Q], st[Q], cp[Q], sp[Q], tq[Q];
-      int s0[Q], s1[Q];
-      bool clamp[Q];
+      bool valid[Q];
+      unsigned short e0 = pm[q0];
+      if (e0 == kPadSlot) e0 = 0;  // not produced by the sort; kept in range regardless
+      const int key = pk[q0];
+      const bool clamp = key == 0 || key == n;
+      const int s1 = clamp ? (key == 0 ? 0 : n - 1) : key;
+      const int s0 = clamp ? s1 : key - 1;
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const int pos = (q0 + q < len) ? q0 + q : q0;  // a missing tail point repeats the first
-        pq[q] = pb + pm[pos];
+        // a pad slot or a missing tail slot repeats the thread's first point
+        const unsigned short e = (q0 + q < len) ? pm[q0 + q] : kPadSlot;
+        valid[q] = e != kPadSlot;
+        pq[q] = pb + (valid[q] ? e : e0);
         const double dx = A.px[pq[q]] - Ax, dy = A.py[pq[q]] - Ay, dz = A.pz[pq[q]] - Az;
-        const double rxy2 = dx * dx + dy * dy;
-        const double r = sqrt(rxy2 + dz * dz);
+        double rxy, r;
+        point_polar(dx, dy, dz, rxy, r);
         rinv[q] = 1.0 / r;
-        const double rxy = sqrt(rxy2);
         ct[q] = dz * rinv[q];
         st[q] = rxy * rinv[q];
         cp[q] = 1.0;
@@ -772,30 +897,8 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
           cp[q] = dx * ir;
           sp[q] = dy * ir;
         }
-        tq[q] = 0.0;
-        clamp[q] = true;
-        if (r < xlo) {
-          s0[q] = s1[q] = 0;
-        } else if (r >= xhi) {
-          s0[q] = s1[q] = n - 1;
-        } else {
-          int lo = 1, hi = n - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (r <= kn[mid])
-              hi = mid;
-            else
-              lo = mid + 1;
-          }
-          s1[q] = lo;
-          s0[q] = lo - 1;
-          tq[q] = r - kn[s0[q]];
-          clamp[q] = false;
-        }
+        tq[q] = clamp ? 0.0 : r - kn[s0];
       }
-      bool same = true;
-#pragma unroll
-      for (int q = 1; q < Q; ++q) same = same && s0[q] == s0[0] && s1[q] == s1[0];
       double sy0[Q], sd0[Q], sy1[Q], sd1[Q], pmm[Q], cm[Q], sm[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
@@ -837,45 +940,29 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             yp[q] = (m == 0) ? base : base * cm[q];
             yn[q] = base * sm[q];
           }
+          // (a run-time l range: its branches also keep the table reads from being
+          // hoisted en masse, which would spill at 1024 threads)
           if (l >= A.l0 && l < A.l1) {
             const int ip = (l * l + l + m - lmb) * 2;
             const int in = (l * l + l - m - lmb) * 2;
-            if (same) {
-              const double* T0 = tab + s0[0] * nlmc * 2;
-              const double* T1 = tab + s1[0] * nlmc * 2;
-              const double a0 = T0[ip], a1 = T0[ip + 1], b0 = T1[ip], b1 = T1[ip + 1];
+            const double* T0 = tab + s0 * nlmc * 2;
+            const double* T1 = tab + s1 * nlmc * 2;
+            const double a0 = T0[ip], a1 = T0[ip + 1], b0 = T1[ip], b1 = T1[ip + 1];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+              sy0[q] += yp[q] * a0;
+              sd0[q] += yp[q] * a1;
+              sy1[q] += yp[q] * b0;
+              sd1[q] += yp[q] * b1;
+            }
+            if (m > 0) {
+              const double c0 = T0[in], c1 = T0[in + 1], e0 = T1[in], e1 = T1[in + 1];
 #pragma unroll
               for (int q = 0; q < Q; ++q) {
-                sy0[q] += yp[q] * a0;
-                sd0[q] += yp[q] * a1;
-                sy1[q] += yp[q] * b0;
-                sd1[q] += yp[q] * b1;
-              }
-              if (m > 0) {
-                const double c0 = T0[in], c1 = T0[in + 1], e0 = T1[in], e1 = T1[in + 1];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                  sy0[q] += yn[q] * c0;
-                  sd0[q] += yn[q] * c1;
-                  sy1[q] += yn[q] * e0;
-                  sd1[q] += yn[q] * e1;
-                }
-              }
-            } else {
-#pragma unroll
-              for (int q = 0; q < Q; ++q) {
-                const double* T0 = tab + s0[q] * nlmc * 2;
-                const double* T1 = tab + s1[q] * nlmc * 2;
-                sy0[q] += yp[q] * T0[ip];
-                sd0[q] += yp[q] * T0[ip + 1];
-                sy1[q] += yp[q] * T1[ip];
-                sd1[q] += yp[q] * T1[ip + 1];
-                if (m > 0) {
-                  sy0[q] += yn[q] * T0[in];
-                  sd0[q] += yn[q] * T0[in + 1];
-                  sy1[q] += yn[q] * T1[in];
-                  sd1[q] += yn[q] * T1[in + 1];
-                }
+                sy0[q] += yn[q] * c0;
+                sd0[q] += yn[q] * c1;
+                sy1[q] += yn[q] * e0;
+                sd1[q] += yn[q] * e1;
               }
             }
           }
@@ -883,12 +970,12 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       }
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        if (q0 + q >= len) break;
+        if (!valid[q]) continue;
         double val;
-        if (clamp[q]) {
+        if (clamp) {
           val = sy0[q];
         } else {
-          const double hi_ = ih[s1[q]];
+          const double hi_ = ih[s1];
           const double t = tq[q];
           const double dy = (sy1[q] - sy0[q]) * hi_;
           const double c = hi_ * (3.0 * dy - 2.0 * sd0[q] - sd1[q]);
@@ -1648,13 +1735,15 @@ struct dg_handle {
   double *q = nullptr, *qc = nullptr, *exc_part = nullptr, *scalars = nullptr;
   double* qpart = nullptr;  // split cell populations (cell_split > 1)
   int cell_split = 1;
-  double *ylm = nullptr, *ylmT = nullptr, *minvT = nullptr, *shc = nullptr;
+  double *ylm = nullptr, *minvT = nullptr, *shc = nullptr;
   double *knots = nullptr, *invh = nullptr, *KD = nullptr;
   double *rho_lm = nullptr, *U = nullptr, *tab = nullptr;
   int* blocks = nullptr;
   int* items = nullptr;
-  unsigned short* perm = nullptr;  // per item, point order by distance to the item's atom
-  long npairs = 0;
+  unsigned short* perm = nullptr;  // per item, point order by distance to the item's atom (padded)
+  unsigned short* pkey = nullptr;  // interval key of every perm slot
+  int* plen = nullptr;             // per item, padded length of its order
+  long npairs = 0;                 // capacity of perm
   int nitems = 0, cross_wgs = 0;
   // screening masks
   double screen = 0.0;
@@ -1695,18 +1784,19 @@ struct dg_handle {
   ~dg_handle() {
     double* bufs[] = {atoms, invR,  rad,   wrad,   leb,   px,     py,    pz,   w,     wb,
                       phi,   pr_alpha, pr_cn0, pr_cn1, P, pfrag, rho,  uxc,   exd,   uJ,
-                      V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, ylmT, minvT, shc,
+                      V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, minvT, shc,
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC,
                       adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP,
                       mol_z, mol_fac, mol_qraw, atom_exc, atom_ej, qpart};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, ash_start, ash_list,
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, plen, ash_start, ash_list,
                     atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0,
                     cb, cb_groups, cblist, cbloff};
     if (bmask) (void)hipFree(bmask);
     if (vbase) (void)hipFree(vbase);
     if (perm) (void)hipFree(perm);
+    if (pkey) (void)hipFree(pkey);
     if (mol_poff) (void)hipFree(mol_poff);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1785,7 +1875,7 @@ int sort_items(dg_handle* h) {
   if (h->nitems == 0) return DG_OK;
   return run(h, "sort_items", [&] {
     hipLaunchKernelGGL(k_sort_items, dim3(h->nitems), dim3(1024), 0, h->stream, h->items, h->atoms, h->px, h->py,
-                       h->pz, h->perm);
+                       h->pz, h->knots, h->nrad, h->perm, h->pkey, h->plen);
   });
 }
 
@@ -1917,6 +2007,21 @@ hipError_t allow_cross_lds(size_t bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// NT lm tiles per block: all of them up to lmax 8 (81 functions), else two groups.
+void launch_rho_lm(dg_handle* h, int ti) {
+  const int tiles = (h->nlm + 15) / 16, nt = tiles <= 6 ? tiles : (tiles + 1) / 2;
+  const dim3 grid(h->natoms * ti, (tiles + nt - 1) / nt), block(64 * kRhoWaves);
+#define DG_RHO(N) \
+  case N: \
+    hipLaunchKernelGGL(k_rho_lm<N>, grid, block, 0, h->stream, h->rho, h->wb, h->leb, h->ylm, h->nrad, h->nang, \
+                       h->nlm, h->rho_lm); \
+    break;
+  switch (nt) {
+    DG_RHO(1) DG_RHO(2) DG_RHO(3) DG_RHO(4) DG_RHO(5) DG_RHO(6)
+  }
+#undef DG_RHO
+}
+
 hipError_t allow_cross_lds_l(int lmax, size_t bytes) {
   switch (lmax) {
 #define DG_L(N) \
@@ -1943,21 +2048,16 @@ void launch_cross(dg_handle* h, CrossArgs& a, dim3 grid) {
 }
 
 int stage_hartree(dg_handle* h) {
-  DG_TRY(run(h, "rho_lm", [&] {
-    const int groups = (h->nrad + kShells - 1) / kShells;
-    hipLaunchKernelGGL(k_rho_lm, dim3(h->natoms * groups), dim3(256),
-                       (kShells * h->nang + 4 * kShells * 192) * sizeof(double), h->stream, h->rho, h->wb, h->leb,
-                       h->ylm, h->nrad, h->nang, h->nlm, h->rho_lm);
-  }));
+  const int ti = (h->nrad + 15) / 16;
+  DG_TRY(run(h, "rho_lm", [&] { launch_rho_lm(h, ti); }));
   DG_TRY(run(h, "poisson", [&] {
-    hipLaunchKernelGGL(k_poisson, dim3(h->natoms * h->nlm), dim3(round_up(h->nrad + 2, 64)),
-                       (h->nrad + 2) * sizeof(double), h->stream, h->rho_lm, h->qc, h->rad, h->minvT, h->KD,
-                       h->shc, h->nrad, h->nlm, h->U, h->tab);
+    const int tcmax = (h->natoms * (2 * h->lmax + 1) + 15) / 16;
+    hipLaunchKernelGGL(k_poisson, dim3((h->lmax + 1) * ti * tcmax * 2), dim3(64), 0, h->stream, h->rho_lm, h->qc,
+                       h->rad, h->minvT, h->KD, h->shc, h->natoms, h->nrad, h->lmax, tcmax, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    const int jw = (int)std::min<long>(512, round_up(h->nang, 64));
-    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * h->nrad), dim3(2 * jw), (h->nlm + 512) * sizeof(double), h->stream,
-                       h->U, h->ylmT, h->atoms, h->px, h->py, h->pz, h->nrad, h->nang, h->nlm, h->vcell);
+    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * ti * ((h->nang + 15) / 16)), dim3(64), 0, h->stream, h->U, h->ylm,
+                       h->rad, h->nrad, h->nang, h->nlm, h->vcell);
   }));
   if (h->natoms > 1) {
     CrossArgs a;
@@ -1971,6 +2071,8 @@ int stage_hartree(dg_handle* h) {
     a.shc = h->shc;
     a.items = h->items;
     a.perm = h->perm;
+    a.plen = h->plen;
+    a.pkey = h->pkey;
     a.vpart = h->vpart;
     a.vbase = h->vbase;
     a.nitems = h->nitems;
@@ -2086,7 +2188,7 @@ struct BatchSpec {
   int nmol = 0;
   std::vector<int> mol_atom0, mol_shell0;
 };
-std::vector<int> cross_items(const dg_handle* h, int len);
+std::vector<int> cross_items(const dg_handle* h, int len, long* perm_len = nullptr);
 int batch_alloc(dg_handle* h, const BatchSpec* bs, const std::vector<int>& amol);
 int batch_setup(dg_handle* h);
 
@@ -2213,11 +2315,8 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   // radial grid, angular tables
   std::vector<double> rad, wrad;
   dg::radial_grid(h->nrad, rad, wrad);
-  std::vector<double> ylm, ylmT;
+  std::vector<double> ylm;
   dg::ylm_table(h->nang, lebt, h->lmax, ylm);
-  ylmT.resize(ylm.size());
-  for (int j = 0; j < h->nang; ++j)
-    for (int lm = 0; lm < h->nlm; ++lm) ylmT[(size_t)lm * h->nang + j] = ylm[(size_t)j * h->nlm + lm];
   // Poisson inverses, rows 1..N, stored [l][k][i]
   const int n2 = h->nrad + 2;
   std::vector<double> minvT((size_t)(h->lmax + 1) * n2 * h->nrad);
@@ -2317,10 +2416,14 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     for (int k = 0; k < h->natoms; ++k) vbase[k] = (long)k * h->npts_pad;
     vtotal = (long)h->natoms * h->npts_pad;
   }
-  std::vector<int> items = cross_items(h, 1024 * kCrossQ);
+  std::vector<int> items = cross_items(h, 1024 * kCrossQ, &h->npairs);
   h->nitems = (int)items.size() / 4;
-  h->items_cap = std::max<size_t>(bs ? cross_items(h, 1024).size() : items.size(), 4);
-  h->npairs = items.empty() ? 0 : (long)items[items.size() - 1] + items[items.size() - 2] - items[items.size() - 3];
+  h->items_cap = std::max<size_t>(items.size(), 4);
+  if (bs) {  // the batch autotuner may split the items down to 1024 points
+    long cap = 0;
+    h->items_cap = std::max(h->items_cap, cross_items(h, 1024, &cap).size());
+    h->npairs = std::max(h->npairs, cap);
+  }
   // host-side checks and tables are done: from here on the device is used
   HIP_TRY(hipGetDevice(&h->device));
   HIP_TRY(hipStreamCreate(&h->stream));
@@ -2368,7 +2471,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->exc_part, h->nlda_blocks));
   DG_TRY(dalloc(&h->scalars, 4 * (size_t)std::max(1, h->nmol)));
   DG_TRY(dalloc(&h->ylm, ylm.size()));
-  DG_TRY(dalloc(&h->ylmT, ylmT.size()));
   DG_TRY(dalloc(&h->minvT, minvT.size()));
   DG_TRY(dalloc(&h->shc, shc.size()));
   DG_TRY(dalloc(&h->knots, h->nrad));
@@ -2379,7 +2481,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->tab, (size_t)h->natoms * h->nrad * h->nlm * 2));
   DG_TRY(dalloc(&h->blocks, blocks.size()));
   DG_TRY(dalloc(&h->items, h->items_cap));
+  DG_TRY(dalloc(&h->plen, h->items_cap / 4));
   HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
+  HIP_TRY(hipMalloc((void**)&h->pkey, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
   DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit_small * h->ngroups,
                                      (size_t)h->nsplit2d * h->ncb * 4) * 2 * 1024));
   DG_TRY(dalloc(&h->cb, cbs.size()));
@@ -2410,7 +2514,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   HIP_TRY(up(h->pr_cn0, cn0.data(), nprim_tot * 8));
   HIP_TRY(up(h->pr_cn1, cn1.data(), nprim_tot * 8));
   HIP_TRY(up(h->ylm, ylm.data(), ylm.size() * 8));
-  HIP_TRY(up(h->ylmT, ylmT.data(), ylmT.size() * 8));
   HIP_TRY(up(h->minvT, minvT.data(), minvT.size() * 8));
   HIP_TRY(up(h->shc, shc.data(), shc.size() * 8));
   HIP_TRY(up(h->knots, knots.data(), knots.size() * 8));
