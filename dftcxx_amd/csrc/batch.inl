@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_density_batch(const double* __restrict_
       dbl4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s = 0; s < 4 * (b + 1); ++s) d = mfma16x16x4(fr[128 * b * (b + 1) + s * 64 + lane], f[s], d);
-      const double* rb = row + 16 * b + g;
-      const double t = d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
+      // register v of lane (c, g) holds T_b[pc][16b + 4g + v]
+      const double t = d[0] * f[4 * b] + d[1] * f[4 * b + 1] + d[2] * f[4 * b + 2] + d[3] * f[4 * b + 3];
       if (m == mine) acc += t;
     }
   }

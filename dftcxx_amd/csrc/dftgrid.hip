@@ -216,15 +216,19 @@ __global__ __launch_bounds__(64) void k_block_mask(const double* __restrict__ ph
 // symmetry: rho/2 = sum_b phi_b^T T_b with T_b = sum_{b' <= b} P'_{bb'} phi_b',
 // P'_{bb'} = 2 P_{bb'} for b' < b and P_{bb} on the diagonal tile (16-wide tiles).
 // Tile b holds 4(b+1) reduction steps s; entry [s][lane] is
-// P'(16b + lane%16, kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4, stored from
-// offset 128 b (b+1).  The same permutation of the reduction index is used
-// for phi in k_density, so each result lands on the lanes that hold phi.
+// P'(16b + pi(lane%16), kk(s, lane/16)), kk(s,g) = 16(s/4) + 4g + s%4,
+// pi(r) = 4(r%4) + r/4, stored from offset 128 b (b+1).  The same permutation
+// kk of the reduction index is used for phi in k_density, and pi puts
+// T_b[p][16b + 4g + v] in result register v of lane (c = p%16, g): the lane
+// that holds phi[p][16b + 4g + v] as its B operand of step 4b + v, so the
+// final dot product needs no further loads.
+__device__ __forceinline__ int pfrag_row(int r) { return 4 * (r & 3) + (r >> 2); }
 __device__ __forceinline__ double pfrag_value(const double* __restrict__ P, int nbf, long e) {
   int b = 0;
   while (128L * (b + 1) * (b + 2) <= e) ++b;
   const int rem = (int)(e - 128L * b * (b + 1));
   const int s = rem / 64, l = rem % 64;
-  const int j = 16 * b + (l & 15);
+  const int j = 16 * b + pfrag_row(l & 15);
   const int k = 16 * (s / 4) + 4 * (l >> 4) + (s & 3);
   const double f = (s / 4 < b) ? 2.0 : 1.0;
   return (j < nbf && k < nbf) ? f * P[j + (long)k * nbf] : 0.0;
@@ -240,38 +244,75 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // phi[p0+c][kk(s,g)] for every reduction step s in registers (B operand).
 // For output tile b the wave forms T_b[p][16b+4v+g] (result register v) with
 // 4(b+1) MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one
-// tile are staged in LDS and shared by the block's 4 waves.  NT = ceil(nbf/16).
-// With screening, mask (one word per 64-point block, bit T for basis tile T)
-// drops the tiles whose amplitudes are negligible on the whole block; the
-// mask is uniform over the work-group, so every branch on it is too.
+// tile are staged in LDS and shared by the block's 4 waves, double-buffered:
+// the next live tile's fragments are copied from global memory straight into
+// the other buffer (global_load_lds, 1 KB per wave instruction) while the
+// current tile's MFMAs run; one wait and barrier per tile.
+// NT = ceil(nbf/16).  With screening, mask (one word per 64-point block, bit T
+// for basis tile T) drops the tiles whose amplitudes are negligible on the
+// whole block; the mask is uniform over the work-group, so every branch on it is too.
 template <int NT>
 __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
                                                  const double* __restrict__ frag, double* rho,
                                                  const unsigned* __restrict__ mask) {
-  __shared__ double pl[4 * NT * 64];
+  __shared__ double pl[2][4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long p0 = (long)blockIdx.x * 64 + wave * 16;
   const double* row = phi + (p0 + c) * ldphi;
-  const unsigned mk = mask ? mask[blockIdx.x] : 0xffffffffu;
+  const unsigned mk = (mask ? mask[blockIdx.x] : 0xffffffffu) & (NT >= 32 ? 0xffffffffu : ((1u << NT) - 1u));
   double f[4 * NT];
 #pragma unroll
   for (int s = 0; s < 4 * NT; ++s) f[s] = ((mk >> (s >> 2)) & 1u) ? row[16 * (s >> 2) + 4 * g + (s & 3)] : 0.0;
-  double acc = 0.0;
-  for (int b = 0; b < NT; ++b) {
-    if (!((mk >> b) & 1u)) continue;
-    const int ns = 4 * (b + 1);
-    __syncthreads();
+  // tile b's fragments, 256 (b + 1) doubles from offset 128 b (b + 1): chunks of
+  // 128 doubles, chunk q copied by wave q % 4, 16 bytes per lane
+  auto fetch = [&](int b, double* dst) {
     const double* src = frag + 128L * b * (b + 1);
-    for (int e = threadIdx.x; e < ns * 64; e += 256) pl[e] = src[e];
-    __syncthreads();
+    for (int q = wave; q < 2 * (b + 1); q += 4)
+      __builtin_amdgcn_global_load_lds((const void*)(src + q * 128 + lane * 2),
+                                       (__attribute__((address_space(3))) void*)(dst + q * 128), 16, 0, 0);
+  };
+  auto next_live = [&](int b) {  // first tile > b with its mask bit set, NT if none
+    const unsigned rest = (b + 1 >= 32) ? 0u : (mk >> (b + 1));
+    return rest ? b + 1 + __builtin_ctz(rest) : NT;
+  };
+  double acc = 0.0;
+  int b = next_live(-1), buf = 0;
+  if (b < NT) fetch(b, pl[0]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  while (b < NT) {
+    const int nb = next_live(b);
+    // phi[p][16b + 4g + v], v < 4, for the final dot; loaded ahead of the copies
+    // and MFMAs (16-byte aligned: ldphi and the row offset are even)
+    const double2 r01 = *reinterpret_cast<const double2*>(row + 16 * b + 4 * g);
+    const double2 r23 = *reinterpret_cast<const double2*>(row + 16 * b + 4 * g + 2);
+    if (nb < NT) fetch(nb, pl[buf ^ 1]);
+    const double* cur = pl[buf];
     dbl4 d = {0.0, 0.0, 0.0, 0.0};
+    // reduction tiles k <= b, four steps each, one branch per tile; the LDS
+    // operands of tile k + 1 are read before the MFMAs of tile k issue
+    double nxt[4];
 #pragma unroll
-    for (int s = 0; s < 4 * NT; ++s)
-      if (s < ns && ((mk >> (s >> 2)) & 1u)) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
-    // register v of lane (c, g) holds T_b[p0+c][16b + 4v + g]
-    const double* rb = row + 16 * b + g;
-    acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
+    for (int j = 0; j < 4; ++j) nxt[j] = cur[j * 64 + lane];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      double av[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) av[j] = nxt[j];
+      if (k + 1 <= b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nxt[j] = cur[(4 * (k + 1) + j) * 64 + lane];
+      if (k <= b && ((mk >> k) & 1u))
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d = mfma16x16x4(av[j], f[4 * k + j], d);
+    }
+    // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]
+    acc += d[0] * r01.x + d[1] * r01.y + d[2] * r23.x + d[3] * r23.y;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
+    __syncthreads();
+    buf ^= 1;
+    b = nb;
   }
   acc += __shfl_xor(acc, 16);
   acc += __shfl_xor(acc, 32);
@@ -312,7 +353,6 @@ __global__ __launch_bounds__(256) void k_density_pl(const double* __restrict__ p
       for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
     }
     double acc = 0.0;
-    const double* row = phi + (p0 + c) * ldphi;
 #pragma unroll
     for (int b = 0; b < NT; ++b) {
       if (!((mk >> b) & 1u)) continue;
@@ -321,9 +361,8 @@ __global__ __launch_bounds__(256) void k_density_pl(const double* __restrict__ p
 #pragma unroll
       for (int s = 0; s < 4 * (b + 1); ++s)
         if ((mk >> (s >> 2)) & 1u) d = mfma16x16x4(src[s * 64 + lane], cf[s], d);
-      // register v of lane (c, g) holds T_b[p0+c][16b + 4v + g]
-      const double* rb = row + 16 * b + g;
-      acc += d[0] * rb[0] + d[1] * rb[4] + d[2] * rb[8] + d[3] * rb[12];
+      // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]
+      acc += d[0] * cf[4 * b] + d[1] * cf[4 * b + 1] + d[2] * cf[4 * b + 2] + d[3] * cf[4 * b + 3];
     }
     acc += __shfl_xor(acc, 16);
     acc += __shfl_xor(acc, 32);
@@ -1044,91 +1083,127 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // 4 waves are added in LDS in wave order and written as 4 reduction groups
 // (I0 + r, J0, nJ) in the k_contract_reduce layout.  T = float (reduced
 // precision): FP32 amplitudes and u Phi on the FP32 16x16x4 MFMA, whose sums
-// are added into FP64 accumulators every 16 steps (64 points per wave); its
-// register v holds row 4g + v and is stored in the FP64 order.
+// are added, wave by wave, into the work-group's FP64 accumulators in LDS every
+// 16 steps (64 points per wave; no FP64 copy in registers, which keeps four
+// waves per SIMD); its register v holds row 4g + v and is stored in the FP64 order.
+// Work-group order: the hardware deals work-groups round-robin to the kXcds
+// XCDs, each with its own L2.  Work-group L (1-D grid, padded to a multiple of
+// kXcds) takes virtual id v = (L % kXcds) per + L / kXcds, split v / ncb, tile
+// block v % ncb: an XCD runs whole splits, all tile blocks of a split together,
+// so the amplitude rows of a point range come from HBM into one L2 once.
+constexpr int kXcds = 8;
 template <typename T, int NOUT>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 2) void k_contract2d(const T* __restrict__ phi, int ldphi,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(const T* __restrict__ phi, int ldphi,
                                                     const double* __restrict__ u0,
                                                     const double* __restrict__ u1,
                                                     const int* __restrict__ cblocks,
                                                     const int* __restrict__ blist,
-                                                    const int* __restrict__ loff, int ngroups, double* part) {
-  __shared__ double red[8 * 256];
-  const int cbi = blockIdx.x, split = blockIdx.y;
+                                                    const int* __restrict__ loff, int ngroups, int nsplit,
+                                                    double* part) {
+  constexpr bool f32 = sizeof(T) == 4;
+  __shared__ double red[(f32 ? NOUT : 1) * 8 * 256];
+  const int ncb = ngroups / 4, per = gridDim.x / kXcds;
+  const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (v >= ncb * nsplit) return;
+  const int cbi = v % ncb, split = v / ncb;
   const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
-  const int* bl = blist + l0 + cnt * split / gridDim.y;
-  const int nst = 4 * (int)(cnt * (split + 1) / gridDim.y - cnt * split / gridDim.y);
+  const int* bl = blist + l0 + cnt * split / nsplit;
+  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - cnt * split / nsplit);
   auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
   const int ia = 16 * I0 + c, jb = 16 * J0 + c;
-  constexpr bool f32 = sizeof(T) == 4;
   typedef typename std::conditional<f32, flt4, dbl4>::type accT;
   accT acc[NOUT][4][2];
-  dbl4 accd[f32 ? NOUT : 1][4][2];
 #pragma unroll
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int t = 0; t < 2; ++t) acc[o][r][t] = accT{0, 0, 0, 0};
-#pragma unroll
-  for (int o = 0; o < (f32 ? NOUT : 1); ++o)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int t = 0; t < 2; ++t) accd[o][r][t] = dbl4{0.0, 0.0, 0.0, 0.0};
-  T a[4] = {0, 0, 0, 0}, bb[2] = {0, 0}, w0 = 0, w1 = 0;
-  auto load = [&](long p) {
+  if constexpr (f32) {
+    for (int e = threadIdx.x; e < NOUT * 8 * 256; e += 256) red[e] = 0.0;
+    __syncthreads();
+  }
+  // operands of the next PF steps in flight
+  constexpr int PF = 1;
+  T a[PF][4], bb[PF][2], w0[PF], w1[PF];
+  auto load = [&](int k, long p) {
     const T* row = phi + p * ldphi;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[r] = (r < nI) ? row[ia + 16 * r] : (T)0;
+    for (int r = 0; r < 4; ++r) a[k][r] = (r < nI) ? row[ia + 16 * r] : (T)0;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) bb[t] = (t < nJ) ? row[jb + 16 * t] : (T)0;
-    w0 = (T)u0[p];
-    if (NOUT == 2) w1 = (T)u1[p];
+    for (int t = 0; t < 2; ++t) bb[k][t] = (t < nJ) ? row[jb + 16 * t] : (T)0;
+    w0[k] = (T)u0[p];
+    w1[k] = (NOUT == 2) ? (T)u1[p] : (T)0;
   };
-  if (nst > 0) load(point(0));
+#pragma unroll
+  for (int k = 0; k < PF; ++k)
+    if (k < nst) load(k, point(k));
   for (int st = 0; st < nst; ++st) {
     T ca[4], cb[2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) ca[r] = a[r];
+    for (int r = 0; r < 4; ++r) ca[r] = a[0][r];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) cb[t] = bb[t];
-    const T cw0 = w0, cw1 = w1;
-    if (st + 1 < nst) load(point(st + 1));
+    for (int t = 0; t < 2; ++t) cb[t] = bb[0][t];
+    const T cw0 = w0[0], cw1 = w1[0];
+#pragma unroll
+    for (int k = 0; k + 1 < PF; ++k) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[k][r] = a[k + 1][r];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) bb[k][t] = bb[k + 1][t];
+      w0[k] = w0[k + 1];
+      w1[k] = w1[k + 1];
+    }
+    if (st + PF < nst) load(PF - 1, point(st + PF));
+    auto mma = [&](int r, int t, T b0, T b1) {
+      if constexpr (f32) {
+        acc[0][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b0, acc[0][r][t], 0, 0, 0);
+        if (NOUT == 2)
+          acc[NOUT - 1][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b1, acc[NOUT - 1][r][t], 0, 0, 0);
+      } else {
+        acc[0][r][t] = mfma16x16x4(ca[r], b0, acc[0][r][t]);
+        if (NOUT == 2) acc[NOUT - 1][r][t] = mfma16x16x4(ca[r], b1, acc[NOUT - 1][r][t]);
+      }
+    };
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const T b0 = cw0 * cb[t], b1 = cw1 * cb[t];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r < nI && t < nJ && I0 + r <= J0 + t) {
-          if constexpr (f32) {
-            acc[0][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b0, acc[0][r][t], 0, 0, 0);
-            if (NOUT == 2)
-              acc[NOUT - 1][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b1, acc[NOUT - 1][r][t], 0, 0, 0);
-          } else {
-            acc[0][r][t] = mfma16x16x4(ca[r], b0, acc[0][r][t]);
-            if (NOUT == 2) acc[NOUT - 1][r][t] = mfma16x16x4(ca[r], b1, acc[NOUT - 1][r][t]);
+      for (int r = 0; r < 4; ++r)
+        if (r < nI && t < nJ && I0 + r <= J0 + t) mma(r, t, b0, b1);
+    }
+    if constexpr (f32) {
+      if ((st & 15) == 15 || st + 1 == nst) {  // the same steps in every wave
+        for (int wv = 0; wv < 4; ++wv) {
+          __syncthreads();
+          if (wave == wv) {
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+#pragma unroll
+                  for (int v = 0; v < 4; ++v) red[(o * 8 + r * 2 + t) * 256 + g * 64 + c + 16 * v] += (double)acc[o][r][t][v];
+                  acc[o][r][t] = accT{0, 0, 0, 0};
+                }
           }
         }
       }
     }
-    if constexpr (f32) {
-      if ((st & 15) == 15 || st + 1 == nst) {
+  }
+  if constexpr (f32) {
+    __syncthreads();
 #pragma unroll
-        for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-#pragma unroll
-              for (int v = 0; v < 4; ++v) accd[o][r][t][v] += (double)acc[o][r][t][v];
-              acc[o][r][t] = accT{0, 0, 0, 0};
-            }
+    for (int o = 0; o < NOUT; ++o)
+      for (int e = threadIdx.x; e < 8 * 256; e += 256) {
+        const int r = e >> 9, t = (e >> 8) & 1, rem = e & 255;
+        part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[o * 8 * 256 + e];
       }
-    }
+    return;
   }
 #pragma unroll
   for (int o = 0; o < NOUT; ++o) {
@@ -1141,9 +1216,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 2) void k_contract2d(cons
           for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              // FP64 order: register v of lane (c, g) = row 4v + g; FP32: row 4g + v
-              double* q = red + (r * 2 + t) * 256 + (f32 ? g * 64 + c + 16 * v : v * 64 + lane);
-              const double x = f32 ? accd[o][r][t][v] : (double)acc[o][r][t][v];
+              // register v of lane (c, g) = row 4v + g
+              double* q = red + (r * 2 + t) * 256 + v * 64 + lane;
+              const double x = (double)acc[o][r][t][v];
               *q = (wv == 0) ? x : *q + x;
             }
       }
@@ -2131,21 +2206,21 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
     if (small) {
       launch_contract_small(h, u0, u1, nout);
     } else if (mx) {
-      dim3 g2(h->ncb, h->nsplit2d);
+      const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
         hipLaunchKernelGGL((k_contract2d<float, 2>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
+                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
       else
         hipLaunchKernelGGL((k_contract2d<float, 1>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
+                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
     } else {
-      dim3 g2(h->ncb, h->nsplit2d);
+      const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
         hipLaunchKernelGGL((k_contract2d<double, 2>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
+                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
       else
         hipLaunchKernelGGL((k_contract2d<double, 1>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->cpart);
+                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
     }
   }));
   const bool two_d = !small;
@@ -2378,7 +2453,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       for (int r = 0; r < 4; ++r) cbg.insert(cbg.end(), {I0 + r, J0, r < nI ? nJ : 0});
     }
   h->ncb = (int)cbs.size() / 4;
-  h->nsplit2d = (int)std::max(1L, std::min(std::min(256L, 1024L / h->ncb + 1), h->npts_pad / 64));
   h->nsplit_small = (int)std::max(1L, std::min(512L, h->npts_pad / 256));
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
@@ -2433,6 +2507,11 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     const size_t lds = h->cross_lds;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
+    // 2-D contraction: splits per tile block such that all work-groups fit in two
+    // rounds of the resident slots (3 FP64 / 4 FP32 work-groups per CU), the
+    // second round as full as the tile-block count allows
+    const long slots = (long)cus * (h->precision == DG_PREC_MIXED ? 4 : 3);
+    h->nsplit2d = (int)std::max(1L, std::min(std::min(512L, 2 * slots / std::max(h->ncb, 1)), h->npts_pad / 64));
   }
 
   // allocations
