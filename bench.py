@@ -35,10 +35,11 @@ def kernel_work(name, mol_info):
     npts, nbf, nat, nlm = mol_info["npts"], mol_info["nbf"], mol_info["natoms"], mol_info["nlm"]
     if name == "hartree_cross":
         return HARTREE_FLOPS_PER_TERM * npts * (nat - 1) * nlm / mol_info["cross_launches"]
+    # the symmetric products need only the pairs i <= j: one FMA per point and pair
     if name == "contract":
-        return 2 * 2.0 * npts * nbf * nbf  # J and XC, 2 flops per pt*bf^2 each
+        return 2 * 2.0 * npts * nbf * (nbf + 1) / 2  # J and XC
     if name == "density":
-        return 2.0 * npts * nbf * nbf + 2.0 * npts * nbf
+        return 2.0 * npts * nbf * (nbf + 1) / 2 + 2.0 * npts * nbf
     return None
 
 
@@ -290,7 +291,7 @@ def main():
                           "max_abs_sum_grad": float(np.abs(grad.sum(0)).max())}
 
     if not args.no_mixed and prec == G.PREC_FP64 and args.screen == 0.0:
-        # the same build screened in FP64, and with BF16-split Fock contractions (screened and not)
+        # the same build screened in FP64, and with FP32-MFMA Fock contractions (screened and not)
         J6, X6 = bufs[1].cpu().numpy(), bufs[2].cpu().numpy()
         out["variants"] = {}
         for name, pr, sc in (("fp64_screened", G.PREC_FP64, args.variant_screen),

@@ -1078,9 +1078,11 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // ---------------------------------------------------------------------------
 // Contraction for larger bases with 4 x 2 tile blocks: block {I0, nI, J0, nJ}
 // covers the tiles (I0 + r, J0 + t), r < nI, t < nJ, I <= J, of both outputs,
-// so a wave loads 4 + 2 amplitude tiles per step for 16 MFMAs (a row group of
-// k_contract loads 5 for 8).  Points, splits and prefetch as in k_contract; the
-// 4 waves are added in LDS in wave order and written as 4 reduction groups
+// so a wave loads 4 + 2 amplitude tiles per step (4 points) for 16 MFMAs.  The
+// block's 64-point blocks (its list: all, or with screening those where its
+// tiles are significant) are split evenly over nsplit work-groups (split-K);
+// the 4 waves interleave the 16 points of each block, the next step's operands
+// prefetched.  The 4 waves are added in LDS in wave order and written as 4 reduction groups
 // (I0 + r, J0, nJ) in the k_contract_reduce layout.  T = float (reduced
 // precision): FP32 amplitudes and u Phi on the FP32 16x16x4 MFMA, whose sums
 // are added, wave by wave, into the work-group's FP64 accumulators in LDS every
