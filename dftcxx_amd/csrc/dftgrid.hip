@@ -323,7 +323,6 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
 // doubles, formed from P as k_pfrag does) is staged in LDS once per work-group, which then walks the 64-point
 // blocks b = blockIdx.x, blockIdx.x + G, ... with no further barriers; each
 // wave loads the next block's amplitude row while it multiplies the current.
-// The products and their order are those of k_density.
 template <int NT>
 __global__ __launch_bounds__(256) void k_density_pl(const double* __restrict__ phi, int ldphi,
                                                     const double* __restrict__ P, int nbf, long nblk, double* rho,
@@ -1241,8 +1240,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
 // of each block (next step's operands prefetched).  The waves of an output
 // are summed in LDS in wave order and written in the k_contract_reduce layout
 // (group of row I and columns J0 = I + 4k.., tile t = (J - I) % 4, register v
-// holding row 4v + g).
-template <int NT, int NOUT>
+// holding row 4v + g).  EDGE > 0: the last tile holds only EDGE (<= 4)
+// functions; its NT tile pairs (I, NT-1) are summed on the VALU instead of by
+// MFMAs (15/16 of which would multiply padding): lane (c, g) accumulates
+// phi[p][16I + c] u_p phi[p][16(NT-1) + j] with the second factor taken from
+// lane (j, g), and the four point lanes g are added before the LDS stage.
+template <int NT, int NOUT, int EDGE>
 __global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restrict__ phi, int ldphi,
                                                         const double* __restrict__ u0,
                                                         const double* __restrict__ u1, long nblk,
@@ -1261,6 +1264,11 @@ __global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restr
   dbl4 acc[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+  double e[NT][EDGE > 0 ? EDGE : 1];
+#pragma unroll
+  for (int I = 0; I < NT; ++I)
+#pragma unroll
+    for (int j = 0; j < (EDGE > 0 ? EDGE : 1); ++j) e[I][j] = 0.0;
   double f[NT], w = 0.0;
   if (nst > 0) {
     const long p = point(0);
@@ -1286,7 +1294,25 @@ __global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restr
 #pragma unroll
     for (int I = 0; I < NT; ++I)
 #pragma unroll
-      for (int J = I; J < NT; ++J, ++q) acc[q] = mfma16x16x4(cf[I], bw[J], acc[q]);
+      for (int J = I; J < NT; ++J, ++q)
+        if (EDGE == 0 || J < NT - 1) acc[q] = mfma16x16x4(cf[I], bw[J], acc[q]);
+    if constexpr (EDGE > 0) {
+#pragma unroll
+      for (int j = 0; j < EDGE; ++j) {
+        const double wj = __shfl(bw[NT - 1], (lane & 48) + j);
+#pragma unroll
+        for (int I = 0; I < NT; ++I) e[I][j] += cf[I] * wj;
+      }
+    }
+  }
+  if constexpr (EDGE > 0) {
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int j = 0; j < EDGE; ++j) {
+        e[I][j] += __shfl_xor(e[I][j], 16);
+        e[I][j] += __shfl_xor(e[I][j], 32);
+      }
   }
   for (int oo = 0; oo < NOUT; ++oo) {
     for (int s2 = 0; s2 < NW; ++s2) {
@@ -1299,6 +1325,18 @@ __global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restr
             double* r = red + q * 256 + v * 64 + lane;
             *r = (s2 == 0) ? acc[q][v] : *r + acc[q][v];
           }
+        if constexpr (EDGE > 0) {
+          // element (row c, column j) of tile (I, NT-1): register c/4 of lane 16 (c%4) + j
+          if (g == 0) {
+            int qi = 0;
+#pragma unroll
+            for (int I = 0; I < NT; ++I) {
+              qi += (I == 0) ? NT - 1 : NT - I;  // index of the pair (I, NT-1)
+#pragma unroll
+              for (int j = 0; j < EDGE; ++j) red[qi * 256 + (c >> 2) * 64 + (c & 3) * 16 + j] += e[I][j];
+            }
+          }
+        }
       }
     }
     __syncthreads();
@@ -2181,12 +2219,22 @@ int stage_lda(dg_handle* h) {
 template <int N>
 void launch_contract_small_nt(dg_handle* h, const double* u0, const double* u1, int nout) {
   const long nblk = h->npts_pad / 64;
-  if (nout == 2)
-    hipLaunchKernelGGL((k_contract_small<N, 2>), dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi,
-                       u0, u1, nblk, h->ngroups, h->cpart);
-  else
-    hipLaunchKernelGGL((k_contract_small<N, 1>), dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi,
-                       u0, u0, nblk, h->ngroups, h->cpart);
+  auto go = [&](auto kern2, auto kern1) {
+    if (nout == 2)
+      hipLaunchKernelGGL(kern2, dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, nblk,
+                         h->ngroups, h->cpart);
+    else
+      hipLaunchKernelGGL(kern1, dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, nblk,
+                         h->ngroups, h->cpart);
+  };
+  // the VALU edge path when the last of N >= 2 tiles holds at most 4 functions
+  switch (N >= 2 ? h->nbf - 16 * (N - 1) : 0) {
+    case 1: go(k_contract_small<N, 2, 1>, k_contract_small<N, 1, 1>); break;
+    case 2: go(k_contract_small<N, 2, 2>, k_contract_small<N, 1, 2>); break;
+    case 3: go(k_contract_small<N, 2, 3>, k_contract_small<N, 1, 3>); break;
+    case 4: go(k_contract_small<N, 2, 4>, k_contract_small<N, 1, 4>); break;
+    default: go(k_contract_small<N, 2, 0>, k_contract_small<N, 1, 0>); break;
+  }
 }
 
 void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int nout) {
