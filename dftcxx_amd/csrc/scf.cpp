@@ -190,6 +190,8 @@ void one_electron(const MoleculeData& m, std::vector<double>& S, std::vector<dou
 // Symmetric eigen-solver: Householder reduction to tridiagonal form, then the
 // implicit QL iteration with Wilkinson-type shifts (EISPACK tred2/tql2
 // algorithms).  A column-major n x n; eigenvalues ascending, vectors in Z's columns.
+// Every O(n^3) loop runs down contiguous columns (A v as column axpys, since A
+// is symmetric; Z H as u = Z v, then rank-1 column updates).
 // ---------------------------------------------------------------------------
 void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::vector<double>& Z) {
   std::vector<double> A(Ain);
@@ -197,7 +199,7 @@ void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::ve
   Z.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
   auto z = [&](int i, int j) -> double& { return Z[i + (size_t)j * n]; };
-  std::vector<double> v(n), pv(n), qv(n), d(n), e(n, 0.0);
+  std::vector<double> v(n), pv(n), qv(n), uv(n), d(n), e(n, 0.0);
   for (int k = 0; k + 2 < n; ++k) {
     double norm2 = 0.0;
     for (int i = k + 1; i < n; ++i) norm2 += a(i, k) * a(i, k);
@@ -213,10 +215,11 @@ void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::ve
     const double inv = 1.0 / std::sqrt(vn2);
     for (int i = k + 1; i < n; ++i) v[i] *= inv;
     // A <- H A H on the trailing block, H = I - 2 v v^T
-    for (int i = k + 1; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += a(i, j) * v[j];
-      pv[i] = s;
+    for (int i = k + 1; i < n; ++i) pv[i] = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      const double* col = &a(0, j);
+      for (int i = k + 1; i < n; ++i) pv[i] += col[i] * vj;
     }
     double K = 0.0;
     for (int i = k + 1; i < n; ++i) K += v[i] * pv[i];
@@ -225,11 +228,17 @@ void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::ve
       for (int i = k + 1; i < n; ++i) a(i, j) -= 2.0 * (v[i] * qv[j] + qv[i] * v[j]);
     a(k + 1, k) = a(k, k + 1) = alpha;
     for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
-    // Z <- Z H
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += z(i, j) * v[j];
-      for (int j = k + 1; j < n; ++j) z(i, j) -= 2.0 * s * v[j];
+    // Z <- Z H = Z - 2 (Z v) v^T
+    for (int i = 0; i < n; ++i) uv[i] = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      const double* col = &z(0, j);
+      for (int i = 0; i < n; ++i) uv[i] += col[i] * vj;
+    }
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      double* col = &z(0, j);
+      for (int i = 0; i < n; ++i) col[i] -= 2.0 * uv[i] * vj;
     }
   }
   for (int i = 0; i < n; ++i) d[i] = a(i, i);
@@ -290,13 +299,18 @@ void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::ve
   Z.swap(Zs);
 }
 
-void gemm(int n, const double* A, bool ta, const double* B, double* C) {  // C = op(A) B, column-major
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      for (int k = 0; k < n; ++k) s += (ta ? A[k + (size_t)i * n] : A[i + (size_t)k * n]) * B[k + (size_t)j * n];
-      C[i + (size_t)j * n] = s;
+// C = A B, column-major n x n, each column as axpys over contiguous columns of A
+// (every C_ij is summed over k in ascending order)
+void gemm(int n, const double* A, const double* B, double* C) {
+  for (int j = 0; j < n; ++j) {
+    double* c = C + (size_t)j * n;
+    for (int i = 0; i < n; ++i) c[i] = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double b = B[k + (size_t)j * n];
+      const double* a = A + (size_t)k * n;
+      for (int i = 0; i < n; ++i) c[i] += a[i] * b;
     }
+  }
 }
 
 // Host algebra of the reference's DFT class (src/dft.h:41-216): integrals,
@@ -332,6 +346,9 @@ class HostSCF {
       const double f = 1.0 / std::sqrt(s[j]);
       for (int i = 0; i < n_; ++i) X_[i + (size_t)j * n_] = U[i + (size_t)j * n_] * f;
     }
+    XT_.resize(X_.size());
+    for (int j = 0; j < n_; ++j)
+      for (int i = 0; i < n_; ++i) XT_[j + (size_t)i * n_] = X_[i + (size_t)j * n_];
     J_.assign((size_t)n_ * n_, 0.0);
     XC_.assign((size_t)n_ * n_, 0.0);
     P_.assign((size_t)n_ * n_, 0.0);
@@ -343,24 +360,21 @@ class HostSCF {
     const size_t nn = (size_t)n_ * n_;
     std::vector<double> F(nn), tmp(nn), Fp(nn), Cc, C(nn), w;
     for (size_t i = 0; i < nn; ++i) F[i] = H_[i] + 2.0 * J_[i] + XC_[i];
-    gemm(n_, X_.data(), true, F.data(), tmp.data());
-    gemm(n_, tmp.data(), false, X_.data(), Fp.data());
+    gemm(n_, XT_.data(), F.data(), tmp.data());
+    gemm(n_, tmp.data(), X_.data(), Fp.data());
     eigh(n_, Fp, w, Cc);
-    // C = X Cc
-    for (int j = 0; j < n_; ++j)
-      for (int i = 0; i < n_; ++i) {
-        double s = 0.0;
-        for (int k = 0; k < n_; ++k) s += X_[i + (size_t)k * n_] * Cc[k + (size_t)j * n_];
-        C[i + (size_t)j * n_] = s;
-      }
+    gemm(n_, X_.data(), Cc.data(), C.data());  // C = X Cc
+    // P = C_occ C_occ^T, each element summed over the occupied orbitals in order
     std::vector<double> Pn(nn, 0.0);
     const int nocc = nelec_ / 2;
-    for (int j = 0; j < n_; ++j)
-      for (int i = 0; i < n_; ++i) {
-        double s = 0.0;
-        for (int k = 0; k < nocc; ++k) s += C[i + (size_t)k * n_] * C[j + (size_t)k * n_];
-        Pn[i + (size_t)j * n_] = s;
+    for (int k = 0; k < nocc; ++k) {
+      const double* ck = C.data() + (size_t)k * n_;
+      for (int j = 0; j < n_; ++j) {
+        const double cj = ck[j];
+        double* pcol = Pn.data() + (size_t)j * n_;
+        for (int i = 0; i < n_; ++i) pcol[i] += ck[i] * cj;
       }
+    }
     if (first_) {
       P_ = Pn;
       first_ = false;
@@ -419,7 +433,7 @@ class HostSCF {
   double ej_point_ = 0.0;
   int nelec_ = 0;
   bool first_ = true;
-  std::vector<double> S_, H_, X_, J_, XC_;
+  std::vector<double> S_, H_, X_, XT_, J_, XC_;
 };
 
 // The reference's DFT class with the grid work of every iteration on the GPU
@@ -433,12 +447,17 @@ class DFT : public HostSCF {
   }
   ~DFT() {
     if (dP_) (void)hipFree(dP_);
+    if (hbuf_) (void)hipHostFree(hbuf_);
   }
 
   void construct_matrices() {
     HostSCF::construct_matrices();
     if (hipMalloc((void**)&dP_, sizeof(double) * (3 * (size_t)n_ * n_ + 4)) != hipSuccess)
       throw std::runtime_error("hipMalloc failed");
+    // pinned staging for P, J, XC and the scalars (page-locked copies each iteration)
+    if (hipHostMalloc((void**)&hbuf_, sizeof(double) * (3 * (size_t)n_ * n_ + 4), hipHostMallocDefault) !=
+        hipSuccess)
+      throw std::runtime_error("hipHostMalloc failed");
     density_matrix();
     grid_build(false);
     energy();
@@ -474,21 +493,21 @@ class DFT : public HostSCF {
     void* sp = nullptr;
     dg_throw(dg_get_stream(grid_.handle(), &sp));
     hipStream_t st = (hipStream_t)sp;
-    if (hipMemcpyAsync(dP_, P_.data(), nn * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+    double* hP = hbuf_;
+    double* hJ = hP + nn;  // then XC and the scalars, contiguous like dJ, dXC, dS
+    std::memcpy(hP, P_.data(), nn * 8);
+    if (hipMemcpyAsync(dP_, hP, nn * 8, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("copy of P failed");
     dg_throw(dg_build_device(grid_.handle(), dP_, dJ, dXC, dS));
-    double sc[4];
-    std::vector<double> J(nn), XC(nn);
-    if (hipMemcpyAsync(J.data(), dJ, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(XC.data(), dXC, nn * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(sc, dS, 4 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(hJ, dJ, (2 * nn + 4) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       throw std::runtime_error("copy of grid results failed");
-    take_grid(J.data(), XC.data(), sc, with_xc);
+    take_grid(hJ, hJ + nn, hJ + 2 * nn, with_xc);
   }
 
   MolecularGrid grid_;
   double* dP_ = nullptr;
+  double* hbuf_ = nullptr;
 };
 
 // Batched SCF: every molecule runs the reference's loop (src/dft.cpp:84-142)
