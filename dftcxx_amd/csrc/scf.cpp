@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -299,10 +300,190 @@ void eigh(int n, const std::vector<double>& Ain, std::vector<double>& w, std::ve
   Z.swap(Zs);
 }
 
-// C = A B, column-major n x n, each column as axpys over contiguous columns of A
-// (every C_ij is summed over k in ascending order)
-void gemm(int n, const double* A, const double* B, double* C) {
-  for (int j = 0; j < n; ++j) {
+// Lowest nv eigenpairs of a symmetric n x n matrix (column-major): Householder
+// tridiagonalisation T = Q^T A Q with the reflections kept (Q = H_0 ... H_{n-3}),
+// the eigenvalues of T by the implicit QL iteration without vectors (the same
+// iteration as eigh, so the same values), the eigenvectors of T for the nv
+// lowest by inverse iteration on T - lambda I (LU with partial pivoting, three
+// solves, Gram-Schmidt against the earlier vectors of a cluster of eigenvalues
+// closer than 1e-3 |T|_1), and x = Q y.  Z holds the nv vectors as columns.
+void eigh_lowest(int n, int nv, const std::vector<double>& Ain, std::vector<double>& w, std::vector<double>& Z) {
+  std::vector<double> A(Ain);
+  auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
+  const int nh = std::max(0, n - 2);
+  std::vector<double> V((size_t)n * std::max(nh, 1), 0.0);  // reflection k in column k (zero: none)
+  std::vector<double> pv(n), qv(n), d(n), e(n, 0.0);
+  for (int k = 0; k + 2 < n; ++k) {
+    double* v = &V[(size_t)k * n];
+    double norm2 = 0.0;
+    for (int i = k + 1; i < n; ++i) norm2 += a(i, k) * a(i, k);
+    const double xnorm = std::sqrt(norm2);
+    if (xnorm == 0.0) continue;
+    const double alpha = a(k + 1, k) > 0 ? -xnorm : xnorm;
+    for (int i = k + 1; i < n; ++i) v[i] = a(i, k);
+    v[k + 1] -= alpha;
+    double vn2 = 0.0;
+    for (int i = k + 1; i < n; ++i) vn2 += v[i] * v[i];
+    if (vn2 == 0.0) {
+      for (int i = k + 1; i < n; ++i) v[i] = 0.0;
+      continue;
+    }
+    const double inv = 1.0 / std::sqrt(vn2);
+    for (int i = k + 1; i < n; ++i) v[i] *= inv;
+    for (int i = k + 1; i < n; ++i) pv[i] = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      const double* col = &a(0, j);
+      for (int i = k + 1; i < n; ++i) pv[i] += col[i] * vj;
+    }
+    double K = 0.0;
+    for (int i = k + 1; i < n; ++i) K += v[i] * pv[i];
+    for (int i = k + 1; i < n; ++i) qv[i] = pv[i] - K * v[i];
+    for (int j = k + 1; j < n; ++j)
+      for (int i = k + 1; i < n; ++i) a(i, j) -= 2.0 * (v[i] * qv[j] + qv[i] * v[j]);
+    a(k + 1, k) = a(k, k + 1) = alpha;
+    for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+  }
+  for (int i = 0; i < n; ++i) d[i] = a(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+  const std::vector<double> td(d), te(e);  // T for the inverse iteration
+  // eigenvalues: implicit QL without vectors
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0; iter < 200; ++iter) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-16 * dd) break;
+      }
+      if (m == l) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = m - 1;
+      bool underflow = false;
+      for (; i >= l; --i) {
+        const double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+      }
+      if (underflow) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  std::sort(d.begin(), d.end());
+  w.assign(d.begin(), d.begin() + nv);
+  // inverse iteration on T
+  double tnorm = 0.0;  // |T|_1
+  for (int i = 0; i < n; ++i)
+    tnorm = std::max(tnorm, std::fabs(td[i]) + (i > 0 ? std::fabs(te[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(te[i]) : 0.0));
+  if (tnorm == 0.0) tnorm = 1.0;
+  const double eps = 2.220446049250313e-16, ortol = 1e-3 * tnorm, pertol = 10.0 * eps * tnorm;
+  Z.assign((size_t)n * nv, 0.0);
+  std::vector<double> dl(n), du(n), du2(n), dg(n), y(n);
+  std::vector<int> piv(n);
+  int c0 = 0;  // first vector of the current cluster
+  double xprev = 0.0;
+  uint64_t seed = 0x9e3779b97f4a7c15ull;
+  for (int j = 0; j < nv; ++j) {
+    double x = w[j];
+    if (j > 0) {
+      if (x - w[j - 1] > ortol) c0 = j;
+      if (x - xprev < pertol) x = xprev + pertol;  // separate the shifts within a cluster
+    }
+    xprev = x;
+    // LU of T - x I with partial pivoting (rows i, i+1 swapped when |sub| > |diag|)
+    for (int i = 0; i < n; ++i) dg[i] = td[i] - x;
+    for (int i = 0; i + 1 < n; ++i) du[i] = te[i], dl[i] = te[i];
+    for (int i = 0; i + 2 < n; ++i) du2[i] = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+      if (std::fabs(dg[i]) >= std::fabs(dl[i])) {
+        piv[i] = 0;
+        if (dg[i] == 0.0) dg[i] = pertol;
+        const double f = dl[i] / dg[i];
+        dl[i] = f;
+        dg[i + 1] -= f * du[i];
+      } else {
+        piv[i] = 1;
+        const double f = dg[i] / dl[i];
+        dg[i] = dl[i];
+        dl[i] = f;
+        const double t = du[i];
+        du[i] = dg[i + 1];
+        dg[i + 1] = t - f * dg[i + 1];
+        if (i + 2 < n) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+      }
+    }
+    if (dg[n - 1] == 0.0) dg[n - 1] = pertol;
+    // deterministic start vector
+    for (int i = 0; i < n; ++i) {
+      seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+      y[i] = (double)(seed >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    for (int it = 0; it < 3; ++it) {
+      // solve L U y = P b: forward with the row swaps, then back substitution
+      for (int i = 0; i + 1 < n; ++i) {
+        if (piv[i]) {
+          const double t = y[i];
+          y[i] = y[i + 1];
+          y[i + 1] = t - dl[i] * y[i];
+        } else {
+          y[i + 1] -= dl[i] * y[i];
+        }
+      }
+      y[n - 1] /= dg[n - 1];
+      if (n > 1) y[n - 2] = (y[n - 2] - du[n - 2] * y[n - 1]) / dg[n - 2];
+      for (int i = n - 3; i >= 0; --i) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / dg[i];
+      // orthogonalise against the cluster's earlier vectors (still in T's basis: column j of Z)
+      for (int q = c0; q < j; ++q) {
+        const double* zq = &Z[(size_t)q * n];
+        double dot = 0.0;
+        for (int i = 0; i < n; ++i) dot += zq[i] * y[i];
+        for (int i = 0; i < n; ++i) y[i] -= dot * zq[i];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+      nrm = std::sqrt(nrm);
+      for (int i = 0; i < n; ++i) y[i] /= nrm;
+    }
+    std::memcpy(&Z[(size_t)j * n], y.data(), sizeof(double) * n);
+  }
+  // back-transformation x = H_0 (H_1 (... H_{n-3} y)), all vectors at once
+  for (int k = nh - 1; k >= 0; --k) {
+    const double* v = &V[(size_t)k * n];
+    for (int j = 0; j < nv; ++j) {
+      double* z = &Z[(size_t)j * n];
+      double dot = 0.0;
+      for (int i = k + 1; i < n; ++i) dot += v[i] * z[i];
+      const double f = 2.0 * dot;
+      for (int i = k + 1; i < n; ++i) z[i] -= f * v[i];
+    }
+  }
+}
+
+// C = A B, column-major, A n x n, B n x m, each column of C as axpys over
+// contiguous columns of A (every C_ij is summed over k in ascending order)
+void gemm(int n, const double* A, const double* B, double* C, int m = -1) {
+  if (m < 0) m = n;
+  for (int j = 0; j < m; ++j) {
     double* c = C + (size_t)j * n;
     for (int i = 0; i < n; ++i) c[i] = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -362,11 +543,12 @@ class HostSCF {
     for (size_t i = 0; i < nn; ++i) F[i] = H_[i] + 2.0 * J_[i] + XC_[i];
     gemm(n_, XT_.data(), F.data(), tmp.data());
     gemm(n_, tmp.data(), X_.data(), Fp.data());
-    eigh(n_, Fp, w, Cc);
-    gemm(n_, X_.data(), Cc.data(), C.data());  // C = X Cc
+    // only the occupied orbitals enter P
+    const int nocc = nelec_ / 2;
+    eigh_lowest(n_, nocc, Fp, w, Cc);
+    gemm(n_, X_.data(), Cc.data(), C.data(), nocc);  // C_occ = X Cc
     // P = C_occ C_occ^T, each element summed over the occupied orbitals in order
     std::vector<double> Pn(nn, 0.0);
-    const int nocc = nelec_ / 2;
     for (int k = 0; k < nocc; ++k) {
       const double* ck = C.data() + (size_t)k * n_;
       for (int j = 0; j < n_; ++j) {
