@@ -1558,10 +1558,10 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
                                                     const double* __restrict__ pz,
                                                     const double* __restrict__ ab, long nblk,
                                                     double* part, double* mot, int* overflow) {
-  // the point's cell functions above the skip level: atom, P_n, |r - R_n|
-  __shared__ int act_n[kBeckeActive * 64];
+  // the point's cell functions above the skip level: atom and P_n (|r - R_n| is
+  // recomputed where needed: 10 bytes per entry keep ten work-groups per CU)
+  __shared__ unsigned short act_n[kBeckeActive * 64];
   __shared__ double act_P[kBeckeActive * 64];
-  __shared__ double act_d[kBeckeActive * 64];
   const int lane = threadIdx.x;
   const long p = (long)blockIdx.x * 64 + lane;
   double w0a = 0.0;  // weight without the Becke factor, times a_p
@@ -1606,9 +1606,8 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
     if (n == own) Pown = P;
     if (P > skip) {
       if (nact < kBeckeActive) {
-        act_n[nact * 64 + lane] = n;
+        act_n[nact * 64 + lane] = (unsigned short)n;
         act_P[nact * 64 + lane] = P;
-        act_d[nact * 64 + lane] = dn;
         ++nact;
       } else {
         *overflow = 1;
@@ -1647,7 +1646,7 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
         const int n = act_n[t * 64 + lane];
         if (n == B) continue;
         const double iR = invR[n * natoms + B];
-        const double mu = (act_d[t * 64 + lane] - dB) * iR;
+        const double mu = (atom_dist(atoms, n, x, y, z) - dB) * iR;
         double sv, rt;
         cutoff_and_ratio(mu, sv, rt);
         const double e[3] = {(atoms[4 * n] - atoms[4 * B]) * iR, (atoms[4 * n + 1] - atoms[4 * B + 1]) * iR,
@@ -2906,6 +2905,8 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
     return fail(DG_ERR_STATE, "dg_xc_adjoint: needs a corrected density and dg_xc (or a build) first");
   const long np = h->npts_pad, nblk = np / 64;
   const int na = h->natoms;
+  if (grad && na > 65535)
+    return fail(DG_ERR_UNSUPPORTED, "dg_xc_adjoint: forces for at most 65535 atoms");
   if (!h->adj_ub) {
     for (double** b : {&h->adj_ub, &h->adj_ab, &h->adj_udp}) DG_TRY(dalloc(b, np));
     DG_TRY(dalloc(&h->adj_lam, np / 256 + 1));
