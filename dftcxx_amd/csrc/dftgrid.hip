@@ -2107,7 +2107,7 @@ int stage_correct(dg_handle* h) {
   return DG_OK;
 }
 
-// points per thread of the cross-potential kernel (items hold up to 1024 kCrossQ points)
+// points per thread of the cross-potential kernel (a work-group pass covers 1024 kCrossQ slots)
 constexpr int kCrossQ = 2;
 
 template <int L>
@@ -2523,7 +2523,8 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     }
     h->cross_lds = maxb;
   }
-  // cross-potential work list: point ranges of <= 1024 outside each cell
+  // cross-potential work list: point ranges of at most 4096 (two work-group passes)
+  // outside each cell; fewer, longer items spend less of the kernel between items
   std::vector<long> vbase(h->natoms);
   long vtotal = 0;
   if (bs) {
@@ -2539,7 +2540,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     for (int k = 0; k < h->natoms; ++k) vbase[k] = (long)k * h->npts_pad;
     vtotal = (long)h->natoms * h->npts_pad;
   }
-  std::vector<int> items = cross_items(h, 1024 * kCrossQ, &h->npairs);
+  std::vector<int> items = cross_items(h, 2048 * kCrossQ, &h->npairs);
   h->nitems = (int)items.size() / 4;
   h->items_cap = std::max<size_t>(items.size(), 4);
   if (bs) {  // the batch autotuner may split the items down to 1024 points
