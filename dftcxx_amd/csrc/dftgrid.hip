@@ -2250,7 +2250,9 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
-  const bool small = !mx && h->screen == 0.0 && h->ntb <= 5;
+  // up to 80 functions the dense all-tiles kernel reads each amplitude once and
+  // is faster than skipping tiles, so it also serves screened handles
+  const bool small = !mx && h->ntb <= 5;
   DG_TRY(run(h, "contract", [&] {
     if (small) {
       launch_contract_small(h, u0, u1, nout);
