@@ -1356,6 +1356,130 @@ __global__ __launch_bounds__(256, 2) void k_contract_small(const double* __restr
   }
 }
 
+// k_contract_small in reduced precision: FP32 amplitudes and u Phi on the FP32
+// 16x16x4 MFMA (register v holds row 4g + v), whose sums are added every 16
+// steps (64 points per wave), wave by wave in a fixed order, into the
+// work-group's FP64 accumulators in LDS (one NP-tile set per output); a sparse
+// last tile (EDGE) is summed in FP64 on the VALU as in k_contract_small.
+template <int NT, int NOUT, int EDGE>
+__global__ __launch_bounds__(256, 2) void k_contract_small_f32(const float* __restrict__ phi, int ldphi,
+                                                            const double* __restrict__ u0,
+                                                            const double* __restrict__ u1, long nblk,
+                                                            int ngroups, double* part) {
+  constexpr int NP = NT * (NT + 1) / 2;
+  constexpr int NW = 4 / NOUT;  // waves per output
+  __shared__ double red[NOUT * NP * 256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int o = wave / NW, sw = wave % NW;
+  const double* u = (o == 0) ? u0 : u1;
+  const long b0 = nblk * blockIdx.x / gridDim.x, b1 = nblk * (blockIdx.x + 1) / gridDim.x;
+  constexpr int SPB = 16 / NW;  // this wave's steps per block
+  const int nst = SPB * (int)(b1 - b0);
+  auto point = [&](int t) { return 64L * (b0 + t / SPB) + 4 * (sw + NW * (t % SPB)) + g; };
+  for (int i = threadIdx.x; i < NOUT * NP * 256; i += 256) red[i] = 0.0;
+  __syncthreads();
+  flt4 acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+  double e[NT][EDGE > 0 ? EDGE : 1];
+#pragma unroll
+  for (int I = 0; I < NT; ++I)
+#pragma unroll
+    for (int j = 0; j < (EDGE > 0 ? EDGE : 1); ++j) e[I][j] = 0.0;
+  double* my = red + o * NP * 256;
+  auto flush = [&]() {  // same steps in every wave
+    for (int s2 = 0; s2 < NW; ++s2) {
+      __syncthreads();
+      if (sw == s2) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            my[q * 256 + g * 64 + v * 16 + c] += (double)acc[q][v];
+            acc[q][v] = 0.0f;
+          }
+      }
+    }
+  };
+  float f[NT];
+  double w = 0.0;
+  if (nst > 0) {
+    const long p = point(0);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) f[T] = phi[p * ldphi + 16 * T + c];
+    w = u[p];
+  }
+  for (int st = 0; st < nst; ++st) {
+    float cf[NT];
+#pragma unroll
+    for (int T = 0; T < NT; ++T) cf[T] = f[T];
+    const double cw = w;
+    if (st + 1 < nst) {
+      const long p = point(st + 1);
+#pragma unroll
+      for (int T = 0; T < NT; ++T) f[T] = phi[p * ldphi + 16 * T + c];
+      w = u[p];
+    }
+    float bw[NT];
+#pragma unroll
+    for (int T = 0; T < NT; ++T) bw[T] = (float)cw * cf[T];
+    int q = 0;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = I; J < NT; ++J, ++q)
+        if (EDGE == 0 || J < NT - 1) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(cf[I], bw[J], acc[q], 0, 0, 0);
+    if constexpr (EDGE > 0) {
+#pragma unroll
+      for (int j = 0; j < EDGE; ++j) {
+        const double wj = __shfl((double)bw[NT - 1], (lane & 48) + j);
+#pragma unroll
+        for (int I = 0; I < NT; ++I) e[I][j] += (double)cf[I] * wj;
+      }
+    }
+    if ((st & 15) == 15) flush();
+  }
+  if (nst & 15) flush();
+  if constexpr (EDGE > 0) {
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int j = 0; j < EDGE; ++j) {
+        e[I][j] += __shfl_xor(e[I][j], 16);
+        e[I][j] += __shfl_xor(e[I][j], 32);
+      }
+    for (int s2 = 0; s2 < NW; ++s2) {
+      __syncthreads();
+      if (sw == s2 && g == 0) {
+        int qi = 0;
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+          qi += (I == 0) ? NT - 1 : NT - I;  // index of the pair (I, NT-1)
+#pragma unroll
+          for (int j = 0; j < EDGE; ++j) my[qi * 256 + (c >> 2) * 64 + (c & 3) * 16 + j] += e[I][j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int oo = 0; oo < NOUT; ++oo)
+    for (int ei = threadIdx.x; ei < NP * 256; ei += 256) {
+      const int q = ei >> 8, rem = ei & 255;
+      int I = 0, qq = q;
+      while (qq >= NT - I) {
+        qq -= NT - I;
+        ++I;
+      }
+      const int J = I + qq;
+      int grp = 0;
+      for (int I2 = 0; I2 < I; ++I2) grp += (NT - I2 + 3) / 4;
+      grp += (J - I) / 4;
+      part[(((long)blockIdx.x * ngroups + grp) * NOUT + oo) * 1024 + ((J - I) & 3) * 256 + rem] =
+          red[oo * NP * 256 + ei];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint of E_xc (no counterpart in the reference)
 // ---------------------------------------------------------------------------
@@ -2236,7 +2360,38 @@ void launch_contract_small_nt(dg_handle* h, const double* u0, const double* u1, 
   }
 }
 
+template <int N>
+void launch_contract_small_f32_nt(dg_handle* h, const double* u0, const double* u1, int nout) {
+  const long nblk = h->npts_pad / 64;
+  auto go = [&](auto kern2, auto kern1) {
+    if (nout == 2)
+      hipLaunchKernelGGL(kern2, dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, nblk,
+                         h->ngroups, h->cpart);
+    else
+      hipLaunchKernelGGL(kern1, dim3(h->nsplit_small), dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, nblk,
+                         h->ngroups, h->cpart);
+  };
+  switch (N >= 2 ? h->nbf - 16 * (N - 1) : 0) {
+    case 1: go(k_contract_small_f32<N, 2, 1>, k_contract_small_f32<N, 1, 1>); break;
+    case 2: go(k_contract_small_f32<N, 2, 2>, k_contract_small_f32<N, 1, 2>); break;
+    case 3: go(k_contract_small_f32<N, 2, 3>, k_contract_small_f32<N, 1, 3>); break;
+    case 4: go(k_contract_small_f32<N, 2, 4>, k_contract_small_f32<N, 1, 4>); break;
+    default: go(k_contract_small_f32<N, 2, 0>, k_contract_small_f32<N, 1, 0>); break;
+  }
+}
+
 void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int nout) {
+  if (h->precision == DG_PREC_MIXED) {
+    switch (h->ntb) {
+      case 1: launch_contract_small_f32_nt<1>(h, u0, u1, nout); break;
+      case 2: launch_contract_small_f32_nt<2>(h, u0, u1, nout); break;
+      case 3: launch_contract_small_f32_nt<3>(h, u0, u1, nout); break;
+      case 4: launch_contract_small_f32_nt<4>(h, u0, u1, nout); break;
+      case 5: launch_contract_small_f32_nt<5>(h, u0, u1, nout); break;
+      default: break;
+    }
+    return;
+  }
   switch (h->ntb) {
     case 1: launch_contract_small_nt<1>(h, u0, u1, nout); break;
     case 2: launch_contract_small_nt<2>(h, u0, u1, nout); break;
@@ -2250,9 +2405,10 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
-  // up to 80 functions the dense all-tiles kernel reads each amplitude once and
-  // is faster than skipping tiles, so it also serves screened handles
-  const bool small = !mx && h->ntb <= 5;
+  // up to 80 functions the dense all-tiles kernel (FP64, or FP32 in mixed mode)
+  // reads each amplitude once and is faster than skipping tiles, so it also
+  // serves screened handles
+  const bool small = h->ntb <= 5;
   DG_TRY(run(h, "contract", [&] {
     if (small) {
       launch_contract_small(h, u0, u1, nout);
