@@ -511,12 +511,27 @@ __global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho,
 // scalars[0] = sum of exc_part (fixed tree order); scalars[2] = sum q;
 // scalars[3] = E_J = 2 tr(P J) evaluated pointwise: J = (1/2) Phi^T diag(w V) Phi
 // and rho = c 2 phi^T P phi give 2 tr(P J) = sum w V rho / (2c), c = sum Z / q_raw.
-__global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_part, long nparts,
-                                                 const double* __restrict__ ej_part, long nej,
-                                                 const double* __restrict__ q, int natoms, double zsum,
-                                                 double* scalars) {
+// The whole work-group (256 threads) evaluates them; out2 (if set) receives a copy
+// of all four.
+struct ScalarArgs {
+  const double* exc_part;
+  long nparts;
+  const double* ej_part;
+  long nej;
+  const double* q;
+  int natoms;
+  double zsum;
+  double* scalars;
+  double* out2;
+};
+
+__device__ void scalars_block(const ScalarArgs& a) {
   __shared__ double red[256];
   __shared__ double red2[256];
+  const double* __restrict__ exc_part = a.exc_part;
+  const double* __restrict__ ej_part = a.ej_part;
+  const long nparts = a.nparts, nej = a.nej;
+  double* scalars = a.scalars;
   double s = 0.0, s2 = 0.0;
   for (long i = threadIdx.x; i < nparts; i += 256) s += exc_part[i];
   if (ej_part)
@@ -533,14 +548,18 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
   }
   if (threadIdx.x == 0) {
     scalars[0] = red[0];
-    if (q) {
+    if (a.q) {
       double qs = 0.0;
-      for (int a = 0; a < natoms; ++a) qs += q[a];
+      for (int at = 0; at < a.natoms; ++at) qs += a.q[at];
       scalars[2] = qs;
     }
-    scalars[3] = ej_part ? red2[0] * scalars[1] / (2.0 * zsum) : 0.0;
+    scalars[3] = ej_part ? red2[0] * scalars[1] / (2.0 * a.zsum) : 0.0;
+    if (a.out2)
+      for (int i = 0; i < 4; ++i) a.out2[i] = scalars[i];
   }
 }
+
+__global__ __launch_bounds__(256) void k_scalars(ScalarArgs a) { scalars_block(a); }
 
 // ---------------------------------------------------------------------------
 // Hartree potential (src/atomicgrid.cpp:241-466, src/moleculargrid.cpp:336-389)
@@ -549,9 +568,11 @@ __global__ __launch_bounds__(256) void k_scalars(const double* __restrict__ exc_
 // The three small per-atom products of the Hartree stage run as FP64 MFMA
 // tiles (lane (c, g) feeds A[c][g] and B[g][c]; result register v holds row
 // 4v + g, column c), summing the inner index in steps of 4 with zero padding.
-// Each wave loads the operands of kSteps steps before it issues their MFMAs,
-// so that one memory latency covers kSteps steps.
+// Each wave loads the operands of kSteps steps (kSteps2 in the one-product
+// kernels, whose registers allow more) before it issues their MFMAs, so that
+// one memory latency covers them.
 constexpr int kSteps = 4;
+constexpr int kSteps2 = 8;
 
 // rho_lm[a][i][lm] = 4 pi sum_j F[i][j] Y[j][lm] with F[i][j] = rho_p w_j wb_p,
 // p = (a nrad + i) nang + j (src/atomicgrid.cpp:241-296).  Block (a, 16 shells
@@ -634,10 +655,10 @@ __global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_l
   const int ca = col / w, clm = l * l + col % w;
   const double* M = (which ? KD : KU) + (long)l * n2 * nrad;
   dbl4 d = {0.0, 0.0, 0.0, 0.0};
-  for (int kb = 0; kb < n2; kb += 4 * kSteps) {
-    double av[kSteps], bv[kSteps];
+  for (int kb = 0; kb < n2; kb += 4 * kSteps2) {
+    double av[kSteps2], bv[kSteps2];
 #pragma unroll
-    for (int u = 0; u < kSteps; ++u) {
+    for (int u = 0; u < kSteps2; ++u) {
       const int k = kb + 4 * u + g;
       av[u] = bv[u] = 0.0;
       if (k < n2) {
@@ -651,7 +672,7 @@ __global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_l
       }
     }
 #pragma unroll
-    for (int u = 0; u < kSteps; ++u) d = mfma16x16x4(av[u], bv[u], d);
+    for (int u = 0; u < kSteps2; ++u) d = mfma16x16x4(av[u], bv[u], d);
   }
   if (col >= ncol) return;
   const int m = clm - l * l - l;
@@ -683,16 +704,16 @@ __global__ __launch_bounds__(64) void k_vcell(const double* __restrict__ U, cons
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int i = i0 + c, j = j0 + c;
   dbl4 d = {0.0, 0.0, 0.0, 0.0};
-  for (int kb = 0; kb < nlm; kb += 4 * kSteps) {
-    double av[kSteps], bv[kSteps];
+  for (int kb = 0; kb < nlm; kb += 4 * kSteps2) {
+    double av[kSteps2], bv[kSteps2];
 #pragma unroll
-    for (int u = 0; u < kSteps; ++u) {
+    for (int u = 0; u < kSteps2; ++u) {
       const int k = kb + 4 * u + g;
       av[u] = (k < nlm && i < nrad) ? U[((long)a * nrad + i) * nlm + k] : 0.0;
       bv[u] = (k < nlm && j < nang) ? ylm[(long)j * nlm + k] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSteps; ++u) d = mfma16x16x4(av[u], bv[u], d);
+    for (int u = 0; u < kSteps2; ++u) d = mfma16x16x4(av[u], bv[u], d);
   }
   if (j >= nang) return;
 #pragma unroll
@@ -1858,10 +1879,16 @@ __device__ __forceinline__ flt4 mfma16x16x4f32(float a, float b, flt4 c) {
 // partial sums are added in order, so the result does not depend on timing.
 // Tiles on the diagonal contribute their upper triangle only and every element
 // is mirrored, so the output is exactly symmetric.
+// With sa.scalars set, the last work-group evaluates the build's scalars
+// instead (k_scalars, which depends on nothing this kernel writes).
 __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restrict__ part, int nsplit,
                                                          int ngroups, int nout,
                                                          const int* __restrict__ groups, int nbf,
-                                                         double* out0, double* out1) {
+                                                         double* out0, double* out1, ScalarArgs sa) {
+  if (sa.scalars && blockIdx.x == gridDim.x - 1) {
+    scalars_block(sa);
+    return;
+  }
   __shared__ double red[16][17];
   const int el = threadIdx.x & 15, r = threadIdx.x >> 4;
   const long e = (long)blockIdx.x * 16 + el;
@@ -2402,7 +2429,9 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
   }
 }
 
-int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1) {
+// sa (optional): the build's scalars, evaluated by one more work-group of the reduction
+int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1,
+                   const ScalarArgs* sa = nullptr) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
   // up to 80 functions the dense all-tiles kernel (FP64, or FP32 in mixed mode)
@@ -2432,17 +2461,32 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
   }));
   const bool two_d = !small;
   const int ng = two_d ? 4 * h->ncb : h->ngroups;
+  ScalarArgs none{};
+  const unsigned nb = nblocks((long)ng * nout * 1024, 16) + (sa ? 1 : 0);
   return run(h, "contract_reduce", [&] {
-    hipLaunchKernelGGL(k_contract_reduce, dim3(nblocks((long)ng * nout * 1024, 16)), dim3(256), 0, h->stream,
-                       h->cpart, small ? h->nsplit_small : h->nsplit2d, ng, nout,
-                       two_d ? h->cb_groups : h->blocks, h->nbf, out0, out1);
+    hipLaunchKernelGGL(k_contract_reduce, dim3(nb), dim3(256), 0, h->stream, h->cpart,
+                       small ? h->nsplit_small : h->nsplit2d, ng, nout, two_d ? h->cb_groups : h->blocks, h->nbf,
+                       out0, out1, sa ? *sa : none);
   });
+}
+
+ScalarArgs scalar_args(dg_handle* h, bool with_ej, double* out2) {
+  ScalarArgs a;
+  a.exc_part = h->exc_part;
+  a.nparts = h->nlda_blocks;
+  a.ej_part = with_ej ? h->ej_part : nullptr;
+  a.nej = h->npts_pad / 256;
+  a.q = h->qc;
+  a.natoms = h->natoms;
+  a.zsum = h->zsum;
+  a.scalars = h->scalars;
+  a.out2 = out2;
+  return a;
 }
 
 int stage_scalars(dg_handle* h, bool with_ej) {
   return run(h, "scalars", [&] {
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, h->exc_part, h->nlda_blocks,
-                       with_ej ? h->ej_part : nullptr, h->npts_pad / 256, h->qc, h->natoms, h->zsum, h->scalars);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, h->stream, scalar_args(h, with_ej, nullptr));
   });
 }
 
@@ -2995,10 +3039,8 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
   DG_TRY(stage_correct(h));
   DG_TRY(stage_lda(h));
   DG_TRY(stage_hartree(h));
-  DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J));
-  DG_TRY(stage_scalars(h, true));
-  if (dscalars)
-    HIP_TRY(hipMemcpyAsync(dscalars, h->scalars, 4 * 8, hipMemcpyDeviceToDevice, h->stream));
+  const ScalarArgs sa = scalar_args(h, true, dscalars);
+  DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J, &sa));
   h->has_hartree = h->has_xc = true;
   return DG_OK;
 }
