@@ -471,13 +471,49 @@ __device__ __forceinline__ void vwn_para(double x, const VwnConst& k, double& ep
                 k.k2 * (2.0 / (x - k.x0) - tx / X - 4.0 * (2.0 * k.x0 + k.b) / qq));
 }
 
-__global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho,
-                                             const double* __restrict__ w, long n, VwnConst vk,
-                                             double xfac, double* uxc, double* exd,
-                                             double* exc_part) {
+// SCALE (the device build, fused with the renormalisation): rho is scaled
+// here, by f = sum Z / sum q computed in every block exactly as k_scale_parts
+// (SCALE = 1, populations split S ways in part) or k_scale (SCALE = 2, S = 1,
+// per-atom q in part) compute it, and written back; block 0 also writes q, qc
+// and scalars[1] as those kernels do.
+template <int SCALE>
+__global__ __launch_bounds__(256) void k_lda(double* rho, const double* __restrict__ w, long n, VwnConst vk,
+                                             double xfac, double* uxc, double* exd, double* exc_part,
+                                             const double* __restrict__ part, int S, int natoms, double zsum,
+                                             double* q, double* qc, double* scalars) {
   __shared__ double red[256];
+  __shared__ double qsum;
+  double f = 1.0;
+  if constexpr (SCALE == 1) {
+    if (threadIdx.x < 64) {
+      double t = 0.0;
+      for (int e = threadIdx.x; e < natoms * S; e += 64) t += part[e];
+      t = wave_sum(t);
+      if (threadIdx.x == 0) qsum = t;
+    }
+    __syncthreads();
+    f = zsum / qsum;
+    if (blockIdx.x == 0) {
+      if (threadIdx.x == 0) scalars[1] = qsum;
+      for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
+        double qa = 0.0;
+        for (int sp = 0; sp < S; ++sp) qa += part[a * S + sp];
+        q[a] = qa;
+        qc[a] = f * qa;
+      }
+    }
+  } else if constexpr (SCALE == 2) {
+    double qs = 0.0;
+    for (int a = 0; a < natoms; ++a) qs += part[a];
+    f = zsum / qs;
+    if (blockIdx.x == 0) {
+      if (threadIdx.x == 0) scalars[1] = qs;
+      for (int a = threadIdx.x; a < natoms; a += blockDim.x) qc[a] = f * part[a];
+    }
+  }
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   double e = 0.0;
+  if (SCALE > 0 && p < n) rho[p] *= f;
   if (p < n) {
     const double r = rho[p];
     const double ra = r * 0.5;
@@ -2361,8 +2397,37 @@ int stage_hartree(dg_handle* h) {
 
 int stage_lda(dg_handle* h) {
   return run(h, "lda", [&] {
-    hipLaunchKernelGGL(k_lda, dim3((unsigned)h->nlda_blocks), dim3(256), 0, h->stream, h->rho, h->w,
-                       h->npts_pad, h->vwn, h->xfac, h->uxc, h->exd, h->exc_part);
+    hipLaunchKernelGGL(k_lda<0>, dim3((unsigned)h->nlda_blocks), dim3(256), 0, h->stream, h->rho, h->w,
+                       h->npts_pad, h->vwn, h->xfac, h->uxc, h->exd, h->exc_part, nullptr, 1, h->natoms, 0.0,
+                       nullptr, nullptr, nullptr);
+  });
+}
+
+// The renormalisation and the functional in two launches: the cell sums, then
+// k_lda scaling rho as it goes (the device build; every scalar and every value of
+// rho, q and qc is the one stage_correct + stage_lda give).
+int stage_correct_lda(dg_handle* h) {
+  const int S = h->cell_split;
+  if (S > 1) {
+    DG_TRY(run(h, "cell_sum", [&] {
+      hipLaunchKernelGGL(k_cell_sum_split, dim3(h->natoms * S), dim3(256), 0, h->stream, h->rho, h->w, h->cell, S,
+                         h->qpart);
+    }));
+  } else {
+    DG_TRY(run(h, "cell_sum", [&] {
+      hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
+    }));
+  }
+  h->corrected = true;
+  return run(h, "lda", [&] {
+    if (S > 1)
+      hipLaunchKernelGGL(k_lda<1>, dim3((unsigned)h->nlda_blocks), dim3(256), 0, h->stream, h->rho, h->w,
+                         h->npts_pad, h->vwn, h->xfac, h->uxc, h->exd, h->exc_part, h->qpart, S, h->natoms,
+                         h->zsum, h->q, h->qc, h->scalars);
+    else
+      hipLaunchKernelGGL(k_lda<2>, dim3((unsigned)h->nlda_blocks), dim3(256), 0, h->stream, h->rho, h->w,
+                         h->npts_pad, h->vwn, h->xfac, h->uxc, h->exd, h->exc_part, h->q, 1, h->natoms, h->zsum,
+                         nullptr, h->qc, h->scalars);
   });
 }
 
@@ -3036,8 +3101,7 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
   if (!h || !dP) return fail(DG_ERR_INVALID, "dg_build_device: null argument");
   if (h->nmol) return fail(DG_ERR_STATE, "dg_build_device: not for a batch handle (use dg_batch_build)");
   DG_TRY(stage_density(h, dP));
-  DG_TRY(stage_correct(h));
-  DG_TRY(stage_lda(h));
+  DG_TRY(stage_correct_lda(h));
   DG_TRY(stage_hartree(h));
   const ScalarArgs sa = scalar_args(h, true, dscalars);
   DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J, &sa));
