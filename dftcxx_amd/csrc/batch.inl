@@ -88,25 +88,32 @@ __global__ void k_pfrag_batch(const double* __restrict__ P, const long* __restri
 // products of its own molecule.
 __global__ __launch_bounds__(256) void k_density_batch(const double* __restrict__ phi,
                                                        const double* __restrict__ frag,
-                                                       const int* __restrict__ atom_mol, long cell,
+                                                       const int* __restrict__ atom_mol,
+                                                       const int* __restrict__ mol_nbf, long cell,
                                                        long npts, double* rho) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long p0 = (long)blockIdx.x * 64 + wave * 16;
   const long pc = p0 + c;
   const double* row = phi + pc * 32;
-  double f[8];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
   const int mine = pc < npts ? atom_mol[pc / cell] : -1;
   const int mfirst = p0 < npts ? atom_mol[p0 / cell] : -1;
   const long plast = (p0 + 15 < npts) ? p0 + 15 : npts - 1;
   const int mlast = p0 < npts ? atom_mol[plast / cell] : -1;
+  // the second 16-function tile only for molecules of more than 16 functions
+  // (its amplitudes are zero otherwise, and so are its products)
+  bool two = false;
+  for (int m = mfirst; m >= 0 && m <= mlast; ++m) two = two || mol_nbf[m] > 16;
+  double f[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) f[s] = (s < 4 || two) ? row[16 * (s >> 2) + 4 * g + (s & 3)] : 0.0;
   double acc = 0.0;
   for (int m = mfirst; m >= 0 && m <= mlast; ++m) {
     const double* fr = frag + (long)m * kBatchFrag;
+    const int nb = mol_nbf[m] > 16 ? 2 : 1;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
+      if (b >= nb) continue;
       dbl4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s = 0; s < 4 * (b + 1); ++s) d = mfma16x16x4(fr[128 * b * (b + 1) + s * 64 + lane], f[s], d);
@@ -146,8 +153,10 @@ __global__ void k_scale_batch(double* rho, long npts, long cell, const int* __re
 // p_end} of molecule m's points; its 4 waves interleave 4-point steps.  Lane
 // (c, g) feeds A = phi[p_g][16I + c] and B = u phi[p_g][16J + c] for the tiles
 // t = (I, J) = (0,0), (0,1), (1,1) (the last two only when nbf_m > 16) and
-// receives C[16I + 4v + g][16J + c] in result register v.  The 4 waves are
-// summed in LDS in a fixed order: part[chunk][o][t][v][lane].
+// receives C[16I + 4v + g][16J + c] in result register v.  The second tile of
+// amplitudes is read only when nbf_m > 16, and the operands of the next two
+// steps are in flight while a step's MFMAs run.  The 4 waves are summed in LDS
+// in a fixed order: part[chunk][o][t][v][lane].
 template <int NOUT>
 __global__ __launch_bounds__(256) void k_contract_batch(const double* __restrict__ phi,
                                                         const double* __restrict__ u0,
@@ -165,17 +174,35 @@ __global__ __launch_bounds__(256) void k_contract_batch(const double* __restrict
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
     for (int t = 0; t < 3; ++t) acc[o][t] = dbl4{0.0, 0.0, 0.0, 0.0};
-  for (long p0 = pb + 4 * wave; p0 < pe; p0 += 16) {
+  constexpr int PF = 3;  // the current step and the next two
+  double ra0[PF], ra1[PF], rw[PF][NOUT];
+  auto load = [&](int k, long p0) {
     const long p = p0 + g;
-    double a0 = 0.0, a1 = 0.0, w[NOUT];
+    ra0[k] = ra1[k] = 0.0;
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o) w[o] = 0.0;
+    for (int o = 0; o < NOUT; ++o) rw[k][o] = 0.0;
     if (p < pe) {
-      a0 = phi[p * 32 + c];
-      a1 = phi[p * 32 + 16 + c];
-      w[0] = u0[p];
-      if (NOUT > 1) w[NOUT - 1] = u1[p];
+      ra0[k] = phi[p * 32 + c];
+      if (two) ra1[k] = phi[p * 32 + 16 + c];
+      rw[k][0] = u0[p];
+      if (NOUT > 1) rw[k][NOUT - 1] = u1[p];
     }
+  };
+#pragma unroll
+  for (int k = 0; k < PF; ++k) load(k, pb + 4 * wave + 16 * k);
+  for (long p0 = pb + 4 * wave; p0 < pe; p0 += 16) {
+    const double a0 = ra0[0], a1 = ra1[0];
+    double w[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) w[o] = rw[0][o];
+#pragma unroll
+    for (int k = 0; k + 1 < PF; ++k) {
+      ra0[k] = ra0[k + 1];
+      ra1[k] = ra1[k + 1];
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o) rw[k][o] = rw[k + 1][o];
+    }
+    load(PF - 1, p0 + 16 * PF);
 #pragma unroll
     for (int o = 0; o < NOUT; ++o) {
       acc[o][0] = mfma16x16x4(a0, w[o] * a0, acc[o][0]);
@@ -425,7 +452,7 @@ int batch_build(dg_handle* h, const double* dP, double* dJ, double* dXC, double*
   }));
   DG_TRY(run(h, "density", [&] {
     hipLaunchKernelGGL(k_density_batch, dim3(nblocks(np, 64)), dim3(256), 0, h->stream, h->phi, h->pfrag,
-                       h->atom_mol, h->cell, h->npts, h->rho);
+                       h->atom_mol, h->mol_nbf, h->cell, h->npts, h->rho);
   }));
   DG_TRY(run(h, "cell_sum", [&] {
     hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
