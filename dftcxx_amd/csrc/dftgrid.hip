@@ -672,13 +672,17 @@ __global__ __launch_bounds__(64 * kRhoWaves) void k_rho_lm(const double* __restr
 // column tile, output U or D); tiles past the l's columns exit.
 // tab[a][t][lm] = N_lm {U at knot t, D_t}; knot t is radial node N-1-t, and
 // N_lm (the harmonic's normalisation, shc) is folded in for the cross kernel.
-__global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
-                                                const double* __restrict__ rad, const double* __restrict__ KU,
-                                                const double* __restrict__ KD, const double* __restrict__ shc,
-                                                int natoms, int nrad, int lmax, int tcmax, double* U,
-                                                double* tab) {
+// (Four waves per work-group, one tile each: tile 4 blockIdx.x + wave of the
+// ntiles; small work-groups would cap the waves resident per CU.)
+__global__ __launch_bounds__(256) void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
+                                                 const double* __restrict__ rad, const double* __restrict__ KU,
+                                                 const double* __restrict__ KD, const double* __restrict__ shc,
+                                                 int natoms, int nrad, int lmax, int tcmax, long ntiles, double* U,
+                                                 double* tab) {
   const int nlm = (lmax + 1) * (lmax + 1), n2 = nrad + 2, ti = (nrad + 15) / 16;
-  int b = blockIdx.x;
+  const long tile = 4L * blockIdx.x + (threadIdx.x >> 6);
+  if (tile >= ntiles) return;
+  int b = (int)tile;
   const int which = b & 1;
   b >>= 1;
   const int tc = b % tcmax;
@@ -686,7 +690,7 @@ __global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_l
   const int it = b % ti, l = b / ti;
   const int w = 2 * l + 1, ncol = natoms * w;
   if (16 * tc >= ncol) return;
-  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const int i = 16 * it + c, col = 16 * tc + c;
   const int ca = col / w, clm = l * l + col % w;
   const double* M = (which ? KD : KU) + (long)l * n2 * nrad;
@@ -731,13 +735,16 @@ __global__ __launch_bounds__(64) void k_poisson(const double* __restrict__ rho_l
 // (src/atomicgrid.cpp:434-462): tile (shells i0.., angular points j0..) of atom a.
 // |r_p - R_a| is the radial node r_i (the point is R_a + r_i Omega_j with a unit
 // Omega_j), which spares the coordinate reads.
-__global__ __launch_bounds__(64) void k_vcell(const double* __restrict__ U, const double* __restrict__ ylm,
-                                              const double* __restrict__ rad, int nrad, int nang, int nlm,
-                                              double* vcell) {
+// (Four waves per work-group, one tile each: tile 4 blockIdx.x + wave.)
+__global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U, const double* __restrict__ ylm,
+                                               const double* __restrict__ rad, int nrad, int nang, int nlm,
+                                               long ntiles, double* vcell) {
   const int ti = (nrad + 15) / 16, tj = (nang + 15) / 16;
-  const int a = blockIdx.x / (ti * tj), rem = blockIdx.x % (ti * tj);
+  const long tile = 4L * blockIdx.x + (threadIdx.x >> 6);
+  if (tile >= ntiles) return;
+  const int a = (int)(tile / (ti * tj)), rem = (int)(tile % (ti * tj));
   const int i0 = 16 * (rem / tj), j0 = 16 * (rem % tj);
-  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const int i = i0 + c, j = j0 + c;
   dbl4 d = {0.0, 0.0, 0.0, 0.0};
   for (int kb = 0; kb < nlm; kb += 4 * kSteps2) {
@@ -2353,12 +2360,14 @@ int stage_hartree(dg_handle* h) {
   DG_TRY(run(h, "rho_lm", [&] { launch_rho_lm(h, ti); }));
   DG_TRY(run(h, "poisson", [&] {
     const int tcmax = (h->natoms * (2 * h->lmax + 1) + 15) / 16;
-    hipLaunchKernelGGL(k_poisson, dim3((h->lmax + 1) * ti * tcmax * 2), dim3(64), 0, h->stream, h->rho_lm, h->qc,
-                       h->rad, h->minvT, h->KD, h->shc, h->natoms, h->nrad, h->lmax, tcmax, h->U, h->tab);
+    const long ntiles = (long)(h->lmax + 1) * ti * tcmax * 2;
+    hipLaunchKernelGGL(k_poisson, dim3(nblocks(ntiles, 4)), dim3(256), 0, h->stream, h->rho_lm, h->qc, h->rad,
+                       h->minvT, h->KD, h->shc, h->natoms, h->nrad, h->lmax, tcmax, ntiles, h->U, h->tab);
   }));
   DG_TRY(run(h, "vcell", [&] {
-    hipLaunchKernelGGL(k_vcell, dim3(h->natoms * ti * ((h->nang + 15) / 16)), dim3(64), 0, h->stream, h->U, h->ylm,
-                       h->rad, h->nrad, h->nang, h->nlm, h->vcell);
+    const long ntiles = (long)h->natoms * ti * ((h->nang + 15) / 16);
+    hipLaunchKernelGGL(k_vcell, dim3(nblocks(ntiles, 4)), dim3(256), 0, h->stream, h->U, h->ylm, h->rad, h->nrad,
+                       h->nang, h->nlm, ntiles, h->vcell);
   }));
   if (h->natoms > 1) {
     CrossArgs a;
