@@ -775,7 +775,6 @@ struct CrossArgs {
   const double* knots;  // ascending, nrad
   const double* invh;   // 1/(x_t - x_{t-1})
   const double* tab;    // [natoms][nrad][nlm]{y, D}
-  const double* shc;    // per (l, m>=0): {prefactor, a_lm, b_lm}
   const int* items;     // {k, p_begin, p_end, perm offset} point ranges outside cell k, k ascending
   const unsigned short* perm;  // per item, its points by distance to atom k, interval groups padded to even
   const unsigned short* pkey;  // interval key (spline_key) of every perm slot
@@ -1025,7 +1024,10 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
         for (int q = 0; q < Q; ++q) p1[q] = p2[q] = 0.0;
 #pragma unroll
         for (int l = m; l <= LMAX; ++l) {
-          const int ci = 3 * (l * (l + 1) / 2 + m);
+          // recurrence coefficients (2l-1)/(l-m), (l+m-1)/(l-m) fold to constants
+          // after unrolling (a global read of them waited on its full latency)
+          const double ra = (l >= m + 2) ? (double)(2 * l - 1) / (double)(l - m) : 0.0;
+          const double rb = (l >= m + 2) ? (double)(l + m - 1) / (double)(l - m) : 0.0;
           double yp[Q], yn[Q];
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
@@ -1035,7 +1037,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             else if (l == m + 1)
               pl = ct[q] * (double)(2 * m + 1) * pmm[q];
             else
-              pl = A.shc[ci + 1] * ct[q] * p1[q] - A.shc[ci + 2] * p2[q];
+              pl = ra * ct[q] * p1[q] - rb * p2[q];
             p2[q] = p1[q];
             p1[q] = pl;
             const double base = pl;  // N_lm is folded into the table
@@ -2378,7 +2380,6 @@ int stage_hartree(dg_handle* h) {
     a.knots = h->knots;
     a.invh = h->invh;
     a.tab = h->tab;
-    a.shc = h->shc;
     a.items = h->items;
     a.perm = h->perm;
     a.plen = h->plen;
