@@ -1052,6 +1052,15 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             const double* T0 = tab + s0 * nlmc * 2;
             const double* T1 = tab + s1 * nlmc * 2;
             const double a0 = T0[ip], a1 = T0[ip + 1], b0 = T1[ip], b1 = T1[ip + 1];
+            double c0 = 0.0, c1 = 0.0, e0 = 0.0, e1 = 0.0;
+            if (m > 0) {
+              c0 = T0[in];
+              c1 = T0[in + 1];
+              e0 = T1[in];
+              e1 = T1[in + 1];
+            }
+            // all of the step's table reads issued before its first use
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
               sy0[q] += yp[q] * a0;
@@ -1060,7 +1069,6 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
               sd1[q] += yp[q] * b1;
             }
             if (m > 0) {
-              const double c0 = T0[in], c1 = T0[in + 1], e0 = T1[in], e1 = T1[in + 1];
 #pragma unroll
               for (int q = 0; q < Q; ++q) {
                 sy0[q] += yn[q] * c0;
