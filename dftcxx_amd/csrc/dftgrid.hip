@@ -604,11 +604,12 @@ __global__ __launch_bounds__(256) void k_scalars(ScalarArgs a) { scalars_block(a
 // The three small per-atom products of the Hartree stage run as FP64 MFMA
 // tiles (lane (c, g) feeds A[c][g] and B[g][c]; result register v holds row
 // 4v + g, column c), summing the inner index in steps of 4 with zero padding.
-// Each wave loads the operands of kSteps steps (kSteps2 in the one-product
-// kernels, whose registers allow more) before it issues their MFMAs, so that
-// one memory latency covers them.
+// Each wave loads the operands of kSteps steps (kSteps2 in the Poisson kernel,
+// kStepsV in the own-cell one: 84 inner indices, all 81 harmonics of lmax 8)
+// before it issues their MFMAs, so that one memory latency covers them.
 constexpr int kSteps = 4;
 constexpr int kSteps2 = 8;
+constexpr int kStepsV = 21;
 
 // rho_lm[a][i][lm] = 4 pi sum_j F[i][j] Y[j][lm] with F[i][j] = rho_p w_j wb_p,
 // p = (a nrad + i) nang + j (src/atomicgrid.cpp:241-296).  Block (a, 16 shells
@@ -747,16 +748,16 @@ __global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U, con
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const int i = i0 + c, j = j0 + c;
   dbl4 d = {0.0, 0.0, 0.0, 0.0};
-  for (int kb = 0; kb < nlm; kb += 4 * kSteps2) {
-    double av[kSteps2], bv[kSteps2];
+  for (int kb = 0; kb < nlm; kb += 4 * kStepsV) {
+    double av[kStepsV], bv[kStepsV];
 #pragma unroll
-    for (int u = 0; u < kSteps2; ++u) {
+    for (int u = 0; u < kStepsV; ++u) {
       const int k = kb + 4 * u + g;
       av[u] = (k < nlm && i < nrad) ? U[((long)a * nrad + i) * nlm + k] : 0.0;
       bv[u] = (k < nlm && j < nang) ? ylm[(long)j * nlm + k] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSteps2; ++u) d = mfma16x16x4(av[u], bv[u], d);
+    for (int u = 0; u < kStepsV; ++u) d = mfma16x16x4(av[u], bv[u], d);
   }
   if (j >= nang) return;
 #pragma unroll
@@ -1128,8 +1129,19 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
       k0 = mol_atom0[m];
       k1 = mol_atom0[m + 1];
     }
+    // eight partials loaded together per round (one memory latency), added in k order
     double v = 0.0;
-    for (int k = k0; k < k1; ++k) v += (k == own) ? vcell[p] : vpart[vbase[k] + p];
+    for (int kb = k0; kb < k1; kb += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = kb + u;
+        x[u] = (k < k1) ? ((k == own) ? vcell[p] : vpart[vbase[k] + p]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (kb + u < k1) v += x[u];
+    }
     V[p] = v;
     uJ[p] = (w[p] * v) * 0.5;
     e = w[p] * v * rho[p];
