@@ -1796,31 +1796,60 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
   }
   // Z: the nearest atom's P is a lower bound, and a product that drops below
   // 1e-4 kBeckeSkip of it (below the skip level) is kept as truncated, which
-  // changes Z by less than 1e-24 Z per atom
-  double pnear = 1.0, Z = 1.0;
+  // changes Z by less than 1e-24 Z per atom.  The same pass lists the
+  // candidates P > kBeckeSkip pnear, a superset of the active P > kBeckeSkip Z
+  // (Z >= pnear) whose P are exact (a product above the skip level never
+  // reaches the truncation); the list is then cut to the active ones.
+  double pnear = 1.0, Z = 1.0, Pown = 0.0;
+  const int own = valid ? (int)(p / cell) : 0;
+  int ncand = 0;
+  bool cand_full = false;
   if (valid) {
     pnear = cell_function(atoms, invR, natoms, near, near, dnear, dnear, x, y, z, 0.0);
-    const double cut0 = 1e-4 * kBeckeSkip * pnear;
+    const double cut0 = 1e-4 * kBeckeSkip * pnear, pre = kBeckeSkip * pnear;
     Z = pnear;
-    for (int n = 0; n < natoms; ++n)
-      if (n != near)
-        Z += cell_function(atoms, invR, natoms, n, near, atom_dist(atoms, n, x, y, z), dnear, x, y, z, cut0);
+    for (int n = 0; n < natoms; ++n) {
+      double P = pnear;
+      if (n != near) {
+        P = cell_function(atoms, invR, natoms, n, near, atom_dist(atoms, n, x, y, z), dnear, x, y, z, cut0);
+        Z += P;
+      }
+      if (n == own) Pown = P;
+      if (P > pre) {
+        if (ncand < kBeckeActive) {
+          act_n[ncand * 64 + lane] = (unsigned short)n;
+          act_P[ncand * 64 + lane] = P;
+          ++ncand;
+        } else {
+          cand_full = true;
+        }
+      }
+    }
   }
   const double skip = kBeckeSkip * Z;
   int nact = 0;
-  double Pown = 0.0;
-  const int own = valid ? (int)(p / cell) : 0;
-  for (int n = 0; n < natoms && valid; ++n) {
-    const double dn = (n == near) ? dnear : atom_dist(atoms, n, x, y, z);
-    const double P = (n == near) ? pnear : cell_function(atoms, invR, natoms, n, near, dn, dnear, x, y, z, skip);
-    if (n == own) Pown = P;
-    if (P > skip) {
-      if (nact < kBeckeActive) {
-        act_n[nact * 64 + lane] = (unsigned short)n;
+  if (!cand_full) {
+    for (int t = 0; t < ncand; ++t) {
+      const double P = act_P[t * 64 + lane];
+      if (P > skip) {
+        act_n[nact * 64 + lane] = act_n[t * 64 + lane];
         act_P[nact * 64 + lane] = P;
         ++nact;
-      } else {
-        *overflow = 1;
+      }
+    }
+  } else {
+    // more candidates than slots: list the active cells from scratch
+    for (int n = 0; n < natoms; ++n) {
+      const double dn = (n == near) ? dnear : atom_dist(atoms, n, x, y, z);
+      const double P = (n == near) ? pnear : cell_function(atoms, invR, natoms, n, near, dn, dnear, x, y, z, skip);
+      if (P > skip) {
+        if (nact < kBeckeActive) {
+          act_n[nact * 64 + lane] = (unsigned short)n;
+          act_P[nact * 64 + lane] = P;
+          ++nact;
+        } else {
+          *overflow = 1;
+        }
       }
     }
   }
