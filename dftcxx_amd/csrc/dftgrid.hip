@@ -1856,15 +1856,18 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
   const double pref = (valid && Z > 0.0) ? w0a / Z : 0.0;
   const double wbv = (valid && Z > 0.0) ? Pown / Z : 0.0;
   double own_acc[3] = {0.0, 0.0, 0.0};
+  int tnext = 0;  // the active list is in ascending atom order: its next entry
   for (int B = 0; B < natoms; ++B) {
     double gB[3] = {0.0, 0.0, 0.0};
+    double PB = 0.0;
+    if (tnext < nact && act_n[tnext * 64 + lane] == B) {
+      PB = act_P[tnext * 64 + lane];
+      ++tnext;
+    }
     if (valid && B != own && pref != 0.0) {
       const double dB = atom_dist(atoms, B, x, y, z);
       const double uB[3] = {(x - atoms[4 * B]) / dB, (y - atoms[4 * B + 1]) / dB, (z - atoms[4 * B + 2]) / dB};
       double sumdP[3] = {0.0, 0.0, 0.0}, dPa[3] = {0.0, 0.0, 0.0};
-      double PB = 0.0;
-      for (int t = 0; t < nact; ++t)
-        if (act_n[t * 64 + lane] == B) PB = act_P[t * 64 + lane];
       if (PB > skip) {
         // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) (-(u_B + mu_Bm e_Bm)/R_Bm)
         double acc[3] = {0.0, 0.0, 0.0};
