@@ -239,8 +239,17 @@ __global__ __launch_bounds__(256) void k_contract_batch_reduce(const double* __r
     const int t = (jj < 16) ? 0 : (ii < 16 ? 1 : 2);
     const int r = ii & 15;
     const int idx = (o * 3 + t) * 256 + (r >> 2) * 64 + (jj & 15) + 16 * (r & 3);
+    // eight chunks' partials loaded per round (one memory latency), added in chunk order
     double s = 0.0;
-    for (int ch = c0; ch < c1; ++ch) s += part[(long)ch * nout * 768 + idx];
+    int ch = c0;
+    for (; ch + 8 <= c1; ch += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = part[(long)(ch + u) * nout * 768 + idx];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; ch < c1; ++ch) s += part[(long)ch * nout * 768 + idx];
     (o == 0 ? out0 : out1)[poff[m] + i + (long)j * n] = s;
   }
 }
