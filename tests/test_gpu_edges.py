@@ -1,6 +1,7 @@
 """GPU edge cases: a single atom (no cross-atom potentials), the smallest grids
 (5 radial shells, the 6-point Lebedev rule, lmax 0), a two-atom molecule, a
-batch of one, all against the oracle or the single-molecule handle."""
+batch of one, a radial grid whose spline table needs two l passes of the
+cross-potential kernel, all against the oracle or the single-molecule handle."""
 import numpy as np
 import pytest
 
@@ -70,3 +71,21 @@ def test_batch_of_one():
     assert np.abs(J[0] - Js).max() < 1e-12 * np.abs(Js).max()
     assert np.abs(XC[0] - XCs).max() < 1e-12 * np.abs(XCs).max()
     assert abs(sc[0, 0] - e) < 1e-12 * abs(e)
+
+
+def test_cross_potential_in_two_l_passes():
+    # 130 radial shells at lmax 8: the spline table, 2*130 + 130*81*2 doubles
+    # (170 KB), exceeds the cross kernel's 150 KB LDS budget, so the kernel runs
+    # once per l range, the second pass adding to the first
+    print("[case] H2O 130x26 lmax 8 (two l passes)", flush=True)
+    _compare_with_oracle(M.h2o(), 130, 26, 8)
+    mols = [M.h2o(), M.co("sto3g")]
+    b = G.Batch(mols, 130, 26, 8)
+    Ps = [core_guess(m, *G.one_electron(m)) for m in mols]
+    J, XC, sc = b.build(Ps)
+    for m, (mol, P) in enumerate(zip(mols, Ps)):
+        g = G.MolecularGrid(mol, 130, 26, 8)
+        g.set_density(P)
+        g.correct_densities()
+        Js = g.calculate_hartree_potential()
+        assert np.abs(J[m] - Js).max() < 1e-12 * np.abs(Js).max()
