@@ -1,7 +1,8 @@
 """GPU edge cases: a single atom (no cross-atom potentials), the smallest grids
 (5 radial shells, the 6-point Lebedev rule, lmax 0), a two-atom molecule, a
 batch of one, a radial grid whose spline table needs two l passes of the
-cross-potential kernel, all against the oracle or the single-molecule handle."""
+cross-potential kernel, lmax 10 and 12, all against the oracle or the
+single-molecule handle."""
 import numpy as np
 import pytest
 
@@ -89,3 +90,11 @@ def test_cross_potential_in_two_l_passes():
         g.correct_densities()
         Js = g.calculate_hartree_potential()
         assert np.abs(J[m] - Js).max() < 1e-12 * np.abs(Js).max()
+
+
+@pytest.mark.parametrize("nrad,nang,lmax", [(20, 110, 10), (16, 146, 12)])
+def test_high_lmax(nrad, nang, lmax):
+    # lmax above 8: two harmonic-tile groups in the projection kernel and the
+    # largest cross-kernel instantiations
+    print(f"[case] CH4 {nrad}x{nang} lmax {lmax}", flush=True)
+    _compare_with_oracle(M.ch4("sto3g"), nrad, nang, lmax)
