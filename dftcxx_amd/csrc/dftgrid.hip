@@ -1052,13 +1052,18 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             const int in = (l * l + l - m - lmb) * 2;
             const double* T0 = tab + s0 * nlmc * 2;
             const double* T1 = tab + s1 * nlmc * 2;
-            const double a0 = T0[ip], a1 = T0[ip + 1], b0 = T1[ip], b1 = T1[ip + 1];
+            // 16-byte {y, D} pairs (the table and its rows start on 16-byte boundaries)
+            const double2 A = *reinterpret_cast<const double2*>(T0 + ip);
+            const double2 B = *reinterpret_cast<const double2*>(T1 + ip);
+            const double a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
             double c0 = 0.0, c1 = 0.0, e0 = 0.0, e1 = 0.0;
             if (m > 0) {
-              c0 = T0[in];
-              c1 = T0[in + 1];
-              e0 = T1[in];
-              e1 = T1[in + 1];
+              const double2 C = *reinterpret_cast<const double2*>(T0 + in);
+              const double2 E = *reinterpret_cast<const double2*>(T1 + in);
+              c0 = C.x;
+              c1 = C.y;
+              e0 = E.x;
+              e1 = E.y;
             }
             // all of the step's table reads issued before its first use
             __builtin_amdgcn_sched_barrier(0);
