@@ -273,6 +273,25 @@ def main():
             roof["traffic"] = t.get(dom)
         out["roofline"] = roof
 
+    # the reference's call sequence with host buffers (P in, J and XC out over PCIe,
+    # each call synchronous): a PCIe-inclusive rate, reported beside the value
+    for _ in range(2):
+        grid.set_density(P)
+        grid.correct_densities()
+        grid.calculate_hartree_potential()
+        grid.calculate_exchange_correlation()
+    nhost = 10
+    t0 = time.perf_counter()
+    for _ in range(nhost):
+        grid.set_density(P)
+        grid.correct_densities()
+        grid.calculate_hartree_potential()
+        grid.calculate_exchange_correlation()
+    host_s = (time.perf_counter() - t0) / nhost
+    out["host_buffer_path"] = {"ms_per_build": 1e3 * host_s, "value": units / host_s, "unit": "pts*bf^2/s",
+                               "calls": "set_density, correct_densities, calculate_hartree_potential, "
+                                        "calculate_exchange_correlation"}
+
     if not args.no_adjoint:
         # dE_xc/dP and dE_xc/dR of the build above (c varying), second call timed
         grid.build_device(*ptrs)
