@@ -8,9 +8,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dftcxx/molecular_grid.hpp"
@@ -185,6 +187,44 @@ void one_electron(const MoleculeData& m, std::vector<double>& S, std::vector<dou
           V[i + j * n] = V[j + i * n] = v_blk[ia * nb + ib];
         }
     }
+}
+
+// Threads for the host algebra of large bases: OMP_NUM_THREADS when set, else
+// the hardware's count, at most 16.
+int host_threads() {
+  static const int nt = [] {
+    int t = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+      const int v = std::atoi(e);
+      if (v > 0) t = v;
+    }
+    return std::max(1, std::min(t, 16));
+  }();
+  return nt;
+}
+
+// f(j0, j1) over the column ranges of [0, m), split over the host threads when
+// the work (multiply-adds) is large.  A column is done by one thread in the
+// serial order, so the result does not depend on the split; if a thread cannot
+// be started its range runs here.
+template <class F>
+void parallel_cols(int m, double work, F f) {
+  const int nt = work >= 2e6 ? std::min(host_threads(), m) : 1;
+  if (nt <= 1) {
+    f(0, m);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) {
+    const int j0 = (int)((long)m * t / nt), j1 = (int)((long)m * (t + 1) / nt);
+    try {
+      th.emplace_back(f, j0, j1);
+    } catch (...) {
+      f(j0, j1);
+    }
+  }
+  f(0, (int)((long)m / nt));
+  for (auto& x : th) x.join();
 }
 
 // ---------------------------------------------------------------------------
@@ -466,32 +506,38 @@ void eigh_lowest(int n, int nv, const std::vector<double>& Ain, std::vector<doub
     }
     std::memcpy(&Z[(size_t)j * n], y.data(), sizeof(double) * n);
   }
-  // back-transformation x = H_0 (H_1 (... H_{n-3} y)), all vectors at once
-  for (int k = nh - 1; k >= 0; --k) {
-    const double* v = &V[(size_t)k * n];
-    for (int j = 0; j < nv; ++j) {
-      double* z = &Z[(size_t)j * n];
-      double dot = 0.0;
-      for (int i = k + 1; i < n; ++i) dot += v[i] * z[i];
-      const double f = 2.0 * dot;
-      for (int i = k + 1; i < n; ++i) z[i] -= f * v[i];
+  // back-transformation x = H_0 (H_1 (... H_{n-3} y)), vectors independent of each other
+  const double* Vd = V.data();
+  double* Zd = Z.data();
+  parallel_cols(nv, 2.0 * n * n * nv, [=](int j0, int j1) {
+    for (int j = j0; j < j1; ++j) {
+      double* z = Zd + (size_t)j * n;
+      for (int k = nh - 1; k >= 0; --k) {
+        const double* v = Vd + (size_t)k * n;
+        double dot = 0.0;
+        for (int i = k + 1; i < n; ++i) dot += v[i] * z[i];
+        const double f = 2.0 * dot;
+        for (int i = k + 1; i < n; ++i) z[i] -= f * v[i];
+      }
     }
-  }
+  });
 }
 
 // C = A B, column-major, A n x n, B n x m, each column of C as axpys over
 // contiguous columns of A (every C_ij is summed over k in ascending order)
 void gemm(int n, const double* A, const double* B, double* C, int m = -1) {
   if (m < 0) m = n;
-  for (int j = 0; j < m; ++j) {
-    double* c = C + (size_t)j * n;
-    for (int i = 0; i < n; ++i) c[i] = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const double b = B[k + (size_t)j * n];
-      const double* a = A + (size_t)k * n;
-      for (int i = 0; i < n; ++i) c[i] += a[i] * b;
+  parallel_cols(m, (double)n * n * m, [=](int j0, int j1) {
+    for (int j = j0; j < j1; ++j) {
+      double* c = C + (size_t)j * n;
+      for (int i = 0; i < n; ++i) c[i] = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double b = B[k + (size_t)j * n];
+        const double* a = A + (size_t)k * n;
+        for (int i = 0; i < n; ++i) c[i] += a[i] * b;
+      }
     }
-  }
+  });
 }
 
 // Host algebra of the reference's DFT class (src/dft.h:41-216): integrals,
@@ -549,14 +595,19 @@ class HostSCF {
     gemm(n_, X_.data(), Cc.data(), C.data(), nocc);  // C_occ = X Cc
     // P = C_occ C_occ^T, each element summed over the occupied orbitals in order
     std::vector<double> Pn(nn, 0.0);
-    for (int k = 0; k < nocc; ++k) {
-      const double* ck = C.data() + (size_t)k * n_;
-      for (int j = 0; j < n_; ++j) {
-        const double cj = ck[j];
-        double* pcol = Pn.data() + (size_t)j * n_;
-        for (int i = 0; i < n_; ++i) pcol[i] += ck[i] * cj;
+    const int n = n_;
+    const double* Cd = C.data();
+    double* Pd = Pn.data();
+    parallel_cols(n, (double)n * n * nocc, [=](int j0, int j1) {
+      for (int j = j0; j < j1; ++j) {
+        double* pcol = Pd + (size_t)j * n;
+        for (int k = 0; k < nocc; ++k) {
+          const double* ck = Cd + (size_t)k * n;
+          const double cj = ck[j];
+          for (int i = 0; i < n; ++i) pcol[i] += ck[i] * cj;
+        }
       }
-    }
+    });
     if (first_) {
       P_ = Pn;
       first_ = false;
