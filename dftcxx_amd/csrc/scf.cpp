@@ -79,116 +79,6 @@ const int kCart[3][6][3] = {{{0, 0, 0}},
                             {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
                             {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
 
-// S, T, V (nuclear attraction of all nuclei) over contracted Cartesian
-// Gaussians with primitive norms of src/cgf.cpp:102-114.  The reference's
-// nuclear-attraction prefactor uses pi = 3.14159265359 (src/integrals.cpp:342,
-// 386), reproduced by the factor kPiTruncated / kPi.
-void one_electron(const MoleculeData& m, std::vector<double>& S, std::vector<double>& T,
-                  std::vector<double>& V) {
-  const int n = m.nbf(), ns = (int)m.shell_L.size();
-  S.assign((size_t)n * n, 0.0);
-  T.assign((size_t)n * n, 0.0);
-  V.assign((size_t)n * n, 0.0);
-  std::vector<int> off(ns);
-  for (int s = 0, o = 0; s < ns; ++s) {
-    off[s] = o;
-    o += (m.shell_L[s] + 1) * (m.shell_L[s] + 2) / 2;
-  }
-  const double vfac = dg::kPiTruncated / dg::kPi;
-  for (int sa = 0; sa < ns; ++sa)
-    for (int sb = sa; sb < ns; ++sb) {
-      const int La = m.shell_L[sa], Lb = m.shell_L[sb];
-      const int na = (La + 1) * (La + 2) / 2, nb = (Lb + 1) * (Lb + 2) / 2;
-      const double* A = &m.xyz[3 * m.shell_atom[sa]];
-      const double* B = &m.xyz[3 * m.shell_atom[sb]];
-      std::vector<double> s_blk(na * nb, 0.0), t_blk(na * nb, 0.0), v_blk(na * nb, 0.0);
-      for (int ka = 0; ka < m.shell_nprim[sa]; ++ka)
-        for (int kb = 0; kb < m.shell_nprim[sb]; ++kb) {
-          const double a = m.alpha[m.shell_offset[sa] + ka], b = m.alpha[m.shell_offset[sb] + kb];
-          const double ca = m.coef[m.shell_offset[sa] + ka], cb = m.coef[m.shell_offset[sb] + kb];
-          const double p = a + b;
-          Herm H[3];
-          for (int d = 0; d < 3; ++d) hermite(La, Lb + 2, a, b, A[d], B[d], H[d]);
-          const double sq = std::sqrt(dg::kPi / p);
-          double P[3];
-          for (int d = 0; d < 3; ++d) P[d] = (a * A[d] + b * B[d]) / p;
-          // R_tuv for every nucleus, t+u+v <= La+Lb
-          const int Lt = La + Lb;
-          std::vector<double> Rsum((Lt + 1) * (Lt + 1) * (Lt + 1), 0.0);
-          for (int c = 0; c < m.natoms(); ++c) {
-            const double* C = &m.xyz[3 * c];
-            const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
-            double F[16];
-            boys(Lt, p * (X * X + Y * Y + Z * Z), F);
-            // R^n_{tuv}, built by the MD recursion from R^n_{000} = (-2p)^n F_n
-            const int D = Lt + 1;
-            std::vector<double> R((size_t)(Lt + 1) * D * D * D, 0.0);
-            auto r = [&](int nn, int t, int u, int v) -> double& {
-              return R[(((size_t)nn * D + t) * D + u) * D + v];
-            };
-            double f = 1.0;
-            for (int nn = 0; nn <= Lt; ++nn) {
-              r(nn, 0, 0, 0) = f * F[nn];
-              f *= -2.0 * p;
-            }
-            for (int nn = Lt - 1; nn >= 0; --nn)
-              for (int t = 0; t <= Lt - nn; ++t)
-                for (int u = 0; u <= Lt - nn - t; ++u)
-                  for (int v = 0; v <= Lt - nn - t - u; ++v) {
-                    if (t + u + v == 0) continue;
-                    double val;
-                    if (t > 0)
-                      val = (t > 1 ? (t - 1) * r(nn + 1, t - 2, u, v) : 0.0) + X * r(nn + 1, t - 1, u, v);
-                    else if (u > 0)
-                      val = (u > 1 ? (u - 1) * r(nn + 1, t, u - 2, v) : 0.0) + Y * r(nn + 1, t, u - 1, v);
-                    else
-                      val = (v > 1 ? (v - 1) * r(nn + 1, t, u, v - 2) : 0.0) + Z * r(nn + 1, t, u, v - 1);
-                    r(nn, t, u, v) = val;
-                  }
-            for (int t = 0; t <= Lt; ++t)
-              for (int u = 0; u <= Lt - t; ++u)
-                for (int v = 0; v <= Lt - t - u; ++v)
-                  Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v] += -(double)m.Z[c] * r(0, t, u, v);
-          }
-          for (int ia = 0; ia < na; ++ia)
-            for (int ib = 0; ib < nb; ++ib) {
-              const int* la = kCart[La][ia];
-              const int* lb = kCart[Lb][ib];
-              const double Na = dg::gto_norm(a, la[0], la[1], la[2]);
-              const double Nb = dg::gto_norm(b, lb[0], lb[1], lb[2]);
-              const double pre = ca * cb * Na * Nb;
-              double s1[3], t1[3];
-              for (int d = 0; d < 3; ++d) {
-                const int i = la[d], j = lb[d];
-                s1[d] = H[d].e[i][j][0] * sq;
-                const double sp2 = H[d].e[i][j + 2][0] * sq;
-                const double sm2 = j >= 2 ? H[d].e[i][j - 2][0] * sq : 0.0;
-                t1[d] = -2.0 * b * b * sp2 + b * (2 * j + 1) * s1[d] - 0.5 * j * (j - 1) * sm2;
-              }
-              const double ov = s1[0] * s1[1] * s1[2];
-              const double kin = t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2];
-              double vn = 0.0;
-              for (int t = 0; t <= la[0] + lb[0]; ++t)
-                for (int u = 0; u <= la[1] + lb[1]; ++u)
-                  for (int v = 0; v <= la[2] + lb[2]; ++v)
-                    vn += H[0].e[la[0]][lb[0]][t] * H[1].e[la[1]][lb[1]][u] * H[2].e[la[2]][lb[2]][v] *
-                          Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v];
-              vn *= 2.0 * dg::kPi / p * vfac;
-              s_blk[ia * nb + ib] += pre * ov;
-              t_blk[ia * nb + ib] += pre * kin;
-              v_blk[ia * nb + ib] += pre * vn;
-            }
-        }
-      for (int ia = 0; ia < na; ++ia)
-        for (int ib = 0; ib < nb; ++ib) {
-          const size_t i = off[sa] + ia, j = off[sb] + ib;
-          S[i + j * n] = S[j + i * n] = s_blk[ia * nb + ib];
-          T[i + j * n] = T[j + i * n] = t_blk[ia * nb + ib];
-          V[i + j * n] = V[j + i * n] = v_blk[ia * nb + ib];
-        }
-    }
-}
-
 // Threads for the host algebra of large bases: OMP_NUM_THREADS when set, else
 // the hardware's count, at most 16.
 int host_threads() {
@@ -225,6 +115,125 @@ void parallel_cols(int m, double work, F f) {
   }
   f(0, (int)((long)m / nt));
   for (auto& x : th) x.join();
+}
+
+// S, T, V (nuclear attraction of all nuclei) over contracted Cartesian
+// Gaussians with primitive norms of src/cgf.cpp:102-114.  The reference's
+// nuclear-attraction prefactor uses pi = 3.14159265359 (src/integrals.cpp:342,
+// 386), reproduced by the factor kPiTruncated / kPi.
+void one_electron(const MoleculeData& m, std::vector<double>& S, std::vector<double>& T,
+                  std::vector<double>& V) {
+  const int n = m.nbf(), ns = (int)m.shell_L.size();
+  S.assign((size_t)n * n, 0.0);
+  T.assign((size_t)n * n, 0.0);
+  V.assign((size_t)n * n, 0.0);
+  std::vector<int> off(ns);
+  for (int s = 0, o = 0; s < ns; ++s) {
+    off[s] = o;
+    o += (m.shell_L[s] + 1) * (m.shell_L[s] + 2) / 2;
+  }
+  const double vfac = dg::kPiTruncated / dg::kPi;
+  // shell rows (sa, all sb >= sa) split over the host threads for large bases; the
+  // order pairs heavy early rows with light late ones so that ranges carry similar
+  // work; every pair's block is computed as in the serial loop and written once
+  std::vector<int> order(ns);
+  for (int t = 0, lo = 0, hi = ns - 1; t < ns; ++t) order[t] = (t & 1) ? hi-- : lo++;
+  parallel_cols(ns, 50.0 * ns * ns * m.natoms(), [&](int r0, int r1) {
+    for (int rr = r0; rr < r1; ++rr) {
+      const int sa = order[rr];
+      for (int sb = sa; sb < ns; ++sb) {
+        const int La = m.shell_L[sa], Lb = m.shell_L[sb];
+        const int na = (La + 1) * (La + 2) / 2, nb = (Lb + 1) * (Lb + 2) / 2;
+        const double* A = &m.xyz[3 * m.shell_atom[sa]];
+        const double* B = &m.xyz[3 * m.shell_atom[sb]];
+        std::vector<double> s_blk(na * nb, 0.0), t_blk(na * nb, 0.0), v_blk(na * nb, 0.0);
+        for (int ka = 0; ka < m.shell_nprim[sa]; ++ka)
+          for (int kb = 0; kb < m.shell_nprim[sb]; ++kb) {
+            const double a = m.alpha[m.shell_offset[sa] + ka], b = m.alpha[m.shell_offset[sb] + kb];
+            const double ca = m.coef[m.shell_offset[sa] + ka], cb = m.coef[m.shell_offset[sb] + kb];
+            const double p = a + b;
+            Herm H[3];
+            for (int d = 0; d < 3; ++d) hermite(La, Lb + 2, a, b, A[d], B[d], H[d]);
+            const double sq = std::sqrt(dg::kPi / p);
+            double P[3];
+            for (int d = 0; d < 3; ++d) P[d] = (a * A[d] + b * B[d]) / p;
+            // R_tuv for every nucleus, t+u+v <= La+Lb
+            const int Lt = La + Lb;
+            std::vector<double> Rsum((Lt + 1) * (Lt + 1) * (Lt + 1), 0.0);
+            for (int c = 0; c < m.natoms(); ++c) {
+              const double* C = &m.xyz[3 * c];
+              const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
+              double F[16];
+              boys(Lt, p * (X * X + Y * Y + Z * Z), F);
+              // R^n_{tuv}, built by the MD recursion from R^n_{000} = (-2p)^n F_n
+              const int D = Lt + 1;
+              std::vector<double> R((size_t)(Lt + 1) * D * D * D, 0.0);
+              auto r = [&](int nn, int t, int u, int v) -> double& {
+                return R[(((size_t)nn * D + t) * D + u) * D + v];
+              };
+              double f = 1.0;
+              for (int nn = 0; nn <= Lt; ++nn) {
+                r(nn, 0, 0, 0) = f * F[nn];
+                f *= -2.0 * p;
+              }
+              for (int nn = Lt - 1; nn >= 0; --nn)
+                for (int t = 0; t <= Lt - nn; ++t)
+                  for (int u = 0; u <= Lt - nn - t; ++u)
+                    for (int v = 0; v <= Lt - nn - t - u; ++v) {
+                      if (t + u + v == 0) continue;
+                      double val;
+                      if (t > 0)
+                        val = (t > 1 ? (t - 1) * r(nn + 1, t - 2, u, v) : 0.0) + X * r(nn + 1, t - 1, u, v);
+                      else if (u > 0)
+                        val = (u > 1 ? (u - 1) * r(nn + 1, t, u - 2, v) : 0.0) + Y * r(nn + 1, t, u - 1, v);
+                      else
+                        val = (v > 1 ? (v - 1) * r(nn + 1, t, u, v - 2) : 0.0) + Z * r(nn + 1, t, u, v - 1);
+                      r(nn, t, u, v) = val;
+                    }
+              for (int t = 0; t <= Lt; ++t)
+                for (int u = 0; u <= Lt - t; ++u)
+                  for (int v = 0; v <= Lt - t - u; ++v)
+                    Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v] += -(double)m.Z[c] * r(0, t, u, v);
+            }
+            for (int ia = 0; ia < na; ++ia)
+              for (int ib = 0; ib < nb; ++ib) {
+                const int* la = kCart[La][ia];
+                const int* lb = kCart[Lb][ib];
+                const double Na = dg::gto_norm(a, la[0], la[1], la[2]);
+                const double Nb = dg::gto_norm(b, lb[0], lb[1], lb[2]);
+                const double pre = ca * cb * Na * Nb;
+                double s1[3], t1[3];
+                for (int d = 0; d < 3; ++d) {
+                  const int i = la[d], j = lb[d];
+                  s1[d] = H[d].e[i][j][0] * sq;
+                  const double sp2 = H[d].e[i][j + 2][0] * sq;
+                  const double sm2 = j >= 2 ? H[d].e[i][j - 2][0] * sq : 0.0;
+                  t1[d] = -2.0 * b * b * sp2 + b * (2 * j + 1) * s1[d] - 0.5 * j * (j - 1) * sm2;
+                }
+                const double ov = s1[0] * s1[1] * s1[2];
+                const double kin = t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2];
+                double vn = 0.0;
+                for (int t = 0; t <= la[0] + lb[0]; ++t)
+                  for (int u = 0; u <= la[1] + lb[1]; ++u)
+                    for (int v = 0; v <= la[2] + lb[2]; ++v)
+                      vn += H[0].e[la[0]][lb[0]][t] * H[1].e[la[1]][lb[1]][u] * H[2].e[la[2]][lb[2]][v] *
+                            Rsum[(t * (Lt + 1) + u) * (Lt + 1) + v];
+                vn *= 2.0 * dg::kPi / p * vfac;
+                s_blk[ia * nb + ib] += pre * ov;
+                t_blk[ia * nb + ib] += pre * kin;
+                v_blk[ia * nb + ib] += pre * vn;
+              }
+          }
+        for (int ia = 0; ia < na; ++ia)
+          for (int ib = 0; ib < nb; ++ib) {
+            const size_t i = off[sa] + ia, j = off[sb] + ib;
+            S[i + j * n] = S[j + i * n] = s_blk[ia * nb + ib];
+            T[i + j * n] = T[j + i * n] = t_blk[ia * nb + ib];
+            V[i + j * n] = V[j + i * n] = v_blk[ia * nb + ib];
+          }
+      }
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------

@@ -54,7 +54,8 @@ def test_reference_enum_mapping():
     assert (g.radial_points, g.angular_points, g.lmax) == (50, 110, 8)
 
 
-@pytest.mark.parametrize("make", [M.h2o, M.benzene, lambda: M.ch4("p631"), lambda: M.alkane(3)])
+# C10H22 (94 shells) takes the threaded path of the host integrals
+@pytest.mark.parametrize("make", [M.h2o, M.benzene, lambda: M.ch4("p631"), lambda: M.alkane(3), lambda: M.alkane(10)])
 def test_host_integrals_match_oracle(oracle_lib, make):
     mol = make()
     S, T, V = G.one_electron(mol)
