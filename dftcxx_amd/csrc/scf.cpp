@@ -10,6 +10,8 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -773,8 +775,26 @@ int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& 
         mols[m].energy();
       }
   };
+  // the molecules' density-matrix steps are independent: split over the host
+  // threads (each molecule's result is the serial one); an error is rethrown here
+  double work = 0.0;
+  for (int m = 0; m < nm; ++m) work += (double)nbf[m] * nbf[m] * nbf[m];
+  auto densities = [&](const std::vector<char>& act) {
+    std::exception_ptr err = nullptr;
+    std::mutex mu;
+    parallel_cols(nm, work, [&](int m0, int m1) {
+      try {
+        for (int m = m0; m < m1; ++m)
+          if (act[m]) mols[m].density_matrix();
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!err) err = std::current_exception();
+      }
+    });
+    if (err) std::rethrow_exception(err);
+  };
   std::vector<char> active(nm, 1);
-  for (auto& d : mols) d.density_matrix();
+  densities(active);
   build(false, active);
   std::vector<double> old(nm), diff(nm, 1.0);
   for (int m = 0; m < nm; ++m) old[m] = mols[m].et_;
@@ -782,8 +802,7 @@ int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& 
   const auto t0 = std::chrono::steady_clock::now();
   while (done < nm) {
     iteration++;
-    for (int m = 0; m < nm; ++m)
-      if (active[m]) mols[m].density_matrix();
+    densities(active);
     build(true, active);
     for (int m = 0; m < nm; ++m) {
       if (!active[m]) continue;
