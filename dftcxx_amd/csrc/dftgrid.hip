@@ -1053,17 +1053,17 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             const double* T0 = tab + s0 * nlmc * 2;
             const double* T1 = tab + s1 * nlmc * 2;
             // 16-byte {y, D} pairs (the table and its rows start on 16-byte boundaries)
-            const double2 A = *reinterpret_cast<const double2*>(T0 + ip);
-            const double2 B = *reinterpret_cast<const double2*>(T1 + ip);
-            const double a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
+            const double2 t0p = *reinterpret_cast<const double2*>(T0 + ip);
+            const double2 t1p = *reinterpret_cast<const double2*>(T1 + ip);
+            const double a0 = t0p.x, a1 = t0p.y, b0 = t1p.x, b1 = t1p.y;
             double c0 = 0.0, c1 = 0.0, e0 = 0.0, e1 = 0.0;
             if (m > 0) {
-              const double2 C = *reinterpret_cast<const double2*>(T0 + in);
-              const double2 E = *reinterpret_cast<const double2*>(T1 + in);
-              c0 = C.x;
-              c1 = C.y;
-              e0 = E.x;
-              e1 = E.y;
+              const double2 t0n = *reinterpret_cast<const double2*>(T0 + in);
+              const double2 t1n = *reinterpret_cast<const double2*>(T1 + in);
+              c0 = t0n.x;
+              c1 = t0n.y;
+              e0 = t1n.x;
+              e1 = t1n.y;
             }
             // all of the step's table reads issued before its first use
             __builtin_amdgcn_sched_barrier(0);
