@@ -1,8 +1,8 @@
 """GPU edge cases: a single atom (no cross-atom potentials), the smallest grids
 (5 radial shells, the 6-point Lebedev rule, lmax 0), a two-atom molecule, a
 batch of one, a radial grid whose spline table needs two l passes of the
-cross-potential kernel, lmax 10 and 12, all against the oracle or the
-single-molecule handle."""
+cross-potential kernel, lmax 10 and 12, a 134-function SCF, all against the
+oracle or the single-molecule handle."""
 import numpy as np
 import pytest
 
@@ -98,3 +98,17 @@ def test_high_lmax(nrad, nang, lmax):
     # largest cross-kernel instantiations
     print(f"[case] CH4 {nrad}x{nang} lmax {lmax}", flush=True)
     _compare_with_oracle(M.ch4("sto3g"), nrad, nang, lmax)
+
+
+def test_large_basis_scf_against_oracle():
+    # C10H22/6-31G (134 functions): the host SCF's orthogonalisation products take
+    # the threaded path; the first iterations' energies against the oracle SCF
+    from oracle import oracle as O
+    mol = M.alkane(10)
+    print("[case] C10H22 SCF 16x26 lmax 4, 4 iterations", flush=True)
+    en, res, _ = G.scf_run(mol, 16, 26, 4, threshold=0.0, min_iter=4, max_iter=4)
+    eo, _, _ = O.Grid(mol, 16, 26, 4).scf(threshold=0.0, min_iter=4, max_iter=4)
+    assert res["iterations"] == 4
+    n = min(len(en), len(eo))
+    assert n >= 4
+    assert np.abs(np.asarray(en[:n]) - np.asarray(eo[:n])).max() < 1e-8
