@@ -2083,6 +2083,7 @@ struct dg_handle {
   int natoms = 0, nbf = 0, nshell = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0, precision = 0;
   long npts = 0, npts_pad = 0, cell = 0;
   int ldphi = 0, ntb = 0, ngroups = 0, nsplit_small = 0;
+  int cus = 256;  // compute units of the device
   // 4 x 2 tile blocks of k_contract2d: {I0, nI, J0, nJ}, their reduction groups and point lists
   int *cb = nullptr, *cb_groups = nullptr, *cblist = nullptr, *cbloff = nullptr;
   int ncb = 0, nsplit2d = 0;
@@ -2374,9 +2375,11 @@ hipError_t allow_cross_lds(size_t bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// NT lm tiles per block: all of them up to lmax 8 (81 functions), else two groups.
+// NT lm tiles per block: all of them up to lmax 8 (81 functions), else two
+// groups; one per block when the (atom, shell tile) blocks would not fill the CUs.
 void launch_rho_lm(dg_handle* h, int ti) {
-  const int tiles = (h->nlm + 15) / 16, nt = tiles <= 6 ? tiles : (tiles + 1) / 2;
+  const int tiles = (h->nlm + 15) / 16;
+  const int nt = (long)h->natoms * ti < h->cus ? 1 : (tiles <= 6 ? tiles : (tiles + 1) / 2);
   const dim3 grid(h->natoms * ti, (tiles + nt - 1) / nt), block(64 * kRhoWaves);
 #define DG_RHO(N) \
   case N: \
@@ -2888,6 +2891,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   {
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    h->cus = cus;
     const size_t lds = h->cross_lds;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
