@@ -1134,17 +1134,17 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
       k0 = mol_atom0[m];
       k1 = mol_atom0[m + 1];
     }
-    // eight partials loaded together per round (one memory latency), added in k order
+    // sixteen partials loaded together per round (one memory latency), added in k order
     double v = 0.0;
-    for (int kb = k0; kb < k1; kb += 8) {
-      double x[8];
+    for (int kb = k0; kb < k1; kb += 16) {
+      double x[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int k = kb + u;
         x[u] = (k < k1) ? ((k == own) ? vcell[p] : vpart[vbase[k] + p]) : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 16; ++u)
         if (kb + u < k1) v += x[u];
     }
     V[p] = v;
