@@ -244,3 +244,22 @@ def test_create_rejects_invalid_input(change, msg):
     rc = G.lib().dg_create(C.byref(m), C.byref(b), C.byref(prm), C.byref(h))
     assert rc == 1 and not h.value
     assert msg in G.lib().dg_last_error().decode()
+
+
+def test_host_integrals_independent_of_thread_count():
+    # the threaded host integrals (C10H22, 94 shells) are bitwise those of one thread
+    import os
+    import subprocess
+    import sys
+    code = ("import hashlib, numpy as np; from dftcxx_amd import gridlib as G, molecule as M; "
+            "S, T, V = G.one_electron(M.alkane(10)); "
+            "print(hashlib.sha256(np.concatenate([S.ravel(), T.ravel(), V.ravel()]).tobytes()).hexdigest())")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = []
+    for nt in ("1", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=nt)
+        r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        out.append(r.stdout.strip())
+    assert out[0] == out[1]
