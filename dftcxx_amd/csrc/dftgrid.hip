@@ -2839,7 +2839,6 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       for (int r = 0; r < 4; ++r) cbg.insert(cbg.end(), {I0 + r, J0, r < nI ? nJ : 0});
     }
   h->ncb = (int)cbs.size() / 4;
-  h->nsplit_small = (int)std::max(1L, std::min(512L, h->npts_pad / 256));
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS
   {
@@ -2892,6 +2891,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     h->cus = cus;
+    // small contraction: exactly one round of resident work-groups (two per CU);
+    // a partial second round measured slower, and so did fewer splits
+    h->nsplit_small = (int)std::max(1L, std::min(2L * cus, h->npts_pad / 256));
     const size_t lds = h->cross_lds;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
