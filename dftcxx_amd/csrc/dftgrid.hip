@@ -2003,7 +2003,16 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const double* __restric
   if (live) {
     const long stride = (long)ngroups * nout * 1024;
     const double* src = part + ((long)grp * nout + o) * 1024 + idx;
-    for (int sp = r; sp < nsplit; sp += 16) s += src[sp * stride];
+    // eight splits' partials loaded per round (one memory latency), added in split order
+    int sp = r;
+    for (; sp + 16 * 7 < nsplit; sp += 16 * 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = src[(long)(sp + 16 * u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; sp < nsplit; sp += 16) s += src[sp * stride];
   }
   red[el][r] = s;
   __syncthreads();
