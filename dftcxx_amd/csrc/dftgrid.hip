@@ -376,7 +376,15 @@ __global__ __launch_bounds__(1024) void k_cell_sum(const double* __restrict__ x,
   __shared__ double red[1024];
   const long base = (long)blockIdx.x * cell;
   double s = 0.0;
-  for (long i = threadIdx.x; i < cell; i += 1024) s += x[base + i] * y[base + i];
+  long i = threadIdx.x;
+  for (; i + 7 * 1024 < cell; i += 8 * 1024) {  // eight products per memory round, added in order
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = x[base + i + u * 1024] * y[base + i + u * 1024];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += t[u];
+  }
+  for (; i < cell; i += 1024) s += x[base + i] * y[base + i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int st = 512; st > 0; st >>= 1) {
@@ -395,7 +403,15 @@ __global__ __launch_bounds__(256) void k_cell_sum_split(const double* __restrict
   const int a = blockIdx.x / S, sp = blockIdx.x % S;
   const long base = (long)a * cell, lo = cell * sp / S, hi = cell * (sp + 1) / S;
   double s = 0.0;
-  for (long i = lo + threadIdx.x; i < hi; i += 256) s += x[base + i] * y[base + i];
+  long i = lo + threadIdx.x;
+  for (; i + 7 * 256 < hi; i += 8 * 256) {  // eight products per memory round, added in order
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = x[base + i + u * 256] * y[base + i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += t[u];
+  }
+  for (; i < hi; i += 256) s += x[base + i] * y[base + i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
