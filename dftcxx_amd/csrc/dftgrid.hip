@@ -3152,6 +3152,7 @@ int dg_correct_densities(dg_handle* h, double* nelec_before) {
 
 int dg_calculate_density(dg_handle* h, double* nelec) {
   if (!h || !nelec) return fail(DG_ERR_INVALID, "dg_calculate_density: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_calculate_density: not for a batch handle (use dg_batch_build)");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_calculate_density: no density set");
   DG_TRY(run(h, "cell_sum", [&] {
     hipLaunchKernelGGL(k_cell_sum, dim3(h->natoms), dim3(1024), 0, h->stream, h->rho, h->w, h->cell, h->q);
@@ -3166,6 +3167,7 @@ int dg_calculate_density(dg_handle* h, double* nelec) {
 
 int dg_hartree(dg_handle* h, double* J) {
   if (!h) return fail(DG_ERR_INVALID, "dg_hartree: null handle");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_hartree: not for a batch handle (use dg_batch_build)");
   if (!h->has_density) return fail(DG_ERR_STATE, "dg_hartree: no density set");
   // cell populations of the current density (the Poisson boundary values)
   DG_TRY(run(h, "cell_sum", [&] {

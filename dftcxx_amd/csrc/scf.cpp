@@ -654,8 +654,11 @@ class HostSCF {
     et_ = e_one_ + e_j_ + enuc_ + exc_;
   }
 
-  void fill(dg_scf_result* r, int iterations, double setup, double sec_iter) const {
+  void fill(dg_scf_result* r, int iterations, double setup, double sec_iter, bool converged,
+            double delta) const {
     r->iterations = iterations;
+    r->converged = converged ? 1 : 0;
+    r->delta = delta;
     r->energy = et_;
     r->e_one = e_one_;
     r->e_j = e_j_;
@@ -708,7 +711,9 @@ class DFT : public HostSCF {
   }
 
   // src/dft.cpp:84-142
-  int scf(double threshold, int min_iter, int max_iter, double* energies, double* sec_iter) {
+  // converged: the loop ended on |dE| <= threshold, not at max_iter (src/dft.cpp:126-131)
+  int scf(double threshold, int min_iter, int max_iter, double* energies, double* sec_iter, bool* converged,
+          double* delta) {
     double old_energy = et_, difference = 1.0;
     int iteration = 0;
     const auto t0 = std::chrono::steady_clock::now();
@@ -724,6 +729,8 @@ class DFT : public HostSCF {
     }
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (sec_iter) *sec_iter = iteration ? dt / iteration : 0.0;
+    *converged = difference <= threshold && iteration >= min_iter;
+    *delta = difference;
     return iteration;
   }
 
@@ -760,7 +767,7 @@ class DFT : public HostSCF {
 // matrix and results (its later builds are discarded), so its energy sequence
 // is the one a single-molecule run produces.
 int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& prm, int* iterations,
-              double* sec_iter) {
+              double* sec_iter, std::vector<char>& converged, std::vector<double>& diff) {
   const int nm = (int)mols.size();
   std::vector<int> nbf(nm);
   std::vector<long> off(nm + 1);
@@ -796,7 +803,9 @@ int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& 
   std::vector<char> active(nm, 1);
   densities(active);
   build(false, active);
-  std::vector<double> old(nm), diff(nm, 1.0);
+  std::vector<double> old(nm);
+  diff.assign(nm, 1.0);
+  converged.assign(nm, 0);
   for (int m = 0; m < nm; ++m) old[m] = mols[m].et_;
   int iteration = 0, done = 0;
   const auto t0 = std::chrono::steady_clock::now();
@@ -809,7 +818,8 @@ int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& 
       diff[m] = std::fabs(mols[m].et_ - old[m]);
       old[m] = mols[m].et_;
       iterations[m] = iteration;
-      if ((diff[m] <= prm.threshold && iteration >= prm.min_iter) || iteration >= prm.max_iter) {
+      converged[m] = diff[m] <= prm.threshold && iteration >= prm.min_iter;
+      if (converged[m] || iteration >= prm.max_iter) {
         active[m] = 0;
         ++done;
       }
@@ -841,6 +851,16 @@ dftcxx::MoleculeData to_data(const dg_molecule* mol, const dg_basis* basis) {
     if (m.shell_atom[s] < 0 || m.shell_atom[s] >= mol->natoms) throw std::runtime_error("shell atom out of range");
   }
   return m;
+}
+// The reference hard-codes these (src/dft.cpp:85, 95, 331); as parameters they are checked.
+bool check_scf_params(const dg_scf_params* p, const char* who) {
+  const char* bad = nullptr;
+  if (p->max_iter < 1) bad = "max_iter must be >= 1";
+  else if (p->min_iter < 0) bad = "min_iter must be >= 0";
+  else if (!std::isfinite(p->threshold) || p->threshold < 0.0) bad = "threshold must be finite and >= 0";
+  else if (!(p->mixing >= 0.0 && p->mixing < 1.0)) bad = "mixing must lie in [0, 1)";
+  if (bad) dg::set_last_error(std::string(who) + ": " + bad);
+  return bad == nullptr;
 }
 }  // namespace
 
@@ -881,6 +901,7 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     dg::set_last_error("dg_scf_run: null argument");
     return DG_ERR_INVALID;
   }
+  if (!check_scf_params(scf, "dg_scf_run")) return DG_ERR_INVALID;
   try {
     dftcxx::MoleculeData m = to_data(mol, basis);
     if (m.nelec() % 2 != 0) throw std::runtime_error("closed-shell SCF needs an even electron count");
@@ -889,9 +910,10 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     dft.mixing = scf->mixing;
     dft.construct_matrices();
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    double sec_iter = 0.0;
-    const int it = dft.scf(scf->threshold, scf->min_iter, scf->max_iter, energies, &sec_iter);
-    dft.fill(result, it, setup, sec_iter);
+    double sec_iter = 0.0, delta = 0.0;
+    bool converged = false;
+    const int it = dft.scf(scf->threshold, scf->min_iter, scf->max_iter, energies, &sec_iter, &converged, &delta);
+    dft.fill(result, it, setup, sec_iter, converged, delta);
     if (P_out) std::memcpy(P_out, dft.P_.data(), sizeof(double) * dft.P_.size());
     return DG_OK;
   } catch (const std::exception& e) {
@@ -907,6 +929,7 @@ extern "C" int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basi
     dg::set_last_error("dg_batch_scf_run: null or empty argument");
     return DG_ERR_INVALID;
   }
+  if (!check_scf_params(scf, "dg_batch_scf_run")) return DG_ERR_INVALID;
   dg_batch* batch = nullptr;
   try {
     const auto t0 = std::chrono::steady_clock::now();
@@ -923,8 +946,10 @@ extern "C" int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basi
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::vector<int> its(nmol, 0);
     double sec_iter = 0.0;
-    const int total = dftcxx::batch_scf(dfts, batch, *scf, its.data(), &sec_iter);
-    for (int m = 0; m < nmol; ++m) dfts[m].fill(&results[m], its[m], setup, sec_iter);
+    std::vector<char> conv;
+    std::vector<double> delta;
+    const int total = dftcxx::batch_scf(dfts, batch, *scf, its.data(), &sec_iter, conv, delta);
+    for (int m = 0; m < nmol; ++m) dfts[m].fill(&results[m], its[m], setup, sec_iter, conv[m] != 0, delta[m]);
     if (iterations) *iterations = total;
     dg_batch_destroy(batch);
     return DG_OK;

@@ -68,7 +68,7 @@ class _ScfResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("energy", C.c_double), ("e_one", C.c_double),
                 ("e_j", C.c_double), ("e_xc", C.c_double), ("e_nuc", C.c_double),
                 ("nelec", C.c_double), ("seconds_setup", C.c_double),
-                ("seconds_iter", C.c_double)]
+                ("seconds_iter", C.c_double), ("converged", C.c_int), ("delta", C.c_double)]
 
 
 def lib():
@@ -503,7 +503,7 @@ def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, m
     m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision, screen)
     sp = _ScfParams(threshold, min_iter, max_iter, mixing)
     res = _ScfResult()
-    en = np.zeros(max_iter)
+    en = np.zeros(max(int(max_iter), 1))
     n = mol.nbf()
     P = np.zeros((n, n))
     _check(lib().dg_scf_run(C.byref(m), C.byref(b), C.byref(g), C.byref(sp), C.byref(res), _d(en), _d(P)))

@@ -178,11 +178,17 @@ typedef struct {
   double nelec;         /* grid electron count after the last iteration */
   double seconds_setup; /* integrals + grid construction */
   double seconds_iter;  /* mean wall seconds per SCF iteration */
+  int converged;        /* 1 if |dE| <= threshold was reached after min_iter, 0 if the
+                         * loop stopped at max_iter instead (the reference prints
+                         * "Stopping because maximum number of iterations has been
+                         * reached", src/dft.cpp:126-131) */
+  double delta;         /* |dE| of the last iteration */
 } dg_scf_result;
 
 /* Runs the reference SCF with the grid work on the GPU.  energies (may be NULL)
  * receives the total energy of each iteration (max_iter doubles); P_out (may
- * be NULL) the final density matrix. */
+ * be NULL) the final density matrix.  DG_ERR_INVALID for max_iter < 1,
+ * min_iter < 0, a negative or non-finite threshold, or mixing outside [0, 1). */
 int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const dg_grid_params* grid,
                const dg_scf_params* scf, dg_scf_result* result, double* energies, double* P_out);
 

@@ -1204,37 +1204,64 @@ void or_eri_J(int natoms, const double* xyz, int nshell, const int* shell_atom, 
 
 void or_eigh(int n, const double* A, double* evals, double* evecs) { jacobi_eigh(n, A, evals, evecs); }
 
-int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
-           const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
-           const double* coef, double threshold, int min_iter, int max_iter, double mixing, double* energies,
-           double* terms, double* P_out) {
-  const int n = g->nbf;
-  const size_t nn = (size_t)n * n;
-  std::vector<double> S(nn), T(nn), Vn(nn), H(nn), X(nn), J(nn, 0.0), XC(nn, 0.0), P(nn, 0.0);
-  or_one_electron(natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef, S.data(),
-                  T.data(), Vn.data());
-  for (size_t i = 0; i < nn; i++) H[i] = T[i] + Vn[i];
-  double enuc = 0.0;  // src/dft.cpp:235-246
-  for (int i = 0; i < natoms; i++)
-    for (int j = i + 1; j < natoms; j++) {
-      Vec3 a{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, b{xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]};
-      enuc += Z[i] * Z[j] / norm(sub(a, b));
-    }
-  int nelec = 0;
-  for (int i = 0; i < natoms; i++) nelec += Z[i];
-  {  // canonical orthogonalisation, src/dft.cpp:300-316
+}  // extern "C"
+
+namespace {
+
+// The SCF state of DFT (src/dft.h:41-216): one-electron matrices, the
+// canonical orthogonalisation, the grid matrices and the energy terms.
+struct ScfCore {
+  or_grid* g;
+  int n = 0, nelec = 0;
+  size_t nn = 0;
+  std::vector<double> S, T, Vn, H, X, Xt, J, XC, P;
+  double enuc = 0.0, exc = 0.0, et = 0.0, e_one = 0.0, e_j = 0.0, c = 1.0;
+  double mixing = 0.5;
+  bool is_first = true;
+
+  ScfCore(or_grid* grid, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+          const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+          const double* coef)
+      : g(grid), n(grid->nbf), nn((size_t)grid->nbf * grid->nbf) {
+    S.assign(nn, 0.0);
+    T.assign(nn, 0.0);
+    Vn.assign(nn, 0.0);
+    H.assign(nn, 0.0);
+    X.assign(nn, 0.0);
+    Xt.assign(nn, 0.0);
+    J.assign(nn, 0.0);
+    XC.assign(nn, 0.0);
+    P.assign(nn, 0.0);
+    // DFT::construct_matrices, src/dft.cpp:184-201
+    or_one_electron(natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef,
+                    S.data(), T.data(), Vn.data());
+    for (size_t i = 0; i < nn; i++) H[i] = T[i] + Vn[i];
+    // DFT::calculate_nuclear_repulsion, src/dft.cpp:235-246
+    for (int i = 0; i < natoms; i++)
+      for (int j = i + 1; j < natoms; j++) {
+        Vec3 a{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, b{xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]};
+        enuc += Z[i] * Z[j] / norm(sub(a, b));
+      }
+    for (int i = 0; i < natoms; i++) nelec += Z[i];
+    // canonical orthogonalisation, src/dft.cpp:300-316
     std::vector<double> ev(n), U(nn);
     jacobi_eigh(n, S.data(), ev.data(), U.data());
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) X[i + (size_t)j * n] = U[i + (size_t)j * n] * (1.0 / std::sqrt(ev[j]));
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) Xt[i + (size_t)j * n] = X[j + (size_t)i * n];
   }
-  bool is_first = true;
-  double exc = 0.0, et = 0.0, e_one = 0.0, e_j = 0.0;
-  std::vector<double> F(nn), tmp(nn), Fp(nn), Cc(nn), C(nn), ev(n), Xt(nn);
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) Xt[i + (size_t)j * n] = X[j + (size_t)i * n];
-  auto density_matrix = [&]() {  // src/dft.cpp:330-366
+
+  // F = H + 2J + XC, src/dft.cpp:334
+  void fock(std::vector<double>& F) const {
     for (size_t i = 0; i < nn; i++) F[i] = H[i] + 2.0 * J[i] + XC[i];
+  }
+
+  // DFT::calculate_density_matrix, src/dft.cpp:330-366 (P mixed with the old P
+  // after the first pass), without its grid calls
+  void density_matrix() {
+    std::vector<double> F(nn), tmp(nn), Fp(nn), Cc(nn), C(nn), ev(n);
+    fock(F);
     matmul(n, Xt.data(), F.data(), tmp.data());
     matmul(n, tmp.data(), X.data(), Fp.data());
     jacobi_eigh(n, Fp.data(), ev.data(), Cc.data());
@@ -1249,10 +1276,27 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, 
     } else {
       for (size_t i = 0; i < nn; i++) P[i] = (1.0 - mixing) * Pnew[i] + mixing * P[i];
     }
+  }
+
+  // set_density + correct_densities (src/dft.cpp:362-365); hold_c > 0 scales
+  // by that factor instead of the renormalisation's own
+  void grid_density(double hold_c) {
     or_grid_set_density(g, P.data());
-    or_grid_correct_densities(g);
-  };
-  auto energy = [&]() {  // src/dft.cpp:441-447
+    if (hold_c > 0.0) {
+      for (double& v : g->rho) v *= hold_c;
+      std::fill(g->cell_cached.begin(), g->cell_cached.end(), 0);
+      c = hold_c;
+    } else {
+      const double before = or_grid_calculate_density(g);
+      or_grid_correct_densities(g);
+      double charge = 0.0;
+      for (int a = 0; a < g->natoms; a++) charge += g->Z[a];
+      c = charge / before;
+    }
+  }
+
+  // DFT::calculate_energy, src/dft.cpp:441-447
+  void energy() {
     double t1 = 0.0, t2 = 0.0;
     for (int i = 0; i < n; i++)
       for (int k = 0; k < n; k++) {
@@ -1262,32 +1306,75 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, 
     e_one = 2.0 * t1;
     e_j = 2.0 * t2;
     et = e_one + e_j + enuc + exc;
-  };
-  density_matrix();
-  or_grid_hartree(g, J.data());
-  energy();
-  double old_energy = et, difference = 1.0;
+  }
+
+  void terms(double* t) const {
+    t[0] = et;
+    t[1] = e_one;
+    t[2] = e_j;
+    t[3] = exc;
+    t[4] = enuc;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+           const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+           const double* coef, double threshold, int min_iter, int max_iter, double mixing, double* energies,
+           double* terms, double* P_out) {
+  ScfCore s(g, natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef);
+  s.mixing = mixing;
+  // DFT::DFT -> construct_matrices: the core guess, J of it, the energy (E_xc = 0, Q3)
+  s.density_matrix();
+  s.grid_density(0.0);
+  or_grid_hartree(g, s.J.data());
+  s.energy();
+  // DFT::scf, src/dft.cpp:84-142
+  double old_energy = s.et, difference = 1.0;
   int iteration = 0;
   while (difference > threshold || iteration < min_iter) {
     iteration++;
-    density_matrix();
-    or_grid_hartree(g, J.data());
-    or_grid_xc(g, XC.data(), &exc);
-    energy();
-    if (energies) energies[iteration - 1] = et;
-    difference = std::fabs(et - old_energy);
-    old_energy = et;
+    s.density_matrix();
+    s.grid_density(0.0);
+    or_grid_hartree(g, s.J.data());
+    or_grid_xc(g, s.XC.data(), &s.exc);
+    s.energy();
+    if (energies) energies[iteration - 1] = s.et;
+    difference = std::fabs(s.et - old_energy);
+    old_energy = s.et;
     if (iteration >= max_iter) break;
   }
-  if (terms) {
-    terms[0] = et;
-    terms[1] = e_one;
-    terms[2] = e_j;
-    terms[3] = exc;
-    terms[4] = enuc;
-  }
-  if (P_out) std::memcpy(P_out, P.data(), sizeof(double) * nn);
+  if (terms) s.terms(terms);
+  if (P_out) std::memcpy(P_out, s.P.data(), sizeof(double) * s.nn);
   return iteration;
+}
+
+int or_scf_evaluate(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+                    const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+                    const double* coef, const double* P, double hold_c, double* terms, double* F, double* H,
+                    double* J, double* XC) {
+  ScfCore s(g, natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef);
+  std::memcpy(s.P.data(), P, sizeof(double) * s.nn);
+  s.grid_density(hold_c);
+  or_grid_hartree(g, s.J.data());
+  or_grid_xc(g, s.XC.data(), &s.exc);
+  s.energy();
+  if (terms) {
+    s.terms(terms);
+    terms[5] = s.c;
+  }
+  if (F) {
+    std::vector<double> f(s.nn);
+    s.fock(f);
+    std::memcpy(F, f.data(), sizeof(double) * s.nn);
+  }
+  if (H) std::memcpy(H, s.H.data(), sizeof(double) * s.nn);
+  if (J) std::memcpy(J, s.J.data(), sizeof(double) * s.nn);
+  if (XC) std::memcpy(XC, s.XC.data(), sizeof(double) * s.nn);
+  return 0;
 }
 
 }  // extern "C"

@@ -96,6 +96,17 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz,
            const double* alpha, const double* coef,
            double threshold, int min_iter, int max_iter, double mixing,
            double* energies, double* terms, double* P_out);
+/* One evaluation of the SCF's grid work and energy at a given P, by the same
+ * code or_scf runs each iteration: set_density, correct_densities (or, with
+ * hold_c > 0, rho scaled by hold_c instead), Hartree J, XC, calculate_energy.
+ * terms = {E_total, E_one, E_J, E_xc, E_nuc, c}; F = H + 2J + XC
+ * (src/dft.cpp:334).  Any output may be NULL. */
+int or_scf_evaluate(or_grid* g, int natoms, const int* Z, const double* xyz,
+                    int nshell, const int* shell_atom, const int* shell_L,
+                    const int* shell_nprim, const int* shell_offset,
+                    const double* alpha, const double* coef,
+                    const double* P, double hold_c, double* terms,
+                    double* F, double* H, double* J, double* XC);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,9 @@ def lib():
         L.or_scf.argtypes = [C.c_void_p, C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
                              C.c_double, C.c_int, C.c_int, C.c_double, _D, _D, _D]
         L.or_scf.restype = C.c_int
+        L.or_scf_evaluate.argtypes = [C.c_void_p, C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
+                                      _D, C.c_double, _D, _D, _D, _D, _D]
+        L.or_scf_evaluate.restype = C.c_int
         _lib = L
     return _lib
 
@@ -192,6 +195,22 @@ class Grid:
                           _d(en), _d(terms), _d(P))
         return en[:it], dict(zip(("total", "one", "J", "xc", "nuc"), terms)), P
 
+    def evaluate(self, P, hold_c=0.0):
+        """The SCF's grid work and energy at P (the code or_scf runs per iteration).
+
+        hold_c > 0 scales rho by that factor instead of renormalising it.
+        Returns (terms dict with total, one, J, xc, nuc, c; F; H; J; XC)."""
+        Z, xyz, at, L, npr, off, al, co = self._keep
+        n = self.nbf
+        Pc = _f64(np.asarray(P).T)
+        terms = np.zeros(6)
+        F, H, J, XC = (np.empty((n, n)) for _ in range(4))
+        lib().or_scf_evaluate(self.h, self.mol.natoms, _i(Z), _d(xyz), len(at), _i(at), _i(L), _i(npr),
+                              _i(off), _d(al), _d(co), _d(Pc), float(hold_c), _d(terms), _d(F), _d(H),
+                              _d(J), _d(XC))
+        t = dict(zip(("total", "one", "J", "xc", "nuc", "c"), terms))
+        return t, F.T.copy(), H.T.copy(), J.T.copy(), XC.T.copy()
+
 
 def one_electron(mol):
     s = mol.shells()
@@ -218,12 +237,13 @@ def eri_J(mol, P):
 
 
 def eigh(A):
-    A = _f64(A)
+    """Ascending eigenvalues and eigenvectors (columns) of a symmetric matrix."""
+    A = _f64(np.asarray(A).T)
     n = A.shape[0]
     w = np.empty(n)
-    V = np.empty((n, n))
+    V = np.empty((n, n))  # filled column-major
     lib().or_eigh(n, _d(A), _d(w), _d(V))
-    return w, V
+    return w, V.T.copy()
 
 
 def lda(rho):

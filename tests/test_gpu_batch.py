@@ -101,9 +101,20 @@ def test_batch_of_256_full_grid():
 
 
 def test_batch_rejects_single_molecule_calls():
-    b = G.Batch(M.batch(count=2, seed=3), 10, 26, 4)
+    mols = M.batch(count=2, seed=3)
+    b = G.Batch(mols, 10, 26, 4)
     with pytest.raises(G.DgError):
         G._check(G.lib().dg_set_density(b.h, G._d(np.zeros(4)), 2))
+    # after a batch build the handle holds a density; its single-molecule calls
+    # still return DG_ERR_STATE before any launch
+    b.build([_guess(m) for m in mols])
+    L = G.lib()
+    out = np.zeros(64 * 64)
+    x = np.zeros(1)
+    for rc in (L.dg_hartree(b.h, G._d(out)), L.dg_calculate_density(b.h, G._d(x)),
+               L.dg_xc(b.h, G._d(out), G._d(x)), L.dg_correct_densities(b.h, G._d(x)),
+               L.dg_build_device(b.h, 1, 0, 0, 0)):
+        assert rc == 4 and "batch handle" in L.dg_last_error().decode()
 
 
 def test_batch_scf_matches_single_scf():

@@ -263,3 +263,29 @@ def test_host_integrals_independent_of_thread_count():
         assert r.returncode == 0, r.stderr
         out.append(r.stdout.strip())
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("max_iter", 0, "max_iter"),
+    ("min_iter", -1, "min_iter"),
+    ("threshold", float("nan"), "threshold"),
+    ("threshold", -1e-6, "threshold"),
+    ("mixing", 1.0, "mixing"),
+    ("mixing", -0.1, "mixing"),
+])
+def test_scf_rejects_invalid_params(field, value, msg):
+    # the SCF parameters are checked before any device call, so this holds without a GPU
+    import ctypes as C
+    mol = M.h2o()
+    m, b, g, keep = G._c_inputs(mol, 20, 26, 4, G.PREC_FP64)
+    kw = dict(threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5)
+    kw[field] = value
+    sp = G._ScfParams(kw["threshold"], kw["min_iter"], kw["max_iter"], kw["mixing"])
+    res = G._ScfResult()
+    rc = G.lib().dg_scf_run(C.byref(m), C.byref(b), C.byref(g), C.byref(sp), C.byref(res), None, None)
+    assert rc == 1 and msg in G.lib().dg_last_error().decode()
+    rs = (G._ScfResult * 1)()
+    ms = (G._Molecule * 1)(m)
+    bs = (G._Basis * 1)(b)
+    rc = G.lib().dg_batch_scf_run(1, ms, bs, C.byref(g), C.byref(sp), rs, None)
+    assert rc == 1 and msg in G.lib().dg_last_error().decode()

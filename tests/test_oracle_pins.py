@@ -393,3 +393,154 @@ def test_total_energy_translation_invariant(oracle_lib):
         en, terms, P = g.scf(threshold=1e-6, max_iter=40)
         e.append(terms["total"])
     assert abs(e[0] - e[1]) < 1e-9
+
+
+# --- a16: SCF energy assembly (src/dft.cpp:84-142, 330-366, 441-447) --------------------------
+#
+# E = 2 tr(PH) + 2 tr(PJ) + E_nuc + E_xc and F = H + 2J + XC.  Each term is pinned
+# on its own (closed forms, grid identities, a library eigen-solver), and F
+# against E through the derivative: with the renormalisation factor c held,
+# dE/dP = 2 (H + 2J + c XC), which is 2F up to (c - 1) XC and the small
+# asymmetry of the numerical Hartree operator.
+
+def _oracle_h2o(nrad=50, nang=110, lmax=8):
+    from oracle import oracle as O
+    mol = M.h2o("sto3g")
+    g = O.Grid(mol, nrad, nang, lmax)
+    S, T, V = O.one_electron(mol)
+    return O, mol, g, core_guess(mol, S, T, V)
+
+
+def test_scf_energy_gradient_is_twice_the_fock_matrix(oracle_lib):
+    O, mol, g, P0 = _oracle_h2o()
+    t0, F, H, J, XC = g.evaluate(P0)
+    c0 = t0["c"]
+    assert abs(c0 - 1.0) < 1e-4
+    rng = np.random.default_rng(1)
+    checks = []
+    for _ in range(3):
+        D = rng.standard_normal(P0.shape)
+        D = 1e-3 * (D + D.T)
+        ep = g.evaluate(P0 + D, hold_c=c0)[0]["total"]
+        em = g.evaluate(P0 - D, hold_c=c0)[0]["total"]
+        fd = (ep - em) / 2
+        checks.append((fd, D))
+        assert abs(fd - np.sum(D * 2 * F)) < 2e-3 * abs(fd), (fd, np.sum(D * 2 * F))
+    # a dropped factor on J or XC, or a doubled H, in either E or F would fail the
+    # comparison above by far more than its tolerance
+    for wrong in (H + J + XC, H + 2 * J + 2 * XC, 2 * H + 2 * J + XC, H + 2 * J):
+        assert max(abs(fd - np.sum(D * 2 * wrong)) / abs(fd) for fd, D in checks) > 0.05
+
+
+def test_scf_hartree_energy_equals_pointwise_integral(oracle_lib):
+    # 2 tr(PJ) with J = (1/2) Phi^T diag(w V) Phi is sum_p w_p V_p rho_p / (2c): the
+    # same number through the grid values instead of the matrix
+    O, mol, g, P0 = _oracle_h2o(30, 50, 4)
+    t, F, H, J, XC = g.evaluate(P0)
+    _, w, _ = g.points()
+    V = g.hartree_parts()["V"]
+    rho = g.densities()
+    assert abs(t["J"] - np.sum(w * V * rho) / (2 * t["c"])) < 1e-11 * t["J"]
+    assert abs(t["J"] - 2 * np.sum(P0 * J)) < 1e-12 * t["J"]
+    # E_one from the closed-form-pinned one-electron matrices, E_xc from the grid
+    S, T, Vn = O.one_electron(mol)
+    assert abs(t["one"] - 2 * np.sum(P0 * (T + Vn))) < 1e-12 * abs(t["one"])
+    exd, _ = g.xc_parts()
+    assert abs(t["xc"] - np.sum(w * exd)) < 1e-12 * abs(t["xc"])
+    assert abs(t["total"] - (t["one"] + t["J"] + t["xc"] + t["nuc"])) < 1e-12 * abs(t["total"])
+
+
+def test_scf_nuclear_repulsion(oracle_lib):
+    from oracle import oracle as O
+    R = 1.4
+    h2 = M.Molecule([1, 1], [[0, 0, 0], [0, 0, R]], "sto3g")
+    g = O.Grid(h2, 10, 26, 2)
+    _, terms, _ = g.scf(threshold=1.0, min_iter=0, max_iter=1)
+    assert abs(terms["nuc"] - 1.0 / R) < 1e-15
+    mol = M.h2o("sto3g")
+    g = O.Grid(mol, 10, 26, 2)
+    _, terms, _ = g.scf(threshold=1.0, min_iter=0, max_iter=1)
+    Z, x = np.array(mol.Z, float), mol.xyz
+    e = sum(Z[i] * Z[j] / np.linalg.norm(x[i] - x[j]) for i in range(3) for j in range(i + 1, 3))
+    assert abs(terms["nuc"] - e) < 1e-13 * e
+
+
+def test_scf_fixed_point_diagonalises_fock(oracle_lib):
+    # at convergence the mixed P is a fixed point of its own aufbau step: idempotent
+    # in the S metric, nelec/2 electrons, and commuting with F (F P S = S P F)
+    O, mol, g, _ = _oracle_h2o(30, 50, 4)
+    en, terms, P = g.scf(threshold=1e-11, min_iter=3, max_iter=200)
+    assert abs(en[-1] - en[-2]) <= 1e-11
+    t, F, H, J, XC = g.evaluate(P)
+    assert abs(t["total"] - terms["total"]) < 1e-9
+    S, _, _ = O.one_electron(mol)
+    assert np.abs(P @ S @ P - P).max() < 1e-5
+    assert abs(np.trace(P @ S) - mol.nelec / 2) < 1e-5
+    comm = F @ P @ S - S @ P @ F
+    assert np.abs(comm).max() < 1e-5 * np.abs(F).max()
+
+
+def test_oracle_eigh_matches_numpy(oracle_lib):
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 7, 20):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        w, V = O.eigh(A)
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * max(1.0, np.abs(w).max())
+        assert np.all(np.diff(w) >= 0)
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+        assert np.abs(A @ V - V * w).max() < 1e-11 * max(1.0, np.abs(w).max())
+    # degenerate spectrum: the eigenvalues, and the projector on each cluster
+    Q, _ = np.linalg.qr(rng.standard_normal((7, 7)))
+    lam = np.array([-1.0, -1.0, 0.5, 2.0, 2.0, 2.0, 3.0])
+    A = Q @ np.diag(lam) @ Q.T
+    w, V = O.eigh(A)
+    assert np.abs(w - lam).max() < 1e-13
+    for sel in ([0, 1], [2], [3, 4, 5], [6]):
+        Pm = V[:, sel] @ V[:, sel].T
+        Qm = Q[:, sel] @ Q[:, sel].T
+        assert np.abs(Pm - Qm).max() < 1e-12
+
+
+def _prim(Z, xyz, L, alpha):
+    return M.Molecule(Z, xyz, {z: [(L, [alpha], [1.0])] for z in set(Z)})
+
+
+@pytest.mark.parametrize("alpha", [0.3, 1.0, 4.7])
+def test_one_electron_closed_forms(oracle_lib, alpha):
+    from oracle import oracle as O
+    Z = 7
+    # one normalised s primitive on its own nucleus: T = 3a/2, V = -2Z sqrt(2a/pi) F_0(0),
+    # where the Boys argument 0 is clamped to 1e-8 (src/gamma.cpp:42-43)
+    f0 = math.sqrt(math.pi) * math.erf(1e-4) / (2 * 1e-4)
+    S, T, V = O.one_electron(_prim([Z], [[0.1, -0.2, 0.3]], 0, alpha))
+    assert abs(S[0, 0] - 1.0) < 1e-11
+    assert abs(T[0, 0] - 1.5 * alpha) < 1e-11 * alpha
+    v_ss = -2 * Z * math.sqrt(2 * alpha / math.pi)
+    assert abs(V[0, 0] - v_ss * f0) < 1e-11 * abs(v_ss)
+    # p primitives: T = 5a/2, V = -(4/3) Z sqrt(2a/pi) (to the clamp's 1e-8), the three
+    # components orthogonal
+    S, T, V = O.one_electron(_prim([Z], [[0.0, 0.0, 0.0]], 1, alpha))
+    assert np.abs(S - np.eye(3)).max() < 1e-11
+    assert np.abs(T - 2.5 * alpha * np.eye(3)).max() < 1e-11 * alpha
+    v_pp = -4 / 3 * Z * math.sqrt(2 * alpha / math.pi)
+    assert np.abs(V - v_pp * np.eye(3)).max() < 1e-8 * abs(v_pp)
+    # two s primitives on two nuclei: overlap and nuclear attraction in closed form
+    # (Gaussian product theorem, F_0(t) = sqrt(pi/t) erf(sqrt t) / 2)
+    b, R = 0.8, 1.3
+    mol = M.Molecule([1, 3], [[0, 0, 0], [0, 0, R]], {1: [(0, [alpha], [1.0])], 3: [(0, [b], [1.0])]})
+    S, T, V = O.one_electron(mol)
+    g = alpha + b
+    K = math.exp(-alpha * b * R * R / g)
+    assert abs(S[0, 1] - (2 * math.sqrt(alpha * b) / g) ** 1.5 * K) < 1e-12
+    Na, Nb = (2 * alpha / math.pi) ** 0.75, (2 * b / math.pi) ** 0.75
+    Pz = b * R / g
+    v = 0.0
+    for Zc, zc in ((1, 0.0), (3, R)):
+        t = g * (Pz - zc) ** 2
+        f0 = 1.0 if t < 1e-15 else 0.5 * math.sqrt(math.pi / t) * math.erf(math.sqrt(t))
+        v -= Zc * 2 * math.pi / g * K * Na * Nb * f0
+    assert abs(V[0, 1] - v) < 1e-10 * abs(v)
+    # kinetic: <a|-1/2 lap|b> = ab/g (3 - 2 ab R^2/g) S_ab
+    assert abs(T[0, 1] - alpha * b / g * (3 - 2 * alpha * b * R * R / g) * S[0, 1]) < 1e-12
