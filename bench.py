@@ -4,14 +4,17 @@ A step is one grid build on synthetic input already resident in HBM: from the
 density matrix P to rho, its renormalisation, Slater+VWN, the Becke-Poisson
 Hartree potential, J = Phi^T diag(w V/2) Phi and XC = Phi^T diag(w v_xc) Phi
 (the reference's per-iteration grid work, src/dft.cpp:330-433).  The metric is
-N_pts * N_bf^2 per second.  Default workload: benzene, 6-31G (66 CGFs),
-12 atoms x 75 radial x 302 Lebedev points, lmax 8; P is the core-Hamiltonian
-guess.
+N_pts * N_bf^2 per second.  Default workload: linear C30H62, 6-31G (394 CGFs),
+92 atoms x 75 radial x 302 Lebedev points (2,083,800 points), lmax 8; P is the
+core-Hamiltonian guess.
 
 Besides the throughput line it reports, for the kernel with the largest share
 of the build, its achieved rate against the MI355X peak (timed with HIP events
-on the stream the kernel runs on), and the reference-faithful CPU restatement
-timed on this host on one full build of the same workload.
+on the stream the kernel runs on), the screened and reduced-precision builds,
+SCF energies and s/iteration, the adjoint, and the reference-faithful CPU
+restatement timed on this host (on the whole build for grids up to 300k
+points, else on the points of one atomic cell with the per-stage times scaled
+to the whole grid).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -26,35 +29,88 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 HARTREE_FLOPS_PER_TERM = 10.0
+CPU_SAMPLE_MAX_POINTS = 300000
 
 
 def kernel_work(name, mol_info):
-    """Algorithmic flops of one launch of a build kernel (None if not a compute kernel)."""
+    """(algorithmic flops of one launch, the unit that bounds it) of a build kernel,
+    or (None, None) for kernels without a flop model.
+
+    One convention throughout: the symmetric products count only the pairs
+    i <= j (the work the result needs), 2 flops per multiply-add; the Hartree
+    cross term counts 10 flops per (point, atom, lm) spline + harmonic term."""
     npts, nbf, nat, nlm = mol_info["npts"], mol_info["nbf"], mol_info["natoms"], mol_info["nlm"]
-    if name == "hartree_cross":
-        return HARTREE_FLOPS_PER_TERM * npts * (nat - 1) * nlm / mol_info["cross_launches"]
-    # the symmetric products need only the pairs i <= j: one FMA per point and pair
-    if name == "contract":
-        return 2 * 2.0 * npts * nbf * (nbf + 1) / 2  # J and XC
+    if name == "hartree_cross":  # FP64 vector ALU: the kernel issues no MFMA
+        return HARTREE_FLOPS_PER_TERM * npts * (nat - 1) * nlm / mol_info["cross_launches"], "valu_fp64"
+    if name == "contract":  # J and XC, FP64 (or FP32 in mixed mode) MFMA
+        return 2 * 2.0 * npts * nbf * (nbf + 1) / 2, "mfma"
     if name == "density":
-        return 2.0 * npts * nbf * (nbf + 1) / 2 + 2.0 * npts * nbf
-    return None
+        return 2.0 * npts * nbf * (nbf + 1) / 2 + 2.0 * npts * nbf, "mfma"
+    return None, None
 
 
-def cpu_baseline(mol, nrad, nang, lmax, P):
-    """Time the oracle (reference-faithful FP64 CPU restatement) on one full build."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mol, nrad, nang, lmax, P, units):
+    """Time the oracle (reference-faithful FP64 CPU restatement) on this host.
+
+    Grids up to CPU_SAMPLE_MAX_POINTS are timed whole.  Larger ones are timed on
+    the points of the first atomic cell (all atoms still enter the Becke weights,
+    Poisson solves and cross-atom potential) and each stage is scaled to the
+    whole grid by how the reference's loops run: the density loop is OpenMP over
+    cells (rounds of `threads` cells), the XC loop is serial over all points,
+    the projection, cross-atom potential and J scale with the number of cells,
+    and the per-atom Poisson solves are already done for every atom."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    g = O.Grid(mol, nrad, nang, lmax)
+    nat = mol.natoms
+    whole = nat * nrad * nang <= CPU_SAMPLE_MAX_POINTS
+    ncell = nat if whole else 1
+    g = O.Grid(mol, nrad, nang, lmax, ncell=ncell)
+    t = {}
     t0 = time.perf_counter()
     g.set_density(P)
+    t1 = time.perf_counter()
     g.correct_densities()
+    t2 = time.perf_counter()
     g.hartree()
+    t3 = time.perf_counter()
     g.xc()
-    dt = time.perf_counter() - t0
-    return dt, threads
+    t4 = time.perf_counter()
+    th = g.hartree_times()
+    t = {"density": t1 - t0, "correct": t2 - t1, "rho_lm": th[0], "poisson_vcell_splines": th[1],
+         "cross_potential": th[2], "J": th[3], "xc": t4 - t3}
+    if whole:
+        total = t4 - t0
+        sample = f"one full grid build of {nat} cells ({total:.1f} s)"
+        scaled = None
+    else:
+        f = nat / ncell
+        rounds = -(-nat // threads)
+        scaled = {"density": t["density"] * rounds, "correct": t["correct"] * f, "rho_lm": th[0] * f,
+                  "poisson_vcell_splines": th[1], "cross_potential": th[2] * f, "J": th[3] * f,
+                  "xc": t["xc"] * f}
+        total = sum(scaled.values())
+        sample = (f"per-iteration build on the {nrad * nang} points of {ncell} of {nat} cells "
+                  f"({t4 - t0:.1f} s), stage times scaled to the whole grid ({total:.0f} s)")
+    out = {"value": units / total, "unit": "pts*bf^2/s", "cores": threads, "kind": "port",
+           "sample": sample, "cpu_model": cpu_model(), "seconds_per_build": total,
+           "sample_stage_seconds": t}
+    if scaled is not None:
+        out["scaled_stage_seconds"] = scaled
+    return out
 
 
 def bench_batch(args, G, M, np, torch):
@@ -116,9 +172,16 @@ def bench_batch(args, G, M, np, torch):
         out["kernels_ms_per_step"] = {k: ms / 3 for k, (ms, _) in sorted(prof.items())}
     if not args.no_scf:
         res, its = G.batch_scf_run(mols, nrad, nang, lmax, threshold=1e-6)
+        conv = [r["converged"] == 1 for r in res]
+        its_conv = [r["iterations"] for r, c in zip(res, conv) if c]
+        # wall time per batch iteration up to the last molecule that converged
+        # (iterations after it only carry molecules that stopped at max_iter)
         out["scf"] = {"batch_iterations": its, "s_per_iter": res[0]["seconds_iter"],
-                      "setup_s": res[0]["seconds_setup"],
-                      "max_molecule_iterations": max(r["iterations"] for r in res)}
+                      "setup_s": res[0]["seconds_setup"], "converged": int(sum(conv)),
+                      "not_converged": [{"molecule": m, "name": mols[m].name, "basis": mols[m].basis,
+                                         "iterations": res[m]["iterations"], "last_dE": res[m]["delta"]}
+                                        for m in range(len(res)) if not conv[m]],
+                      "max_converged_iterations": max(its_conv) if its_conv else None}
     print(json.dumps(out))
 
 
@@ -127,7 +190,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="benzene_631g")
+    ap.add_argument("--workload", default="c30h62_631g")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--screen", type=float, default=0.0,
                     help="amplitude screening threshold of the timed build (0 = dense, as the reference)")
@@ -253,19 +316,20 @@ def main():
         dom = max(prof.items(), key=lambda kv: kv[1][0])[0]
         ms, cnt = prof[dom]
         avg_s = ms / cnt / 1e3
-        work = kernel_work(dom, info)
+        work, bound = kernel_work(dom, info)
         roof = {"kernel": dom, "share_of_build": prof[dom][0] / total if total else None}
         if work is not None:
             ach = work / avg_s / 1e12
-            roof.update({"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
-                         "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS})
+            peak = PEAK_FP32_MFMA_TFLOPS if (dom == "contract" and prec == G.PREC_MIXED) else PEAK_FP64_TFLOPS
+            roof.update({"bound": bound, "achieved": ach, "peak": peak,
+                         "unit": "TFLOP/s", "frac": ach / peak, "flops_per_launch": work})
         else:
             roof.update({"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": None})
         roof["avg_launch_ms"] = avg_s * 1e3
         roof["traffic"] = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-        if args.precision != "fp64":
+        if args.precision != "fp64" or args.screen != 0.0:
             tpath = ""
         if os.path.exists(tpath):
             with open(tpath) as f:
@@ -310,6 +374,7 @@ def main():
                           "max_abs_sum_grad": float(np.abs(grad.sum(0)).max())}
 
     if not args.no_mixed and prec == G.PREC_FP64 and args.screen == 0.0:
+        wall_screened = None
         # the same build screened in FP64, and with FP32-MFMA Fock contractions (screened and not)
         J6, X6 = bufs[1].cpu().numpy(), bufs[2].cpu().numpy()
         out["variants"] = {}
@@ -317,11 +382,14 @@ def main():
                              ("mixed", G.PREC_MIXED, 0.0),
                              ("mixed_screened", G.PREC_MIXED, args.variant_screen)):
             gv, pv, bv, wv, dv, _ = timed_builds(pr, sc)
+            if name == "fp64_screened":
+                wall_screened = wv
             Jv, Xv = bv[1].cpu().numpy(), bv[2].cpu().numpy()
             out["variants"][name] = {
                 "precision": "fp64" if pr == G.PREC_FP64 else "mixed", "screen": sc,
                 "value": units * args.steps / wv, "ms_per_step": 1e3 * wv / args.steps,
                 "speedup_vs_dense_fp64": wall / wv,
+                "speedup_vs_screened_fp64": wall_screened / wv if wall_screened else None,
                 "J_max_rel_err": float(np.abs(Jv - J6).max() / np.abs(J6).max()),
                 "XC_max_rel_err": float(np.abs(Xv - X6).max() / np.abs(X6).max()),
                 "e_xc_err": float(bv[3].cpu().numpy()[0] - scal[0]),
@@ -338,23 +406,22 @@ def main():
 
     if not args.no_scf:
         scf = {}
-        for name, pr, sc in (("fp64", G.PREC_FP64, 0.0), ("mixed", G.PREC_MIXED, 0.0),
+        for name, pr, sc in (("fp64", G.PREC_FP64, 0.0), ("fp64_screened", G.PREC_FP64, args.variant_screen),
+                             ("mixed", G.PREC_MIXED, 0.0),
                              ("mixed_screened", G.PREC_MIXED, args.variant_screen)):
             if name != "fp64" and args.no_mixed:
                 continue
             en, res, _ = G.scf_run(mol, nrad, nang, lmax, threshold=1e-6, precision=pr, screen=sc)
             scf[name] = {"energy_Ha": res["energy"], "iterations": res["iterations"],
+                         "converged": bool(res["converged"]), "last_dE": res["delta"],
                          "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"]}
-        for name in ("mixed", "mixed_screened"):
+        for name in ("fp64_screened", "mixed", "mixed_screened"):
             if name in scf:
                 scf[name]["energy_error_vs_fp64_Ha"] = scf[name]["energy_Ha"] - scf["fp64"]["energy_Ha"]
         out["scf"] = scf
 
     if not args.no_cpu_baseline:
-        dt, threads = cpu_baseline(mol, nrad, nang, lmax, P)
-        out["cpu_baseline"] = {"value": units / dt, "unit": "pts*bf^2/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"one full grid build of {args.workload} ({dt:.1f} s)"}
+        out["cpu_baseline"] = cpu_baseline(mol, nrad, nang, lmax, P, units)
     print(json.dumps(out))
 
 

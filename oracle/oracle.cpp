@@ -15,6 +15,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -444,6 +445,10 @@ double cutoff(double mu) { return 0.5 * (1.0 - fk(3, mu)); }
 // ---------------------------------------------------------------------------
 struct or_grid {
   int natoms = 0, nbf = 0, nrad = 0, nang = 0, lmax = 0, nlm = 0;
+  // cells that carry points: all natoms, or the first ncell of them when a
+  // timing sample builds only part of the grid (every atom still enters the
+  // Becke partition and the cross-atom potential)
+  int ncell = 0;
   std::vector<Vec3> atoms;
   std::vector<int> Z;
   std::vector<CGF> cgfs;
@@ -460,7 +465,10 @@ struct or_grid {
   std::vector<std::vector<Spline>> splines;    // natoms x nlm
   // XC state
   std::vector<double> exc_density, wva;
-  long npts() const { return (long)natoms * nrad * nang; }
+  // wall seconds of the last or_grid_hartree: projection, Poisson + own cell +
+  // splines, cross-atom potential, J (for CPU timing samples)
+  double t_hartree[4] = {0.0, 0.0, 0.0, 0.0};
+  long npts() const { return (long)ncell * nrad * nang; }
   long cell() const { return (long)nrad * nang; }
 };
 
@@ -522,6 +530,7 @@ void calculate_rho_lm(or_grid* g, int a) {
   const long base = (long)a * g->cell();
   double* rho_lm = &g->rho_lm[(size_t)a * g->nrad * g->nlm];
   for (int i = 0; i < g->nrad * g->nlm; i++) rho_lm[i] = 0.0;
+  if (a >= g->ncell) return;  // timing sample: a cell without points
 #pragma omp parallel for
   for (int i = 0; i < g->nrad; i++) {
     int cnt = 0;
@@ -551,7 +560,7 @@ void calculate_rho_lm(or_grid* g, int a) {
 void calculate_U_lm(or_grid* g, int a) {
   const int N = g->nrad;
   static const double sqrt4pi = std::sqrt(4.0 * M_PI);
-  const double q_n = cell_density(g, a);
+  const double q_n = a < g->ncell ? cell_density(g, a) : 0.0;
   std::vector<double> A;
   build_fd_matrix(N, g->r_n.data(), A);
   const double* rho_lm = &g->rho_lm[(size_t)a * N * g->nlm];
@@ -583,9 +592,10 @@ void calculate_U_lm(or_grid* g, int a) {
     }
   }
   const long base = (long)a * g->cell();
-  for (long i = 0; i < g->cell(); i++) g->vcell[base + i] = 0.0;
+  const int nshell_pts = a < g->ncell ? N : 0;  // timing sample: a cell without points
+  for (long i = 0; i < (long)nshell_pts * g->nang; i++) g->vcell[base + i] = 0.0;
 #pragma omp parallel for
-  for (int i = 0; i < N; i++) {
+  for (int i = 0; i < nshell_pts; i++) {
     for (int l = 0; l <= g->lmax; l++) {
       for (int m = -l; m <= l; m++) {
         int lm = lm_index(l, m);
@@ -616,12 +626,13 @@ void calculate_U_lm(or_grid* g, int a) {
 
 extern "C" {
 
-or_grid* or_grid_create(int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
-                        const int* shell_L, const int* shell_nprim, const int* shell_offset,
-                        const double* alpha, const double* coef, int nrad, int nang, const double* leb,
-                        int lmax) {
+or_grid* or_grid_create_cells(int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+                              const int* shell_L, const int* shell_nprim, const int* shell_offset,
+                              const double* alpha, const double* coef, int nrad, int nang, const double* leb,
+                              int lmax, int ncell) {
   or_grid* g = new or_grid;
   g->natoms = natoms;
+  g->ncell = ncell < 1 || ncell > natoms ? natoms : ncell;
   g->nrad = nrad;
   g->nang = nang;
   g->lmax = lmax;
@@ -649,9 +660,9 @@ or_grid* or_grid_create(int natoms, const int* Z, const double* xyz, int nshell,
   g->U_lm.assign((size_t)natoms * nrad * g->nlm, 0.0);
   g->splines.resize(natoms);
   // src/moleculargrid.cpp:215-226 (grids serially, as the reference)
-  for (int a = 0; a < natoms; a++) create_atomic_grid(g, a);
+  for (int a = 0; a < g->ncell; a++) create_atomic_grid(g, a);
   // src/moleculargrid.cpp:229-254 (serial triple loop)
-  for (int i = 0; i < natoms; i++) {
+  for (int i = 0; i < g->ncell; i++) {
     const long base = (long)i * g->cell();
     for (long j = 0; j < g->cell(); j++) {
       double denom = 0.0, nom = 1.0;
@@ -667,6 +678,14 @@ or_grid* or_grid_create(int natoms, const int* Z, const double* xyz, int nshell,
     }
   }
   return g;
+}
+
+or_grid* or_grid_create(int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+                        const int* shell_L, const int* shell_nprim, const int* shell_offset,
+                        const double* alpha, const double* coef, int nrad, int nang, const double* leb,
+                        int lmax) {
+  return or_grid_create_cells(natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef,
+                              nrad, nang, leb, lmax, natoms);
 }
 
 void or_grid_destroy(or_grid* g) { delete g; }
@@ -696,7 +715,7 @@ void or_grid_radial(const or_grid* g, double* r) {
 void or_grid_set_density(or_grid* g, const double* P) {
   const int nbf = g->nbf;
 #pragma omp parallel for
-  for (int a = 0; a < g->natoms; a++) {
+  for (int a = 0; a < g->ncell; a++) {
     std::vector<double> t(nbf);
     const long base = (long)a * g->cell();
     for (long i = 0; i < g->cell(); i++) {
@@ -715,12 +734,12 @@ void or_grid_set_density(or_grid* g, const double* P) {
 // MolecularGrid::correct_densities, src/moleculargrid.cpp:132-146
 void or_grid_correct_densities(or_grid* g) {
   double sum = 0.0, charge = 0.0;
-  for (int a = 0; a < g->natoms; a++) {
+  for (int a = 0; a < g->ncell; a++) {
     sum += cell_density(g, a);
     charge += g->Z[a];
   }
   double factor = charge / sum;
-  for (int a = 0; a < g->natoms; a++) {
+  for (int a = 0; a < g->ncell; a++) {
     const long base = (long)a * g->cell();
     for (long i = 0; i < g->cell(); i++) g->rho[base + i] *= factor;
     g->cell_cached[a] = 0;
@@ -730,24 +749,27 @@ void or_grid_correct_densities(or_grid* g) {
 // MolecularGrid::calculate_density, src/moleculargrid.cpp:157-166
 double or_grid_calculate_density(or_grid* g) {
   double sum = 0.0;
-  for (int a = 0; a < g->natoms; a++) sum += cell_density(g, a);
+  for (int a = 0; a < g->ncell; a++) sum += cell_density(g, a);
   return sum;
 }
 void or_grid_densities(const or_grid* g, double* rho) {
   std::memcpy(rho, g->rho.data(), sizeof(double) * g->rho.size());
 }
 void or_grid_cell_densities(or_grid* g, double* q) {
-  for (int a = 0; a < g->natoms; a++) q[a] = cell_density(g, a);
+  for (int a = 0; a < g->ncell; a++) q[a] = cell_density(g, a);
 }
 
 // MolecularGrid::calculate_hartree_potential, src/moleculargrid.cpp:336-389
 void or_grid_hartree(or_grid* g, double* J) {
-  for (int a = 0; a < g->natoms; a++) {
-    calculate_rho_lm(g, a);
-    calculate_U_lm(g, a);
-  }
+  using clk = std::chrono::steady_clock;
+  auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+  const auto t0 = clk::now();
+  for (int a = 0; a < g->natoms; a++) calculate_rho_lm(g, a);
+  const auto t1 = clk::now();
+  for (int a = 0; a < g->natoms; a++) calculate_U_lm(g, a);
+  const auto t2 = clk::now();
   const int N = g->nrad;
-  for (int i = 0; i < g->natoms; i++) {
+  for (int i = 0; i < g->ncell; i++) {
     const long base = (long)i * g->cell();
 #pragma omp parallel for
     for (long j = 0; j < g->cell(); j++) {
@@ -775,11 +797,12 @@ void or_grid_hartree(or_grid* g, double* J) {
       g->V[base + j] = V;
     }
   }
+  const auto t3 = clk::now();
   // AtomicGrid::calculate_J_matrix, src/atomicgrid.cpp:471-488, summed at :382-388
   const int nbf = g->nbf;
   for (size_t i = 0; i < (size_t)nbf * nbf; i++) J[i] = 0.0;
   std::vector<double> Jj((size_t)nbf * nbf);
-  for (int a = 0; a < g->natoms; a++) {
+  for (int a = 0; a < g->ncell; a++) {
     const long base = (long)a * g->cell();
     const long np = g->cell();
 #pragma omp parallel for
@@ -795,6 +818,15 @@ void or_grid_hartree(or_grid* g, double* J) {
     }
     for (size_t i = 0; i < (size_t)nbf * nbf; i++) J[i] += Jj[i];
   }
+  const auto t4 = clk::now();
+  g->t_hartree[0] = secs(t0, t1);
+  g->t_hartree[1] = secs(t1, t2);
+  g->t_hartree[2] = secs(t2, t3);
+  g->t_hartree[3] = secs(t3, t4);
+}
+
+void or_grid_hartree_times(const or_grid* g, double* t) {
+  for (int i = 0; i < 4; i++) t[i] = g->t_hartree[i];
 }
 
 void or_grid_hartree_parts(const or_grid* g, double* rho_lm, double* U_lm, double* vcell, double* v,

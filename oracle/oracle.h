@@ -29,6 +29,15 @@ or_grid* or_grid_create(int natoms, const int* Z, const double* xyz,
                         const int* shell_nprim, const int* shell_offset,
                         const double* alpha, const double* coef,
                         int nrad, int nang, const double* leb, int lmax);
+/* The same grid with points in only the first ncell cells (every atom still
+ * enters the Becke partition, the Poisson solves and the cross-atom
+ * potential); the per-iteration calls then do the work of those cells, and
+ * the renormalisation uses their atoms' charge.  For CPU timing samples. */
+or_grid* or_grid_create_cells(int natoms, const int* Z, const double* xyz,
+                              int nshell, const int* shell_atom, const int* shell_L,
+                              const int* shell_nprim, const int* shell_offset,
+                              const double* alpha, const double* coef,
+                              int nrad, int nang, const double* leb, int lmax, int ncell);
 void or_grid_destroy(or_grid* g);
 int or_grid_nbf(const or_grid* g);
 long or_grid_npts(const or_grid* g);
@@ -50,6 +59,9 @@ void or_grid_hartree(or_grid* g, double* J);
  * spline coefficients (natoms x nlm x (nrad-1) x 4); any pointer may be NULL */
 void or_grid_hartree_parts(const or_grid* g, double* rho_lm, double* U_lm,
                            double* vcell, double* v, double* spl);
+/* wall seconds of the last or_grid_hartree's stages: projection rho_lm,
+ * Poisson solves + own-cell potential + splines, cross-atom potential, J */
+void or_grid_hartree_times(const or_grid* g, double* t);
 /* exchange-correlation matrix and energy (src/dft.cpp:394-433) */
 void or_grid_xc(or_grid* g, double* XC, double* exc);
 /* pointwise LDA outputs of the last or_grid_xc: ex+ec (energy density) and

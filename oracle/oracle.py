@@ -30,6 +30,9 @@ def lib():
         L.or_grid_create.restype = C.c_void_p
         L.or_grid_create.argtypes = [C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
                                      C.c_int, C.c_int, _D, C.c_int]
+        L.or_grid_create_cells.restype = C.c_void_p
+        L.or_grid_create_cells.argtypes = [C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
+                                           C.c_int, C.c_int, _D, C.c_int, C.c_int]
         L.or_grid_destroy.argtypes = [C.c_void_p]
         L.or_grid_nbf.argtypes = [C.c_void_p]
         L.or_grid_npts.argtypes = [C.c_void_p]
@@ -43,6 +46,7 @@ def lib():
         L.or_grid_calculate_density.argtypes = [C.c_void_p]
         L.or_grid_calculate_density.restype = C.c_double
         L.or_grid_hartree_parts.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
+        L.or_grid_hartree_times.argtypes = [C.c_void_p, _D]
         L.or_grid_xc.argtypes = [C.c_void_p, _D, _D]
         L.or_grid_xc_parts.argtypes = [C.c_void_p, _D, _D]
         L.or_gto_norm.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
@@ -96,7 +100,8 @@ def _i32(a):
 class Grid:
     """Oracle molecular grid for a dftcxx_amd.molecule.Molecule."""
 
-    def __init__(self, mol, nrad, nang, lmax=8, leb=None):
+    def __init__(self, mol, nrad, nang, lmax=8, leb=None, ncell=None):
+        """ncell: build points in only the first ncell cells (a CPU timing sample)."""
         from dftcxx_amd import lebedev
         self.mol = mol
         self.sh = mol.shells()
@@ -107,8 +112,10 @@ class Grid:
         self._keep = [_i32(mol.Z), _f64(mol.xyz), _i32(s["atom"]), _i32(s["L"]), _i32(s["nprim"]),
                       _i32(s["offset"]), _f64(s["alpha"]), _f64(s["coef"])]
         Z, xyz, at, L, npr, off, al, co = self._keep
-        self.h = lib().or_grid_create(mol.natoms, _i(Z), _d(xyz), len(at), _i(at), _i(L), _i(npr),
-                                      _i(off), _d(al), _d(co), nrad, nang, _d(self._leb), lmax)
+        self.ncell = mol.natoms if ncell is None else int(ncell)
+        self.h = lib().or_grid_create_cells(mol.natoms, _i(Z), _d(xyz), len(at), _i(at), _i(L), _i(npr),
+                                            _i(off), _d(al), _d(co), nrad, nang, _d(self._leb), lmax,
+                                            self.ncell)
         self.nbf = lib().or_grid_nbf(self.h)
         self.npts = lib().or_grid_npts(self.h)
 
@@ -161,6 +168,12 @@ class Grid:
         J = np.empty((self.nbf, self.nbf))
         lib().or_grid_hartree(self.h, _d(J))
         return J
+
+    def hartree_times(self):
+        """Seconds of the last hartree(): projection, Poisson+own cell+splines, cross-atom V, J."""
+        t = np.zeros(4)
+        lib().or_grid_hartree_times(self.h, _d(t))
+        return t
 
     def hartree_parts(self):
         na = self.mol.natoms

@@ -61,6 +61,28 @@ def test_batch_matches_single_molecule_handles():
         assert np.abs(b.get_densities()[sl] - r).max() < 1e-14 * r.max()
 
 
+def test_batch_matches_oracle():
+    # batched molecules straight against the oracle (not only through the single handle)
+    from oracle import oracle as O
+    mols = M.batch(count=12, seed=7)
+    grid = (20, 50, 6)
+    print("[case] batch of 12 at 20x50 vs oracle", flush=True)
+    b = G.Batch(mols, *grid)
+    Ps = [_guess(m) for m in mols]
+    J, XC, sc = b.build(Ps)
+    for m in (0, 4, 9, 11):
+        o = O.Grid(mols[m], *grid)
+        o.set_density(Ps[m])
+        o.correct_densities()
+        Jo = o.hartree()
+        XCo, exco = o.xc()
+        tag = f"molecule {m} ({mols[m].name}/{mols[m].basis})"
+        assert np.abs(J[m] - Jo).max() < 1e-10 * np.abs(Jo).max(), tag
+        assert np.abs(XC[m] - XCo).max() < 1e-10 * np.abs(XCo).max(), tag
+        assert abs(sc[m, 0] - exco) < 1e-10 * abs(exco), tag
+        assert abs(sc[m, 2] - mols[m].nelec) < 1e-11, tag
+
+
 def test_batch_graph_replays_and_autotune():
     mols = M.batch(count=6, seed=11)
     grid = (16, 26, 4)

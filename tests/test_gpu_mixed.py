@@ -84,3 +84,19 @@ def test_fp32_mfma_layout():
         c, g = lane % 16, lane // 16
         for v in range(4):
             assert d[v, lane] == D[4 * g + v, c], (lane, v)
+
+
+@pytest.mark.parametrize("make,grid", [(lambda: M.h2o("sto3g"), (50, 110, 8)),
+                                       (lambda: M.ch4("p631"), (30, 50, 6))], ids=["h2o", "ch4_631g"])
+def test_mixed_scf_against_oracle(make, grid):
+    # the reduced-precision SCF energy stated against the FP64 oracle's SCF
+    from oracle import oracle as O
+    mol = make()
+    print(f"[case] mixed SCF {mol.name} {grid} vs oracle", flush=True)
+    eo, to, Po = O.Grid(mol, *grid).scf(threshold=1e-8, max_iter=200)
+    em, rm, Pm = G.scf_run(mol, *grid, threshold=1e-8, max_iter=200, precision=G.PREC_MIXED)
+    print(f"  oracle {to['total']:.10f} mixed {rm['energy']:.10f} diff {rm['energy'] - to['total']:.2e}")
+    assert rm["converged"] == 1
+    assert abs(rm["energy"] - to["total"]) < 1e-6
+    assert abs(rm["e_xc"] - to["xc"]) < 1e-6
+    assert np.abs(Pm - Po).max() < 1e-4

@@ -148,6 +148,7 @@ def test_scf_energy_h2o():
     en_o, terms_o, P_o = og.scf(threshold=1e-6)
     en_g, res, P_g = G.scf_run(mol, 50, 110, 8, threshold=1e-6)
     assert len(en_g) == len(en_o)
+    assert res["converged"] == 1 and res["delta"] <= 1e-6
     assert np.abs(en_g - en_o).max() < 1e-8
     assert abs(res["energy"] - terms_o["total"]) < 1e-8
     assert abs(res["e_xc"] - terms_o["xc"]) < 1e-8
