@@ -4,7 +4,8 @@ The reference keeps its basis sets in ``basis/<name>.dat`` (format read by
 src/molecule.cpp:152-235: a ``Z nshell`` header, then per shell a
 ``S|P|D nprim`` line followed by ``exponent coefficient`` lines, ``#`` lines
 being comments).  The numbers are the published STO-nG / Pople basis sets.
-This recipe reads those files where they lie and writes
+Copies of those files ship in ``dftcxx_amd/data/basis`` (read by the library's
+input reader, dg_input_read); this recipe reads them and writes
 ``dftcxx_amd/data/basis/<name>.json`` as::
 
     {"name": ..., "elements": {"<Z>": [["S", [[alpha, c], ...]], ...]}}
@@ -12,7 +13,7 @@ This recipe reads those files where they lie and writes
 keeping the shell order of the file, which fixes the CGF order (Q8 of the
 reference: element in ascending Z, then atom, then shell).
 
-Run:  python dftcxx_amd/data/make_basis.py [/root/reference/basis]
+Run:  python dftcxx_amd/data/make_basis.py [basis directory, default dftcxx_amd/data/basis]
 """
 import json
 import os
@@ -44,8 +45,8 @@ def parse_dat(path):
 
 
 def main():
-    src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/basis"
     here = os.path.dirname(os.path.abspath(__file__))
+    src = sys.argv[1] if len(sys.argv) > 1 else os.path.join(here, "basis")
     os.makedirs(os.path.join(here, "basis"), exist_ok=True)
     for name in ("sto3g", "sto6g", "p321", "p631"):
         data = parse_dat(os.path.join(src, name + ".dat"))

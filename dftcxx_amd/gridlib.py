@@ -33,7 +33,8 @@ EXPORTS = (
     "dg_profile_read", "dg_profile_reset", "dg_selftest_mfma", "dg_mfma_probe", "dg_mfma_probe_f32", "dg_scf_run", "dg_one_electron",
     "dg_eigh", "dg_batch_create", "dg_batch_destroy", "dg_batch_info", "dg_batch_layout",
     "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
-    "dg_batch_autotune", "dg_batch_scf_run",
+    "dg_batch_autotune", "dg_batch_scf_run", "dg_input_read", "dg_input_destroy", "dg_input_molecule",
+    "dg_input_basis", "dg_input_grid", "dg_input_value",
 )
 
 
@@ -127,6 +128,12 @@ def lib():
         L.dg_batch_scf_run.argtypes = [C.c_int, C.POINTER(_Molecule), C.POINTER(_Basis),
                                        C.POINTER(_GridParams), C.POINTER(_ScfParams),
                                        C.POINTER(_ScfResult), _I]
+        L.dg_input_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.dg_input_destroy.argtypes = [C.c_void_p]
+        L.dg_input_molecule.argtypes = [C.c_void_p, C.POINTER(_Molecule)]
+        L.dg_input_basis.argtypes = [C.c_void_p, C.POINTER(_Basis)]
+        L.dg_input_grid.argtypes = [C.c_void_p, C.POINTER(_GridParams), _I, _I]
+        L.dg_input_value.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_char_p)]
         _lib = L
     return _lib
 
@@ -525,3 +532,87 @@ def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter
     it = C.c_int()
     _check(lib().dg_batch_scf_run(n, ms, bs, C.byref(ins[0][2]), C.byref(sp), res, C.byref(it)))
     return [{k: getattr(r, k) for k, _ in _ScfResult._fields_} for r in res], it.value
+
+
+BASIS_DIR = os.path.join(_HERE, "data", "basis")
+HARTREE_BECKE = 0
+HARTREE_ERI = 1
+
+
+class InputFile:
+    """A reference input file (molecules/<name>.in) and its basis set, read by
+    dg_input_read (src/settings.cpp:30-187, src/molecule.cpp:49-238).
+
+    Attributes: ``name``, ``basis``, ``units``, ``Z`` and ``xyz`` (bohr),
+    ``radial_points``, ``angular_points``, ``lebedev_order`` (the reference's
+    index), ``lmax``, ``hartree`` (HARTREE_BECKE or HARTREE_ERI).  It stands in
+    for a :class:`dftcxx_amd.molecule.Molecule` wherever one is taken: its
+    ``shells()`` are the basis file's shells in the reference's CGF order.
+    """
+
+    def __init__(self, path, basis_dir=None):
+        self._h = None
+        h = C.c_void_p()
+        _check(lib().dg_input_read(os.fsencode(path), os.fsencode(basis_dir or BASIS_DIR), C.byref(h)))
+        self._h = h
+        m, b, g = _Molecule(), _Basis(), _GridParams()
+        li, he = C.c_int(), C.c_int()
+        _check(lib().dg_input_molecule(h, C.byref(m)))
+        _check(lib().dg_input_basis(h, C.byref(b)))
+        _check(lib().dg_input_grid(h, C.byref(g), C.byref(li), C.byref(he)))
+        self.Z = np.array(m.Z[:m.natoms], dtype=np.int32)
+        self.xyz = np.array(m.xyz[:3 * m.natoms], dtype=np.float64).reshape(-1, 3)
+        nsh = b.nshell
+        nprim = np.array(b.nprim[:nsh], dtype=np.int32)
+        offset = np.array(b.offset[:nsh], dtype=np.int32)
+        ntot = int((offset + nprim).max()) if nsh else 0
+        L = np.array(b.L[:nsh], dtype=np.int32)
+        self._shells = {"atom": np.array(b.atom[:nsh], dtype=np.int32), "L": L, "nprim": nprim,
+                        "offset": offset, "alpha": np.array(b.alpha[:ntot], dtype=np.float64),
+                        "coef": np.array(b.coef[:ntot], dtype=np.float64),
+                        "nbf": int(sum((l + 1) * (l + 2) // 2 for l in L))}
+        self.radial_points, self.angular_points, self.lmax = g.n_radial, g.n_angular, g.lmax
+        self.lebedev_order, self.hartree = li.value, he.value
+        self.name = self.value("name")
+        self.basis = self.value("basis")
+        self.units = self.value("units") if self.has("units") else ""
+
+    def value(self, key):
+        """Settings::get_value: the trimmed value of a key (DgError "Could not find <key>")."""
+        v = C.c_char_p()
+        _check(lib().dg_input_value(self._h, key.encode(), C.byref(v)))
+        return v.value.decode()
+
+    def has(self, key):
+        v = C.c_char_p()
+        return lib().dg_input_value(self._h, key.encode(), C.byref(v)) == 0
+
+    def close(self):
+        if self._h is not None:
+            lib().dg_input_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def natoms(self):
+        return len(self.Z)
+
+    @property
+    def nelec(self):
+        return int(self.Z.sum())
+
+    def shells(self):
+        return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self._shells.items()}
+
+    def nbf(self):
+        return self._shells["nbf"]
+
+
+def read_input(path, basis_dir=None):
+    """Parse a reference input file; see :class:`InputFile`."""
+    return InputFile(path, basis_dir)

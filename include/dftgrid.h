@@ -232,6 +232,37 @@ int dg_batch_autotune(dg_batch* b, int* chunk_len, int* item_len, double* best_m
 int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basis* bases, const dg_grid_params* grid,
                      const dg_scf_params* scf, dg_scf_result* results, int* iterations);
 
+/* --- reference input files (host only, no GPU needed) ---------------------- */
+#define DG_HARTREE_BECKE 0 /* Becke-grid Poisson path (the reference's default) */
+#define DG_HARTREE_ERI 1   /* "hartree_evaluation = two_electron_integrals" */
+
+/* A parsed input file: settings, atoms and the expanded basis set. */
+typedef struct dg_input dg_input;
+/* Reads a reference input file (molecules/<name>.in) and the basis set
+ * <basis_dir>/<basis>.dat it names.  Replaces Settings(filename) and
+ * Molecule(filename, settings) (src/settings.cpp:30-187, src/molecule.cpp:49-238):
+ * "key = value" lines up to the line "system:", then the atom count and
+ * "Element x y z" lines (angstrom converted with 1.889725989 when
+ * units = angstrom, else bohr); grid = coarse | medium (default) | fine |
+ * ultrafine sets radial points, Lebedev rule and lmax, which radial_points,
+ * lebedev_order (the reference's index 0..10) and lmax override.  The basis
+ * file is read up to the molecule's highest element; the shells come out in
+ * the reference's CGF order.  Errors: DG_ERR_INVALID with the reference's
+ * message ("Could not find name", "Unknown element: X", "Cannot open ...!"),
+ * and also for an unknown grid name and for an element without basis
+ * functions (where the reference runs on with an unset grid or no functions). */
+int dg_input_read(const char* path, const char* basis_dir, dg_input** out);
+int dg_input_destroy(dg_input* in);
+/* Views into the parsed input, valid until dg_input_destroy. */
+int dg_input_molecule(const dg_input* in, dg_molecule* mol);
+int dg_input_basis(const dg_input* in, dg_basis* basis);
+/* Grid parameters (precision FP64, no screening), the Lebedev index and the
+ * Hartree method (DG_HARTREE_BECKE or DG_HARTREE_ERI); any output may be NULL. */
+int dg_input_grid(const dg_input* in, dg_grid_params* grid, int* lebedev_index, int* hartree);
+/* Settings::get_value (src/settings.cpp:82-89): the trimmed value of a key, or
+ * DG_ERR_INVALID "Could not find <key>". */
+int dg_input_value(const dg_input* in, const char* key, const char** value);
+
 /* --- host utilities (no GPU needed) ----------------------------------------- */
 /* Overlap, kinetic and nuclear-attraction matrices (nbf x nbf, column-major)
  * of the basis, as DFT::construct_matrices builds them (src/dft.cpp:184-201,
