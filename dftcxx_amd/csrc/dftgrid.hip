@@ -1722,6 +1722,107 @@ __global__ __launch_bounds__(64) void k_force_basis(
     for (int d = 0; d < 3; ++d) mot[d * npts + p] = 4.0 * G[d];
 }
 
+// ---------------------------------------------------------------------------
+// Basis-function and density gradients (GridPoint::set_basis_func_grad and
+// set_gradient, src/gridpoint.cpp:62-108).  The gradients are those of the
+// amplitude formula; the reference's GTO::get_grad (src/cgf.cpp:67-94) leaves
+// out the normalisation, applies the contraction coefficient twice and drops
+// the factor l of x^(l-1), and is not followed.
+// ---------------------------------------------------------------------------
+// Radial sums of shell s at squared distance r2: r0 = sum cN0 e, d0 = sum cN0 (-2 alpha) e,
+// and r1, d1 with the mixed-d normalisation.
+__device__ __forceinline__ void shell_radial(int s, double r2, const int* __restrict__ sh_nprim,
+                                            const int* __restrict__ sh_off, const double* __restrict__ alpha,
+                                            const double* __restrict__ cn0, const double* __restrict__ cn1,
+                                            double& r0, double& d0, double& r1, double& d1) {
+  r0 = d0 = r1 = d1 = 0.0;
+  for (int k = 0; k < sh_nprim[s]; ++k) {
+    const int q = sh_off[s] + k;
+    const double e = exp(-alpha[q] * r2);
+    const double m2a = -2.0 * alpha[q];
+    r0 += cn0[q] * e;
+    r1 += cn1[q] * e;
+    d0 += cn0[q] * m2a * e;
+    d1 += cn1[q] * m2a * e;
+  }
+}
+
+// Gradients of every basis function at every point, out[p][d][i] (d = x, y, z;
+// per point the nbf x 3 column-major matrix of src/gridpoint.cpp:62-72), one
+// thread per (point, shell); points p0 .. p0 + npts of px/py/pz.
+__global__ void k_phi_grad(const double* __restrict__ atoms, int nshell, const int* __restrict__ sh_atom,
+                           const int* __restrict__ sh_L, const int* __restrict__ sh_nprim,
+                           const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
+                           const double* __restrict__ alpha, const double* __restrict__ cn0,
+                           const double* __restrict__ cn1, long p0, long npts, const double* __restrict__ px,
+                           const double* __restrict__ py, const double* __restrict__ pz, int nbf, double* out) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts * nshell) return;
+  const long p = g / nshell;
+  const int s = (int)(g % nshell);
+  const int a = sh_atom[s];
+  const double dx = px[p0 + p] - atoms[4 * a], dy = py[p0 + p] - atoms[4 * a + 1], dz = pz[p0 + p] - atoms[4 * a + 2];
+  double r0, d0, r1, d1;
+  shell_radial(s, dx * dx + dy * dy + dz * dz, sh_nprim, sh_off, alpha, cn0, cn1, r0, d0, r1, d1);
+  const int L = sh_L[s], nc = (L + 1) * (L + 2) / 2;
+  double* o = out + p * 3 * nbf + sh_cgf[s];
+  for (int c = 0; c < nc; ++c) {
+    double gr[3];
+    shell_grad(L, c, r0, d0, r1, d1, dx, dy, dz, gr);
+    o[c] = gr[0];
+    o[nbf + c] = gr[1];
+    o[2 * nbf + c] = gr[2];
+  }
+}
+
+// Density and its gradient at points p0 .. p0 + npts from T = Phi P (rows
+// [p][ldphi] of tmat, row 0 = point p0): rho = 2 sum_j T_j phi_j and
+// grad rho = 2 phi^T P grad phi + 2 grad phi^T P phi = 4 sum_j T_j grad phi_j
+// (P symmetric), both times fac; fac = zsum / scal[1] (the renormalisation of
+// src/moleculargrid.cpp:132-146) when scal is given, else 1.  One thread per
+// point, shells in order; rho (may be NULL) [npts], grad [npts][3].
+__global__ __launch_bounds__(256) void k_density_grad(
+    const double* __restrict__ atoms, int nshell, const int* __restrict__ sh_atom, const int* __restrict__ sh_L,
+    const int* __restrict__ sh_nprim, const int* __restrict__ sh_off, const int* __restrict__ sh_cgf,
+    const double* __restrict__ alpha, const double* __restrict__ cn0, const double* __restrict__ cn1, long p0,
+    long npts, const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    int ldphi, const double* __restrict__ tmat, const double* __restrict__ scal, double zsum, double* rho,
+    double* grad) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const double x = px[p0 + p], y = py[p0 + p], z = pz[p0 + p];
+  const double* trow = tmat + p * ldphi;
+  double v = 0.0, g[3] = {0.0, 0.0, 0.0};
+  for (int s = 0; s < nshell; ++s) {
+    const int a = sh_atom[s];
+    const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
+    double r0, d0, r1, d1;
+    shell_radial(s, dx * dx + dy * dy + dz * dz, sh_nprim, sh_off, alpha, cn0, cn1, r0, d0, r1, d1);
+    const int L = sh_L[s], nc = (L + 1) * (L + 2) / 2;
+    for (int c = 0; c < nc; ++c) {
+      const double t = trow[sh_cgf[s] + c];
+      double gr[3];
+      shell_grad(L, c, r0, d0, r1, d1, dx, dy, dz, gr);
+      double phi;
+      if (L == 0)
+        phi = r0;
+      else if (L == 1)
+        phi = r0 * (c == 0 ? dx : (c == 1 ? dy : dz));
+      else {
+        const double cx[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+        phi = ((c == 1 || c == 2 || c == 4) ? r1 : r0) * cx[c];
+      }
+      v += t * phi;
+      g[0] += t * gr[0];
+      g[1] += t * gr[1];
+      g[2] += t * gr[2];
+    }
+  }
+  const double fac = scal ? zsum / scal[1] : 1.0;
+  if (rho) rho[p] = 2.0 * v * fac;
+  for (int d = 0; d < 3; ++d) grad[3 * p + d] = 4.0 * g[d] * fac;
+}
+
 // Becke-weight term: sum_p w0_p a_p d wb_p / d R_B (w0 = weight without the
 // Becke factor), one wave per 64 points with the atom distances and the cell
 // functions P_n in LDS.  The B != owner terms are added to part[B][d][blk]
@@ -2318,9 +2419,26 @@ int launch_density(dg_handle* h, const double* dP) {
 }
 
 template <int NT>
-void launch_tmat_nt(dg_handle* h) {
-  hipLaunchKernelGGL(k_tmat<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi, h->ldphi,
-                     h->adj_frag, h->adj_T);
+void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat) {
+  hipLaunchKernelGGL(k_tmat<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
+}
+
+// T = Phi P on npts_pad (a multiple of 64) rows, from full P fragments of nt tiles.
+void launch_tmat_rows(int nt, hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag,
+                      double* tmat) {
+  switch (nt) {
+#define DG_NT(N)                                                  \
+  case N:                                                         \
+    launch_tmat_nt<N>(st, npts_pad, phi, ldphi, frag, tmat); \
+    break;
+    DG_NT(1) DG_NT(2) DG_NT(3) DG_NT(4) DG_NT(5) DG_NT(6) DG_NT(7) DG_NT(8) DG_NT(9) DG_NT(10) DG_NT(11)
+    DG_NT(12) DG_NT(13) DG_NT(14) DG_NT(15) DG_NT(16) DG_NT(17) DG_NT(18) DG_NT(19) DG_NT(20) DG_NT(21)
+    DG_NT(22) DG_NT(23) DG_NT(24) DG_NT(25) DG_NT(26) DG_NT(27) DG_NT(28) DG_NT(29) DG_NT(30) DG_NT(31)
+    DG_NT(32)
+#undef DG_NT
+    default:
+      break;
+  }
 }
 
 // T = Phi P for the force terms (P = the last density matrix, full fragments).
@@ -2330,21 +2448,8 @@ int launch_tmat(dg_handle* h) {
     hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(nf, 256)), dim3(256), 0, h->stream, h->last_P, h->nbf, h->ntb,
                        h->adj_frag);
   }));
-  return run(h, "adj_tmat", [&] {
-    switch (h->ntb) {
-#define DG_NT(N)          \
-  case N:                 \
-    launch_tmat_nt<N>(h); \
-    break;
-      DG_NT(1) DG_NT(2) DG_NT(3) DG_NT(4) DG_NT(5) DG_NT(6) DG_NT(7) DG_NT(8) DG_NT(9) DG_NT(10) DG_NT(11)
-      DG_NT(12) DG_NT(13) DG_NT(14) DG_NT(15) DG_NT(16) DG_NT(17) DG_NT(18) DG_NT(19) DG_NT(20) DG_NT(21)
-      DG_NT(22) DG_NT(23) DG_NT(24) DG_NT(25) DG_NT(26) DG_NT(27) DG_NT(28) DG_NT(29) DG_NT(30) DG_NT(31)
-      DG_NT(32)
-#undef DG_NT
-      default:
-        break;
-    }
-  });
+  return run(h, "adj_tmat",
+             [&] { launch_tmat_rows(h->ntb, h->stream, h->npts_pad, h->phi, h->ldphi, h->adj_frag, h->adj_T); });
 }
 
 int stage_density(dg_handle* h, const double* dP) {
@@ -2658,6 +2763,13 @@ int sync(dg_handle* h) {
 int copy_out(dg_handle* h, double* dst, const double* src, size_t n) {
   HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return sync(h);
+}
+
+// T = Phi P buffers (the adjoint's force terms and the density gradients), on first use.
+int ensure_tmat_bufs(dg_handle* h) {
+  if (!h->adj_T) DG_TRY(dalloc(&h->adj_T, (size_t)h->npts_pad * h->ldphi));
+  if (!h->adj_frag) DG_TRY(dalloc(&h->adj_frag, (size_t)256 * h->ntb * h->ntb));
+  return DG_OK;
 }
 
 }  // namespace
@@ -3278,9 +3390,8 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
     DG_TRY(dalloc(&h->adj_dP, (size_t)h->nbf * h->nbf));
     DG_TRY(dalloc(&h->adj_out, (size_t)6 * na + 1));
   }
-  if (grad && !h->adj_T) {
-    DG_TRY(dalloc(&h->adj_T, (size_t)np * h->ldphi));
-    DG_TRY(dalloc(&h->adj_frag, (size_t)256 * h->ntb * h->ntb));
+  if (grad) DG_TRY(ensure_tmat_bufs(h));
+  if (grad && !h->adj_part) {
     DG_TRY(dalloc(&h->adj_part, (size_t)3 * na * nblk));
     DG_TRY(dalloc(&h->adj_mot, (size_t)3 * h->npts));
     // shells grouped by centre atom
@@ -3348,6 +3459,164 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
       for (int d = 0; d < 3; ++d) grad[3 * a + d] = o[3 * a + d] + o[3 * na + d * na + a];
   }
   return sync(h);
+}
+
+int dg_get_amplitude_gradients(dg_handle* h, double* dphi) {
+  if (!h || !dphi) return fail(DG_ERR_INVALID, "dg_get_amplitude_gradients: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_get_amplitude_gradients: not for a batch handle");
+  // in chunks of at most 2^27 doubles of device memory
+  const long chunk = std::max(1L, std::min(h->npts, (1L << 27) / (3L * h->nbf)));
+  double* buf = nullptr;
+  DG_TRY(dalloc(&buf, (size_t)chunk * 3 * h->nbf));
+  int rc = DG_OK;
+  for (long p0 = 0; p0 < h->npts && rc == DG_OK; p0 += chunk) {
+    const long n = std::min(chunk, h->npts - p0);
+    rc = run(h, "phi_grad", [&] {
+      hipLaunchKernelGGL(k_phi_grad, dim3(nblocks(n * h->nshell, 256)), dim3(256), 0, h->stream, h->atoms,
+                         h->nshell, h->sh_atom, h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->pr_cn0,
+                         h->pr_cn1, p0, n, h->px, h->py, h->pz, h->nbf, buf);
+    });
+    if (rc == DG_OK) rc = copy_out(h, dphi + p0 * 3 * h->nbf, buf, (size_t)n * 3 * h->nbf);
+  }
+  (void)hipFree(buf);
+  return rc;
+}
+
+int dg_get_density_gradients(dg_handle* h, double* grad) {
+  if (!h || !grad) return fail(DG_ERR_INVALID, "dg_get_density_gradients: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_get_density_gradients: not for a batch handle");
+  if (!h->has_density || !h->last_P)
+    return fail(DG_ERR_STATE, "dg_get_density_gradients: no density evaluated (dg_set_density or a build first)");
+  DG_TRY(ensure_tmat_bufs(h));
+  DG_TRY(launch_tmat(h));
+  double* buf = nullptr;
+  DG_TRY(dalloc(&buf, (size_t)3 * h->npts));
+  int rc = run(h, "density_grad", [&] {
+    hipLaunchKernelGGL(k_density_grad, dim3(nblocks(h->npts, 256)), dim3(256), 0, h->stream, h->atoms, h->nshell,
+                       h->sh_atom, h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->pr_cn0, h->pr_cn1, 0L,
+                       h->npts, h->px, h->py, h->pz, h->ldphi, h->adj_T, h->corrected ? h->scalars : nullptr,
+                       h->zsum, nullptr, buf);
+  });
+  if (rc == DG_OK) rc = copy_out(h, grad, buf, (size_t)3 * h->npts);
+  (void)hipFree(buf);
+  return rc;
+}
+
+int dg_density_at_points(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz,
+                         const double* P, double* rho, double* grad) {
+  g_last_error.clear();
+  if (!mol || !basis || !P || (npts > 0 && !xyz)) return fail(DG_ERR_INVALID, "dg_density_at_points: null argument");
+  if (npts < 0) return fail(DG_ERR_INVALID, "dg_density_at_points: negative point count");
+  if (mol->natoms < 1 || !mol->xyz) return fail(DG_ERR_INVALID, "dg_density_at_points: empty molecule");
+  if (basis->nshell < 1 || !basis->atom || !basis->L || !basis->nprim || !basis->offset || !basis->alpha ||
+      !basis->coef)
+    return fail(DG_ERR_INVALID, "dg_density_at_points: empty basis");
+  if (npts == 0 || (!rho && !grad)) return DG_OK;
+  const int nshell = basis->nshell, na = mol->natoms;
+  std::vector<int> cgf(nshell);
+  int nbf = 0, nprim = 0;
+  for (int s = 0; s < nshell; ++s) {
+    const int L = basis->L[s];
+    if (L < 0 || L > 2) return fail(DG_ERR_INVALID, "dg_density_at_points: shell L must be 0, 1 or 2");
+    if (basis->atom[s] < 0 || basis->atom[s] >= na)
+      return fail(DG_ERR_INVALID, "dg_density_at_points: shell atom out of range");
+    if (basis->nprim[s] < 1 || basis->offset[s] < 0)
+      return fail(DG_ERR_INVALID, "dg_density_at_points: bad primitive range");
+    cgf[s] = nbf;
+    nbf += (L + 1) * (L + 2) / 2;
+    nprim = std::max(nprim, basis->offset[s] + basis->nprim[s]);
+  }
+  const int nt = (nbf + 15) / 16, ld = (int)round_up(nbf, 32);
+  if (nt > 32) return fail(DG_ERR_UNSUPPORTED, "dg_density_at_points: more than 512 basis functions");
+  std::vector<double> alpha(nprim, 1.0), cn0(nprim, 0.0), cn1(nprim, 0.0), atoms(4 * na);
+  for (int s = 0; s < nshell; ++s)
+    for (int k = 0; k < basis->nprim[s]; ++k) {
+      const int i = basis->offset[s] + k, L = basis->L[s];
+      if (!(basis->alpha[i] > 0.0))
+        return fail(DG_ERR_INVALID, "dg_density_at_points: primitive exponent must be positive");
+      alpha[i] = basis->alpha[i];
+      cn0[i] = basis->coef[i] * dg::gto_norm(alpha[i], L, 0, 0);
+      cn1[i] = basis->coef[i] * dg::gto_norm(alpha[i], L == 2 ? 1 : L, L == 2 ? 1 : 0, 0);
+    }
+  for (int a = 0; a < na; ++a) {
+    for (int d = 0; d < 3; ++d) atoms[4 * a + d] = mol->xyz[3 * a + d];
+    atoms[4 * a + 3] = mol->Z ? mol->Z[a] : 0;
+  }
+  const long chunk = std::min(round_up(npts, 64), 1L << 16);
+  struct Bufs {  // device buffers and the stream of this call, released on every return
+    std::vector<void*> p;
+    hipStream_t st = nullptr;
+    ~Bufs() {
+      if (st) (void)hipStreamSynchronize(st);
+      for (void* b : p) (void)hipFree(b);
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } B;
+  auto get = [&](auto** ptr, size_t n) {
+    const int rc = dalloc(ptr, n);
+    if (rc == DG_OK) B.p.push_back((void*)*ptr);
+    return rc;
+  };
+  double *d_atoms, *d_alpha, *d_cn0, *d_cn1, *d_P, *d_frag, *d_px, *d_py, *d_pz, *d_phi, *d_T, *d_rho, *d_grad;
+  int *d_atom, *d_L, *d_np, *d_off, *d_cgf;
+  DG_TRY(get(&d_atoms, atoms.size()));
+  DG_TRY(get(&d_alpha, nprim));
+  DG_TRY(get(&d_cn0, nprim));
+  DG_TRY(get(&d_cn1, nprim));
+  DG_TRY(get(&d_atom, nshell));
+  DG_TRY(get(&d_L, nshell));
+  DG_TRY(get(&d_np, nshell));
+  DG_TRY(get(&d_off, nshell));
+  DG_TRY(get(&d_cgf, nshell));
+  DG_TRY(get(&d_P, (size_t)nbf * nbf));
+  DG_TRY(get(&d_frag, (size_t)256 * nt * nt));
+  DG_TRY(get(&d_px, chunk));
+  DG_TRY(get(&d_py, chunk));
+  DG_TRY(get(&d_pz, chunk));
+  DG_TRY(get(&d_phi, (size_t)chunk * ld));
+  DG_TRY(get(&d_T, (size_t)chunk * ld));
+  DG_TRY(get(&d_rho, chunk));
+  DG_TRY(get(&d_grad, 3 * chunk));
+  HIP_TRY(hipStreamCreate(&B.st));
+  hipStream_t st = B.st;
+  auto up = [&](void* d, const void* src, size_t bytes) { return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st); };
+  HIP_TRY(up(d_atoms, atoms.data(), atoms.size() * 8));
+  HIP_TRY(up(d_alpha, alpha.data(), nprim * 8));
+  HIP_TRY(up(d_cn0, cn0.data(), nprim * 8));
+  HIP_TRY(up(d_cn1, cn1.data(), nprim * 8));
+  HIP_TRY(up(d_atom, basis->atom, nshell * 4));
+  HIP_TRY(up(d_L, basis->L, nshell * 4));
+  HIP_TRY(up(d_np, basis->nprim, nshell * 4));
+  HIP_TRY(up(d_off, basis->offset, nshell * 4));
+  HIP_TRY(up(d_cgf, cgf.data(), nshell * 4));
+  HIP_TRY(up(d_P, P, (size_t)nbf * nbf * 8));
+  // padding rows and columns of Phi stay zero
+  HIP_TRY(hipMemsetAsync(d_phi, 0, (size_t)chunk * ld * 8, st));
+  hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(256L * nt * nt, 256)), dim3(256), 0, st, d_P, nbf, nt, d_frag);
+  std::vector<double> sx(chunk), sy(chunk), sz(chunk);
+  for (long p0 = 0; p0 < npts; p0 += chunk) {
+    const long n = std::min(chunk, npts - p0);
+    HIP_TRY(hipStreamSynchronize(st));  // the host staging arrays are reused
+    for (long i = 0; i < n; ++i) {
+      sx[i] = xyz[3 * (p0 + i)];
+      sy[i] = xyz[3 * (p0 + i) + 1];
+      sz[i] = xyz[3 * (p0 + i) + 2];
+    }
+    HIP_TRY(up(d_px, sx.data(), n * 8));
+    HIP_TRY(up(d_py, sy.data(), n * 8));
+    HIP_TRY(up(d_pz, sz.data(), n * 8));
+    hipLaunchKernelGGL(k_phi, dim3(nblocks(n * nshell, 256)), dim3(256), 0, st, d_atoms, nshell, d_atom, d_L, d_np,
+                       d_off, d_cgf, d_alpha, d_cn0, d_cn1, n, d_px, d_py, d_pz, ld, d_phi);
+    launch_tmat_rows(nt, st, round_up(n, 64), d_phi, ld, d_frag, d_T);
+    hipLaunchKernelGGL(k_density_grad, dim3(nblocks(n, 256)), dim3(256), 0, st, d_atoms, nshell, d_atom, d_L, d_np,
+                       d_off, d_cgf, d_alpha, d_cn0, d_cn1, 0L, n, d_px, d_py, d_pz, ld, d_T, nullptr, 0.0, d_rho,
+                       d_grad);
+    HIP_TRY(hipGetLastError());
+    if (rho) HIP_TRY(hipMemcpyAsync(rho + p0, d_rho, n * 8, hipMemcpyDeviceToHost, st));
+    if (grad) HIP_TRY(hipMemcpyAsync(grad + 3 * p0, d_grad, 3 * n * 8, hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return DG_OK;
 }
 
 int dg_profile_enable(dg_handle* h, int enable) {

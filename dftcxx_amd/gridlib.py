@@ -34,7 +34,8 @@ EXPORTS = (
     "dg_eigh", "dg_batch_create", "dg_batch_destroy", "dg_batch_info", "dg_batch_layout",
     "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
     "dg_batch_autotune", "dg_batch_scf_run", "dg_input_read", "dg_input_destroy", "dg_input_molecule",
-    "dg_input_basis", "dg_input_grid", "dg_input_value",
+    "dg_input_basis", "dg_input_grid", "dg_input_value", "dg_get_amplitude_gradients",
+    "dg_get_density_gradients", "dg_density_at_points",
 )
 
 
@@ -99,7 +100,8 @@ def lib():
         L.dg_hartree.argtypes = [C.c_void_p, _D]
         L.dg_xc.argtypes = [C.c_void_p, _D, _D]
         for n in ("dg_get_weights", "dg_get_becke_weights", "dg_get_positions", "dg_get_densities",
-                  "dg_get_amplitudes", "dg_get_hartree_potential"):
+                  "dg_get_amplitudes", "dg_get_hartree_potential", "dg_get_amplitude_gradients",
+                  "dg_get_density_gradients"):
             getattr(L, n).argtypes = [C.c_void_p, _D]
         L.dg_get_hartree_parts.argtypes = [C.c_void_p, _D, _D, _D]
         L.dg_get_xc_parts.argtypes = [C.c_void_p, _D, _D]
@@ -128,6 +130,7 @@ def lib():
         L.dg_batch_scf_run.argtypes = [C.c_int, C.POINTER(_Molecule), C.POINTER(_Basis),
                                        C.POINTER(_GridParams), C.POINTER(_ScfParams),
                                        C.POINTER(_ScfResult), _I]
+        L.dg_density_at_points.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.c_long, _D, _D, _D, _D]
         L.dg_input_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
         L.dg_input_destroy.argtypes = [C.c_void_p]
         L.dg_input_molecule.argtypes = [C.c_void_p, C.POINTER(_Molecule)]
@@ -320,6 +323,15 @@ class MolecularGrid:
         """Amplitudes as (npts, nbf); the reference returns the transpose (nbf x npts)."""
         return self._vec(lib().dg_get_amplitudes, self.npts * self.nbf).reshape(self.npts, self.nbf)
 
+    def get_amplitude_gradients(self):
+        """Basis-function gradients (npts, 3, nbf) (src/gridpoint.cpp:62-72)."""
+        return self._vec(lib().dg_get_amplitude_gradients, self.npts * 3 * self.nbf).reshape(
+            self.npts, 3, self.nbf)
+
+    def get_density_gradients(self):
+        """Gradient of the current density (npts, 3) (src/gridpoint.cpp:94-108)."""
+        return self._vec(lib().dg_get_density_gradients, 3 * self.npts).reshape(self.npts, 3)
+
     def get_hartree_parts(self):
         self._need()
         shape = (self.natoms, self.radial_points, self.nlm)
@@ -491,6 +503,43 @@ def one_electron(mol):
     S, T, V = (np.empty((n, n)) for _ in range(3))
     _check(lib().dg_one_electron(C.byref(m), C.byref(b), _d(S), _d(T), _d(V)))
     return S.T.copy(), T.T.copy(), V.T.copy()
+
+
+def density_at_points(mol, xyz, P):
+    """(rho [n], grad rho [n, 3]) at arbitrary points (bohr) on the GPU (dg_density_at_points)."""
+    m, b, g, keep = _c_inputs(mol, 5, 6, 0, PREC_FP64)
+    xyz = _f64(np.asarray(xyz).reshape(-1, 3))
+    n = len(xyz)
+    Pc = _f64(np.asarray(P).T)
+    rho, grad = np.empty(n), np.empty((n, 3))
+    _check(lib().dg_density_at_points(C.byref(m), C.byref(b), n, _d(xyz), _d(Pc), _d(rho), _d(grad)))
+    return rho, grad
+
+
+class RectangularGrid:
+    """The reference's RectangularGrid (src/rectangulargrid.cpp:34-95): dp^3 points
+    spanning [-size/2, size/2] (x fastest), density and gradient at each on the GPU,
+    and the gradient file."""
+
+    def __init__(self, mol):
+        self.mol = mol
+        self.positions = np.zeros((0, 3))
+        self.densities = self.gradients = None
+
+    def build_grid(self, size, dp):
+        if dp < 2:
+            raise ValueError("at least two points per direction")
+        t = np.arange(dp) * (size / (dp - 1)) - size / 2.0
+        z, y, x = np.meshgrid(t, t, t, indexing="ij")
+        self.positions = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+
+    def set_density(self, P):
+        self.densities, self.gradients = density_at_points(self.mol, self.positions, P)
+
+    def write_gradient(self, filename):
+        with open(filename, "w") as f:
+            for p, g in zip(self.positions, self.gradients):
+                f.write("%12.8f  %12.8f  %12.8f  %12.8f  %12.8f  %12.8f\n" % (p[0], p[1], p[2], g[0], g[1], g[2]))
 
 
 def eigh(A):

@@ -6,6 +6,8 @@
 #ifndef DFTCXX_MOLECULAR_GRID_HPP
 #define DFTCXX_MOLECULAR_GRID_HPP
 
+#include <algorithm>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -40,6 +42,48 @@ struct MoleculeData {
 
 inline void dg_throw(int rc) {
   if (rc != DG_OK) throw std::runtime_error(std::string("dftgrid: ") + dg_last_error());
+}
+
+// Settings + Molecule of a reference input file (src/settings.cpp:30-187,
+// src/molecule.cpp:49-238): the molecule, its basis from basis_dir/<basis>.dat,
+// and the grid parameters the file sets.
+struct InputData {
+  MoleculeData mol;
+  dg_grid_params grid{15, 110, 8, DG_PREC_FP64, 0.0};
+  int lebedev_order = 7;
+  int hartree = DG_HARTREE_BECKE;
+  std::string name, basis;
+};
+
+inline InputData read_input(const std::string& path, const std::string& basis_dir) {
+  dg_input* in = nullptr;
+  dg_throw(dg_input_read(path.c_str(), basis_dir.c_str(), &in));
+  InputData d;
+  dg_molecule m;
+  dg_basis b;
+  const char* v = nullptr;
+  int rc = dg_input_molecule(in, &m);
+  if (rc == DG_OK) rc = dg_input_basis(in, &b);
+  if (rc == DG_OK) rc = dg_input_grid(in, &d.grid, &d.lebedev_order, &d.hartree);
+  if (rc == DG_OK) {
+    d.mol.Z.assign(m.Z, m.Z + m.natoms);
+    d.mol.xyz.assign(m.xyz, m.xyz + 3 * m.natoms);
+    int nprim = 0;
+    for (int s = 0; s < b.nshell; ++s) nprim = std::max(nprim, b.offset[s] + b.nprim[s]);
+    d.mol.shell_atom.assign(b.atom, b.atom + b.nshell);
+    d.mol.shell_L.assign(b.L, b.L + b.nshell);
+    d.mol.shell_nprim.assign(b.nprim, b.nprim + b.nshell);
+    d.mol.shell_offset.assign(b.offset, b.offset + b.nshell);
+    d.mol.alpha.assign(b.alpha, b.alpha + nprim);
+    d.mol.coef.assign(b.coef, b.coef + nprim);
+    rc = dg_input_value(in, "name", &v);
+    if (rc == DG_OK) d.name = v;
+    if (rc == DG_OK) rc = dg_input_value(in, "basis", &v);
+    if (rc == DG_OK) d.basis = v;
+  }
+  dg_input_destroy(in);
+  dg_throw(rc);
+  return d;
 }
 
 // Reference lebedev_order enum index (src/quadrature.h:30-44) -> point count.
@@ -111,6 +155,10 @@ class MolecularGrid {
   std::vector<double> get_weights() { return vec(dg_get_weights, npts_); }
   std::vector<double> get_densities() { return vec(dg_get_densities, npts_); }
   std::vector<double> get_amplitudes() { return vec(dg_get_amplitudes, npts_ * nbf_); }
+  // per point the nbf x 3 column-major basis_func_grad (src/gridpoint.cpp:62-72)
+  std::vector<double> get_amplitude_gradients() { return vec(dg_get_amplitude_gradients, npts_ * 3 * nbf_); }
+  // per point grad rho (src/gridpoint.cpp:94-108), npts x 3
+  std::vector<double> get_density_gradients() { return vec(dg_get_density_gradients, npts_ * 3); }
 
   dg_handle* handle() { return h_; }
   int nbf() const { return nbf_; }
@@ -127,6 +175,51 @@ class MolecularGrid {
   dg_handle* h_ = nullptr;
   int nbf_ = 0;
   long npts_ = 0;
+};
+
+// The reference's RectangularGrid (src/rectangulargrid.h:30-80,
+// src/rectangulargrid.cpp:34-95): a cube of dp^3 points spanning
+// [-size/2, size/2] in each direction (x fastest, then y, then z), the density
+// and its gradient at every point, and the gradient file.
+class RectangularGrid {
+ public:
+  explicit RectangularGrid(const MoleculeData& mol) : mol_(mol) {}
+  void build_grid(double size, unsigned dp) {
+    if (dp < 2) throw std::runtime_error("RectangularGrid: at least two points per direction");
+    const double gd = size / (double)(dp - 1);
+    pos_.clear();
+    for (unsigned i = 0; i < dp; ++i)
+      for (unsigned j = 0; j < dp; ++j)
+        for (unsigned k = 0; k < dp; ++k) {
+          pos_.push_back((double)k * gd - size / 2.0);
+          pos_.push_back((double)j * gd - size / 2.0);
+          pos_.push_back((double)i * gd - size / 2.0);
+        }
+  }
+  void set_density(const std::vector<double>& P) {
+    const long n = (long)(pos_.size() / 3);
+    rho_.assign(n, 0.0);
+    grad_.assign(3 * n, 0.0);
+    dg_molecule m = mol_.c_molecule();
+    dg_basis b = mol_.c_basis();
+    dg_throw(dg_density_at_points(&m, &b, n, pos_.data(), P.data(), rho_.data(), grad_.data()));
+  }
+  // one line "x y z gx gy gz" per point, %12.8f (src/rectangulargrid.cpp:82-95)
+  void write_gradient(const std::string& filename) const {
+    FILE* f = std::fopen(filename.c_str(), "w");
+    if (!f) throw std::runtime_error("RectangularGrid: cannot write " + filename);
+    for (size_t i = 0; i < pos_.size() / 3; ++i)
+      std::fprintf(f, "%12.8f  %12.8f  %12.8f  %12.8f  %12.8f  %12.8f\n", pos_[3 * i], pos_[3 * i + 1],
+                   pos_[3 * i + 2], grad_[3 * i], grad_[3 * i + 1], grad_[3 * i + 2]);
+    std::fclose(f);
+  }
+  const std::vector<double>& positions() const { return pos_; }
+  const std::vector<double>& densities() const { return rho_; }
+  const std::vector<double>& gradients() const { return grad_; }
+
+ private:
+  MoleculeData mol_;
+  std::vector<double> pos_, rho_, grad_;
 };
 
 }  // namespace dftcxx

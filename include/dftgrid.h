@@ -118,6 +118,29 @@ int dg_get_hartree_parts(dg_handle* h, double* rho_lm, double* U_lm, double* vce
 /* Pointwise XC after dg_xc: energy density e_x+e_c and w*(v_xa+v_xb+v_ca+v_cb)/2 [npts]. */
 int dg_get_xc_parts(dg_handle* h, double* exc_density, double* wv);
 
+/* --- basis-function and density gradients ---------------------------------- */
+/* Gradients of every basis function at every grid point, [npts][3][nbf]: per
+ * point the nbf x 3 column-major matrix GridPoint::basis_func_grad holds
+ * (GridPoint::set_basis_func_grad, src/gridpoint.cpp:62-72).  They are the
+ * derivatives of the amplitude formula; the reference's GTO::get_grad
+ * (src/cgf.cpp:67-94) leaves out the normalisation, applies the contraction
+ * coefficient twice and drops the factor l of x^(l-1), and is not followed. */
+int dg_get_amplitude_gradients(dg_handle* h, double* dphi);
+/* Density gradient [npts][3] of the last evaluated density,
+ * grad rho = 2 phi^T P grad phi + 2 grad phi^T P phi (GridPoint::set_gradient,
+ * src/gridpoint.cpp:94-108), times the renormalisation factor once the density
+ * is corrected (so that it is the gradient of dg_get_densities' rho).  Needs the
+ * density matrix still in the device buffer it was read from. */
+int dg_get_density_gradients(dg_handle* h, double* grad);
+/* Density and density gradient at arbitrary points, the work of the reference's
+ * RectangularGrid (src/rectangulargrid.cpp:34-95: build_grid, set_density and
+ * GridPoint::set_density/set_gradient): xyz [npts][3] in bohr, P the
+ * column-major nbf x nbf density matrix (host pointers); rho [npts] and
+ * grad [npts][3] (either may be NULL).  Runs on the current HIP device and
+ * synchronises. */
+int dg_density_at_points(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz,
+                         const double* P, double* rho, double* grad);
+
 /* --- adjoint of E_xc ------------------------------------------------------ */
 /* No counterpart in the reference: its GTO gradient (src/cgf.cpp:67-94,
  * 164-172) is never called on this path.  Differentiates the E_xc of the

@@ -926,6 +926,86 @@ void or_lu_solve(int n, const double* A, const double* b, double* x) {
   lu_solve(n, M, perm, b, x);
 }
 
+// RectangularGrid::build_grid positions (src/rectangulargrid.cpp:34-57): dp^3
+// points, x fastest, spanning [-size/2, size/2].
+void or_rectangular_points(double size, unsigned dp, double* xyz) {
+  const double gdsize = size / (double)(dp - 1);
+  long q = 0;
+  for (unsigned i = 0; i < dp; i++)
+    for (unsigned j = 0; j < dp; j++)
+      for (unsigned k = 0; k < dp; k++) {
+        xyz[q++] = (double)k * gdsize - size / 2.0;
+        xyz[q++] = (double)j * gdsize - size / 2.0;
+        xyz[q++] = (double)i * gdsize - size / 2.0;
+      }
+}
+
+// RectangularGrid::set_density (src/rectangulargrid.cpp:68-73) at the given
+// points: GridPoint::set_basis_func_amp / set_basis_func_grad
+// (src/gridpoint.cpp:45-72), set_density (82-84) and set_gradient (94-108),
+// in the reference's product-rule form with both products D*x and D*phi.
+// The basis gradient is the derivative of the amplitude formula (the
+// reference's GTO::get_grad, src/cgf.cpp:67-94, is not followed: it omits N,
+// applies c twice and drops the factor l).  P column-major; rho [npts],
+// grad [npts][3], dphi [npts][3][nbf]; any output may be NULL.
+void or_points_density(int natoms, const double* xyz, int nshell, const int* shell_atom, const int* shell_L,
+                       const int* shell_nprim, const int* shell_offset, const double* alpha, const double* coef,
+                       long npts, const double* pts, const double* P, double* rho, double* grad, double* dphi) {
+  (void)natoms;
+  const std::vector<CGF> cgfs =
+      build_cgfs(xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef);
+  const int n = (int)cgfs.size();
+#pragma omp parallel for
+  for (long p = 0; p < npts; ++p) {
+    const Vec3 r{pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    std::vector<double> amp(n), gx(n), gy(n), gz(n);
+    for (int i = 0; i < n; ++i) {
+      amp[i] = cgf_amp(cgfs[i], r);
+      double g[3] = {0.0, 0.0, 0.0};
+      for (const GTO& t : cgfs[i].gtos) {
+        const double x = r.x - t.pos.x, y = r.y - t.pos.y, z = r.z - t.pos.z;
+        const double e = std::exp(-t.alpha * (x * x + y * y + z * z));
+        const double fx = std::pow(x, t.l), fy = std::pow(y, t.m), fz = std::pow(z, t.n);
+        const double dfx = (t.l > 0 ? t.l * std::pow(x, t.l - 1) : 0.0) - 2.0 * t.alpha * x * fx;
+        const double dfy = (t.m > 0 ? t.m * std::pow(y, t.m - 1) : 0.0) - 2.0 * t.alpha * y * fy;
+        const double dfz = (t.n > 0 ? t.n * std::pow(z, t.n - 1) : 0.0) - 2.0 * t.alpha * z * fz;
+        const double cn = t.c * t.norm * e;
+        g[0] += cn * dfx * fy * fz;
+        g[1] += cn * fx * dfy * fz;
+        g[2] += cn * fx * fy * dfz;
+      }
+      gx[i] = g[0];
+      gy[i] = g[1];
+      gz[i] = g[2];
+    }
+    // D * v for a vector v (column-major D)
+    auto dmul = [&](const std::vector<double>& v) {
+      std::vector<double> out(n, 0.0);
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) out[i] += P[i + (long)j * n] * v[j];
+      return out;
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+      double t = 0.0;
+      for (int i = 0; i < n; ++i) t += a[i] * b[i];
+      return t;
+    };
+    const std::vector<double> Dphi = dmul(amp);
+    if (rho) rho[p] = 2.0 * dot(amp, Dphi);
+    if (grad) {
+      grad[3 * p] = 2.0 * dot(amp, dmul(gx)) + 2.0 * dot(gx, Dphi);
+      grad[3 * p + 1] = 2.0 * dot(amp, dmul(gy)) + 2.0 * dot(gy, Dphi);
+      grad[3 * p + 2] = 2.0 * dot(amp, dmul(gz)) + 2.0 * dot(gz, Dphi);
+    }
+    if (dphi)
+      for (int i = 0; i < n; ++i) {
+        dphi[(p * 3 + 0) * n + i] = gx[i];
+        dphi[(p * 3 + 1) * n + i] = gy[i];
+        dphi[(p * 3 + 2) * n + i] = gz[i];
+      }
+  }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -120,6 +120,13 @@ int or_scf_evaluate(or_grid* g, int natoms, const int* Z, const double* xyz,
                     const double* P, double hold_c, double* terms,
                     double* F, double* H, double* J, double* XC);
 
+/* --- rectangular grid and density gradients (src/rectangulargrid.cpp:34-95,
+ *     src/gridpoint.cpp:45-108) ------------------------------------------ */
+void or_rectangular_points(double size, unsigned dp, double* xyz);
+void or_points_density(int natoms, const double* xyz, int nshell, const int* shell_atom, const int* shell_L,
+                       const int* shell_nprim, const int* shell_offset, const double* alpha, const double* coef,
+                       long npts, const double* pts, const double* P, double* rho, double* grad, double* dphi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,9 @@ def lib():
         L.or_scf_evaluate.argtypes = [C.c_void_p, C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
                                       _D, C.c_double, _D, _D, _D, _D, _D]
         L.or_scf_evaluate.restype = C.c_int
+        L.or_rectangular_points.argtypes = [C.c_double, C.c_uint, _D]
+        L.or_points_density.argtypes = [C.c_int, _D, C.c_int, _I, _I, _I, _I, _D, _D, C.c_long, _D, _D,
+                                        _D, _D, _D]
         _lib = L
     return _lib
 
@@ -314,3 +317,27 @@ def gto_norm(alpha, l, m, n):
 
 def boys(m, x):
     return lib().or_boys(m, x)
+
+
+def rectangular_points(size, dp):
+    """RectangularGrid::build_grid positions, (dp^3, 3)."""
+    out = np.empty(3 * dp ** 3)
+    lib().or_rectangular_points(float(size), int(dp), _d(out))
+    return out.reshape(-1, 3)
+
+
+def points_density(mol, pts, P):
+    """(rho [n], grad rho [n, 3], basis gradients [n, 3, nbf]) at the points, the
+    reference's GridPoint::set_density / set_gradient with P symmetric."""
+    s = mol.shells()
+    nbf = s["nbf"]
+    pts = _f64(np.asarray(pts).reshape(-1, 3))
+    n = len(pts)
+    Pc = _f64(np.asarray(P).T)
+    rho, grad, dphi = np.empty(n), np.empty((n, 3)), np.empty((n, 3, nbf))
+    keep = [_f64(mol.xyz), _i32(s["atom"]), _i32(s["L"]), _i32(s["nprim"]), _i32(s["offset"]),
+            _f64(s["alpha"]), _f64(s["coef"])]
+    xyz, at, L, npr, off, al, co = keep
+    lib().or_points_density(mol.natoms, _d(xyz), len(at), _i(at), _i(L), _i(npr), _i(off), _d(al), _d(co), n,
+                            _d(pts), _d(Pc), _d(rho), _d(grad), _d(dphi))
+    return rho, grad, dphi
