@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1775,6 +1776,21 @@ __global__ void k_phi_grad(const double* __restrict__ atoms, int nshell, const i
   }
 }
 
+// x[i] = v for i < n.
+__global__ void k_fill(double* x, long n, double v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// Points [n][3] -> x, y, z arrays.
+__global__ void k_split_xyz(const double* __restrict__ xyz, long n, double* px, double* py, double* pz) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  px[i] = xyz[3 * i];
+  py[i] = xyz[3 * i + 1];
+  pz[i] = xyz[3 * i + 2];
+}
+
 // Density and its gradient at points p0 .. p0 + npts from T = Phi P (rows
 // [p][ldphi] of tmat, row 0 = point p0): rho = 2 sum_j T_j phi_j and
 // grad rho = 2 phi^T P grad phi + 2 grad phi^T P phi = 4 sum_j T_j grad phi_j
@@ -2772,6 +2788,128 @@ int ensure_tmat_bufs(dg_handle* h) {
   return DG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Density and density gradient at arbitrary points (dg_density_at_points and
+// its variant on the static scratch array)
+// ---------------------------------------------------------------------------
+constexpr int kPtsMaxShell = 512, kPtsMaxPrim = 4096, kPtsMaxAtoms = 1024;
+constexpr long kPointChunk = 1L << 16;  // points per device pass at most
+
+// Host tables of the basis (fixed capacity, so that a caller-provided object
+// needs no heap): CGF offset per shell, exponents and c N products, atoms.
+struct PointTables {
+  int nshell = 0, na = 0, nbf = 0, nprim = 0, nt = 0, ld = 0;
+  int ishell[5 * kPtsMaxShell];  // atom, L, nprim, offset, cgf offset per shell (five blocks)
+  double alpha[kPtsMaxPrim], cn0[kPtsMaxPrim], cn1[kPtsMaxPrim];
+  double atoms[4 * kPtsMaxAtoms];
+};
+
+// Device buffers of one evaluation; chunk = points per pass.
+struct PointWork {
+  double *atoms = nullptr, *alpha = nullptr, *cn0 = nullptr, *cn1 = nullptr, *P = nullptr, *frag = nullptr;
+  double *xyz = nullptr, *pxyz = nullptr, *phi = nullptr, *T = nullptr, *rho = nullptr, *grad = nullptr;
+  int* ishell = nullptr;
+  long chunk = 0;
+};
+
+int point_check(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz, const double* P,
+                const char* who) {
+  if (!mol || !basis || !P || (npts > 0 && !xyz)) return fail(DG_ERR_INVALID, std::string(who) + ": null argument");
+  if (npts < 0) return fail(DG_ERR_INVALID, std::string(who) + ": negative point count");
+  if (mol->natoms < 1 || !mol->xyz) return fail(DG_ERR_INVALID, std::string(who) + ": empty molecule");
+  if (basis->nshell < 1 || !basis->atom || !basis->L || !basis->nprim || !basis->offset || !basis->alpha ||
+      !basis->coef)
+    return fail(DG_ERR_INVALID, std::string(who) + ": empty basis");
+  return DG_OK;
+}
+
+int point_tables(const dg_molecule* mol, const dg_basis* basis, PointTables& t, const char* who) {
+  // (messages are built only on failure: the success path allocates nothing)
+  auto w = [who](const char* msg) { return std::string(who) + msg; };
+  t.nshell = basis->nshell;
+  t.na = mol->natoms;
+  if (t.nshell > kPtsMaxShell) return fail(DG_ERR_UNSUPPORTED, w(": more than 512 shells"));
+  if (t.na > kPtsMaxAtoms) return fail(DG_ERR_UNSUPPORTED, w(": more than 1024 atoms"));
+  int nbf = 0, nprim = 0;
+  for (int s = 0; s < t.nshell; ++s) {
+    const int L = basis->L[s];
+    if (L < 0 || L > 2) return fail(DG_ERR_INVALID, w(": shell L must be 0, 1 or 2"));
+    if (basis->atom[s] < 0 || basis->atom[s] >= t.na) return fail(DG_ERR_INVALID, w(": shell atom out of range"));
+    if (basis->nprim[s] < 1 || basis->offset[s] < 0) return fail(DG_ERR_INVALID, w(": bad primitive range"));
+    if (basis->offset[s] + basis->nprim[s] > kPtsMaxPrim)
+      return fail(DG_ERR_UNSUPPORTED, w(": more than 4096 primitives"));
+    t.ishell[s] = basis->atom[s];
+    t.ishell[kPtsMaxShell + s] = L;
+    t.ishell[2 * kPtsMaxShell + s] = basis->nprim[s];
+    t.ishell[3 * kPtsMaxShell + s] = basis->offset[s];
+    t.ishell[4 * kPtsMaxShell + s] = nbf;
+    nbf += (L + 1) * (L + 2) / 2;
+    nprim = std::max(nprim, basis->offset[s] + basis->nprim[s]);
+  }
+  t.nbf = nbf;
+  t.nprim = nprim;
+  t.nt = (nbf + 15) / 16;
+  t.ld = (int)round_up(nbf, 32);
+  if (t.nt > 32) return fail(DG_ERR_UNSUPPORTED, w(": more than 512 basis functions"));
+  for (int i = 0; i < nprim; ++i) {
+    t.alpha[i] = 1.0;
+    t.cn0[i] = t.cn1[i] = 0.0;
+  }
+  for (int s = 0; s < t.nshell; ++s)
+    for (int k = 0; k < basis->nprim[s]; ++k) {
+      const int i = basis->offset[s] + k, L = basis->L[s];
+      if (!(basis->alpha[i] > 0.0)) return fail(DG_ERR_INVALID, w(": primitive exponent must be positive"));
+      t.alpha[i] = basis->alpha[i];
+      t.cn0[i] = basis->coef[i] * dg::gto_norm(t.alpha[i], L, 0, 0);
+      t.cn1[i] = basis->coef[i] * dg::gto_norm(t.alpha[i], L == 2 ? 1 : L, L == 2 ? 1 : 0, 0);
+    }
+  for (int a = 0; a < t.na; ++a) {
+    for (int d = 0; d < 3; ++d) t.atoms[4 * a + d] = mol->xyz[3 * a + d];
+    t.atoms[4 * a + 3] = mol->Z ? mol->Z[a] : 0;
+  }
+  return DG_OK;
+}
+
+// The evaluation on the buffers of w, in passes of w.chunk points on stream st:
+// amplitudes (k_phi), T = Phi P (k_tmat), rho and grad rho (k_density_grad).
+int point_run(const PointTables& t, const dg_basis* basis, const double* P, long npts, const double* xyz, double* rho,
+              double* grad, const PointWork& w, hipStream_t st) {
+  (void)basis;
+  const int ns = t.nshell, K = kPtsMaxShell;
+  auto up = [&](void* d, const void* src, size_t bytes) {
+    return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  HIP_TRY(up(w.atoms, t.atoms, 4 * (size_t)t.na * 8));
+  HIP_TRY(up(w.alpha, t.alpha, (size_t)t.nprim * 8));
+  HIP_TRY(up(w.cn0, t.cn0, (size_t)t.nprim * 8));
+  HIP_TRY(up(w.cn1, t.cn1, (size_t)t.nprim * 8));
+  for (int b = 0; b < 5; ++b) HIP_TRY(up(w.ishell + (size_t)b * ns, t.ishell + (size_t)b * K, (size_t)ns * 4));
+  HIP_TRY(up(w.P, P, (size_t)t.nbf * t.nbf * 8));
+  const int *sa = w.ishell, *sL = sa + ns, *snp = sL + ns, *soff = snp + ns, *scgf = soff + ns;
+  const long C = w.chunk;
+  double *px = w.pxyz, *py = px + C, *pz = py + C;
+  // padding rows and columns of Phi stay zero (a fill kernel: the buffer may hold an earlier call's values)
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(C * t.ld, 256)), dim3(256), 0, st, w.phi, C * t.ld, 0.0);
+  hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(256L * t.nt * t.nt, 256)), dim3(256), 0, st, w.P, t.nbf, t.nt,
+                     w.frag);
+  for (long p0 = 0; p0 < npts; p0 += C) {
+    const long n = std::min(C, npts - p0);
+    HIP_TRY(up(w.xyz, xyz + 3 * p0, (size_t)n * 3 * 8));
+    hipLaunchKernelGGL(k_split_xyz, dim3(nblocks(n, 256)), dim3(256), 0, st, w.xyz, n, px, py, pz);
+    hipLaunchKernelGGL(k_phi, dim3(nblocks(n * ns, 256)), dim3(256), 0, st, w.atoms, ns, sa, sL, snp, soff, scgf,
+                       w.alpha, w.cn0, w.cn1, n, px, py, pz, t.ld, w.phi);
+    launch_tmat_rows(t.nt, st, round_up(n, 64), w.phi, t.ld, w.frag, w.T);
+    hipLaunchKernelGGL(k_density_grad, dim3(nblocks(n, 256)), dim3(256), 0, st, w.atoms, ns, sa, sL, snp, soff, scgf,
+                       w.alpha, w.cn0, w.cn1, 0L, n, px, py, pz, t.ld, w.T, nullptr, 0.0, w.rho, w.grad);
+    HIP_TRY(hipGetLastError());
+    if (rho) HIP_TRY(hipMemcpyAsync(rho + p0, w.rho, n * 8, hipMemcpyDeviceToHost, st));
+    if (grad) HIP_TRY(hipMemcpyAsync(grad + 3 * p0, w.grad, 3 * n * 8, hipMemcpyDeviceToHost, st));
+    // the next pass overwrites the position and result buffers
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return DG_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3505,44 +3643,14 @@ int dg_get_density_gradients(dg_handle* h, double* grad) {
 int dg_density_at_points(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz,
                          const double* P, double* rho, double* grad) {
   g_last_error.clear();
-  if (!mol || !basis || !P || (npts > 0 && !xyz)) return fail(DG_ERR_INVALID, "dg_density_at_points: null argument");
-  if (npts < 0) return fail(DG_ERR_INVALID, "dg_density_at_points: negative point count");
-  if (mol->natoms < 1 || !mol->xyz) return fail(DG_ERR_INVALID, "dg_density_at_points: empty molecule");
-  if (basis->nshell < 1 || !basis->atom || !basis->L || !basis->nprim || !basis->offset || !basis->alpha ||
-      !basis->coef)
-    return fail(DG_ERR_INVALID, "dg_density_at_points: empty basis");
+  const char* who = "dg_density_at_points";
+  DG_TRY(point_check(mol, basis, npts, xyz, P, who));
   if (npts == 0 || (!rho && !grad)) return DG_OK;
-  const int nshell = basis->nshell, na = mol->natoms;
-  std::vector<int> cgf(nshell);
-  int nbf = 0, nprim = 0;
-  for (int s = 0; s < nshell; ++s) {
-    const int L = basis->L[s];
-    if (L < 0 || L > 2) return fail(DG_ERR_INVALID, "dg_density_at_points: shell L must be 0, 1 or 2");
-    if (basis->atom[s] < 0 || basis->atom[s] >= na)
-      return fail(DG_ERR_INVALID, "dg_density_at_points: shell atom out of range");
-    if (basis->nprim[s] < 1 || basis->offset[s] < 0)
-      return fail(DG_ERR_INVALID, "dg_density_at_points: bad primitive range");
-    cgf[s] = nbf;
-    nbf += (L + 1) * (L + 2) / 2;
-    nprim = std::max(nprim, basis->offset[s] + basis->nprim[s]);
-  }
-  const int nt = (nbf + 15) / 16, ld = (int)round_up(nbf, 32);
-  if (nt > 32) return fail(DG_ERR_UNSUPPORTED, "dg_density_at_points: more than 512 basis functions");
-  std::vector<double> alpha(nprim, 1.0), cn0(nprim, 0.0), cn1(nprim, 0.0), atoms(4 * na);
-  for (int s = 0; s < nshell; ++s)
-    for (int k = 0; k < basis->nprim[s]; ++k) {
-      const int i = basis->offset[s] + k, L = basis->L[s];
-      if (!(basis->alpha[i] > 0.0))
-        return fail(DG_ERR_INVALID, "dg_density_at_points: primitive exponent must be positive");
-      alpha[i] = basis->alpha[i];
-      cn0[i] = basis->coef[i] * dg::gto_norm(alpha[i], L, 0, 0);
-      cn1[i] = basis->coef[i] * dg::gto_norm(alpha[i], L == 2 ? 1 : L, L == 2 ? 1 : 0, 0);
-    }
-  for (int a = 0; a < na; ++a) {
-    for (int d = 0; d < 3; ++d) atoms[4 * a + d] = mol->xyz[3 * a + d];
-    atoms[4 * a + 3] = mol->Z ? mol->Z[a] : 0;
-  }
-  const long chunk = std::min(round_up(npts, 64), 1L << 16);
+  std::unique_ptr<PointTables> t(new (std::nothrow) PointTables);
+  if (!t) return fail(DG_ERR_ALLOC, "dg_density_at_points: out of host memory");
+  DG_TRY(point_tables(mol, basis, *t, who));
+  PointWork w;
+  w.chunk = std::min(round_up(npts, 64), kPointChunk);
   struct Bufs {  // device buffers and the stream of this call, released on every return
     std::vector<void*> p;
     hipStream_t st = nullptr;
@@ -3557,66 +3665,22 @@ int dg_density_at_points(const dg_molecule* mol, const dg_basis* basis, long npt
     if (rc == DG_OK) B.p.push_back((void*)*ptr);
     return rc;
   };
-  double *d_atoms, *d_alpha, *d_cn0, *d_cn1, *d_P, *d_frag, *d_px, *d_py, *d_pz, *d_phi, *d_T, *d_rho, *d_grad;
-  int *d_atom, *d_L, *d_np, *d_off, *d_cgf;
-  DG_TRY(get(&d_atoms, atoms.size()));
-  DG_TRY(get(&d_alpha, nprim));
-  DG_TRY(get(&d_cn0, nprim));
-  DG_TRY(get(&d_cn1, nprim));
-  DG_TRY(get(&d_atom, nshell));
-  DG_TRY(get(&d_L, nshell));
-  DG_TRY(get(&d_np, nshell));
-  DG_TRY(get(&d_off, nshell));
-  DG_TRY(get(&d_cgf, nshell));
-  DG_TRY(get(&d_P, (size_t)nbf * nbf));
-  DG_TRY(get(&d_frag, (size_t)256 * nt * nt));
-  DG_TRY(get(&d_px, chunk));
-  DG_TRY(get(&d_py, chunk));
-  DG_TRY(get(&d_pz, chunk));
-  DG_TRY(get(&d_phi, (size_t)chunk * ld));
-  DG_TRY(get(&d_T, (size_t)chunk * ld));
-  DG_TRY(get(&d_rho, chunk));
-  DG_TRY(get(&d_grad, 3 * chunk));
+  const long C = w.chunk;
+  DG_TRY(get(&w.atoms, 4 * (size_t)t->na));
+  DG_TRY(get(&w.alpha, t->nprim));
+  DG_TRY(get(&w.cn0, t->nprim));
+  DG_TRY(get(&w.cn1, t->nprim));
+  DG_TRY(get(&w.ishell, 5 * (size_t)t->nshell));
+  DG_TRY(get(&w.P, (size_t)t->nbf * t->nbf));
+  DG_TRY(get(&w.frag, (size_t)256 * t->nt * t->nt));
+  DG_TRY(get(&w.xyz, 3 * (size_t)C));
+  DG_TRY(get(&w.pxyz, 3 * (size_t)C));
+  DG_TRY(get(&w.phi, (size_t)C * t->ld));
+  DG_TRY(get(&w.T, (size_t)C * t->ld));
+  DG_TRY(get(&w.rho, C));
+  DG_TRY(get(&w.grad, 3 * (size_t)C));
   HIP_TRY(hipStreamCreate(&B.st));
-  hipStream_t st = B.st;
-  auto up = [&](void* d, const void* src, size_t bytes) { return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st); };
-  HIP_TRY(up(d_atoms, atoms.data(), atoms.size() * 8));
-  HIP_TRY(up(d_alpha, alpha.data(), nprim * 8));
-  HIP_TRY(up(d_cn0, cn0.data(), nprim * 8));
-  HIP_TRY(up(d_cn1, cn1.data(), nprim * 8));
-  HIP_TRY(up(d_atom, basis->atom, nshell * 4));
-  HIP_TRY(up(d_L, basis->L, nshell * 4));
-  HIP_TRY(up(d_np, basis->nprim, nshell * 4));
-  HIP_TRY(up(d_off, basis->offset, nshell * 4));
-  HIP_TRY(up(d_cgf, cgf.data(), nshell * 4));
-  HIP_TRY(up(d_P, P, (size_t)nbf * nbf * 8));
-  // padding rows and columns of Phi stay zero
-  HIP_TRY(hipMemsetAsync(d_phi, 0, (size_t)chunk * ld * 8, st));
-  hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(256L * nt * nt, 256)), dim3(256), 0, st, d_P, nbf, nt, d_frag);
-  std::vector<double> sx(chunk), sy(chunk), sz(chunk);
-  for (long p0 = 0; p0 < npts; p0 += chunk) {
-    const long n = std::min(chunk, npts - p0);
-    HIP_TRY(hipStreamSynchronize(st));  // the host staging arrays are reused
-    for (long i = 0; i < n; ++i) {
-      sx[i] = xyz[3 * (p0 + i)];
-      sy[i] = xyz[3 * (p0 + i) + 1];
-      sz[i] = xyz[3 * (p0 + i) + 2];
-    }
-    HIP_TRY(up(d_px, sx.data(), n * 8));
-    HIP_TRY(up(d_py, sy.data(), n * 8));
-    HIP_TRY(up(d_pz, sz.data(), n * 8));
-    hipLaunchKernelGGL(k_phi, dim3(nblocks(n * nshell, 256)), dim3(256), 0, st, d_atoms, nshell, d_atom, d_L, d_np,
-                       d_off, d_cgf, d_alpha, d_cn0, d_cn1, n, d_px, d_py, d_pz, ld, d_phi);
-    launch_tmat_rows(nt, st, round_up(n, 64), d_phi, ld, d_frag, d_T);
-    hipLaunchKernelGGL(k_density_grad, dim3(nblocks(n, 256)), dim3(256), 0, st, d_atoms, nshell, d_atom, d_L, d_np,
-                       d_off, d_cgf, d_alpha, d_cn0, d_cn1, 0L, n, d_px, d_py, d_pz, ld, d_T, nullptr, 0.0, d_rho,
-                       d_grad);
-    HIP_TRY(hipGetLastError());
-    if (rho) HIP_TRY(hipMemcpyAsync(rho + p0, d_rho, n * 8, hipMemcpyDeviceToHost, st));
-    if (grad) HIP_TRY(hipMemcpyAsync(grad + 3 * p0, d_grad, 3 * n * 8, hipMemcpyDeviceToHost, st));
-  }
-  HIP_TRY(hipStreamSynchronize(st));
-  return DG_OK;
+  return point_run(*t, basis, P, npts, xyz, rho, grad, w, B.st);
 }
 
 int dg_profile_enable(dg_handle* h, int enable) {

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -20,6 +21,7 @@
 #include "../../include/dftcxx/molecular_grid.hpp"
 #include "../../include/dftgrid.h"
 #include "grid_tables.h"
+#include "scf_device.h"
 
 namespace dftcxx {
 namespace {
@@ -658,6 +660,8 @@ class HostSCF {
             double delta) const {
     r->iterations = iterations;
     r->converged = converged ? 1 : 0;
+    r->device_step = 0;
+    r->host_fallbacks = 0;
     r->delta = delta;
     r->energy = et_;
     r->e_one = e_one_;
@@ -684,21 +688,34 @@ class HostSCF {
 };
 
 // The reference's DFT class with the grid work of every iteration on the GPU
-// (dg_build_device on one molecule's handle).
+// (dg_build_device on one molecule's handle).  With on_device, the
+// density-matrix step and the energy run on the GPU as well (DeviceStep:
+// F, X^T F X, the occupied projector by SP2, X P' X^T, mixing, energy), so
+// an iteration moves only the energy terms to the host; a projection that
+// does not converge (no HOMO-LUMO gap) is redone by the host eigen-solver.
 class DFT : public HostSCF {
  public:
-  DFT(const MoleculeData& mol, const dg_grid_params& grid)
-      : HostSCF(mol, grid.precision == DG_PREC_MIXED), grid_(mol) {
+  DFT(const MoleculeData& mol, const dg_grid_params& grid, bool on_device)
+      : HostSCF(mol, grid.precision == DG_PREC_MIXED), grid_(mol), on_device_(on_device) {
     grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision, grid.screen);
     grid_.create_grid();
   }
   ~DFT() {
+    ds_.reset();
     if (dP_) (void)hipFree(dP_);
     if (hbuf_) (void)hipHostFree(hbuf_);
   }
 
   void construct_matrices() {
     HostSCF::construct_matrices();
+    if (on_device_) {
+      void* sp = nullptr;
+      dg_throw(dg_get_stream(grid_.handle(), &sp));
+      ds_.reset(new DeviceStep(n_, nelec_ / 2, (hipStream_t)sp));
+      ds_->upload_core(H_.data(), X_.data());
+      device_iteration(false);
+      return;
+    }
     if (hipMalloc((void**)&dP_, sizeof(double) * (3 * (size_t)n_ * n_ + 4)) != hipSuccess)
       throw std::runtime_error("hipMalloc failed");
     // pinned staging for P, J, XC and the scalars (page-locked copies each iteration)
@@ -719,9 +736,13 @@ class DFT : public HostSCF {
     const auto t0 = std::chrono::steady_clock::now();
     while (difference > threshold || iteration < min_iter) {
       iteration++;
-      density_matrix();
-      grid_build(true);
-      energy();
+      if (on_device_) {
+        device_iteration(true);
+      } else {
+        density_matrix();
+        grid_build(true);
+        energy();
+      }
       if (energies) energies[iteration - 1] = et_;
       difference = std::fabs(et_ - old_energy);
       old_energy = et_;
@@ -731,10 +752,37 @@ class DFT : public HostSCF {
     if (sec_iter) *sec_iter = iteration ? dt / iteration : 0.0;
     *converged = difference <= threshold && iteration >= min_iter;
     *delta = difference;
+    if (on_device_) ds_->download(P_.data(), ds_->dP());
     return iteration;
   }
 
+  int host_fallbacks = 0;  // density steps the host eigen-solver redid
+
  private:
+  // One iteration with everything on the device: density step, grid build, energy.
+  void device_iteration(bool with_xc) {
+    ds_->density(with_xc, first_, mixing);
+    dg_throw(dg_build_device(grid_.handle(), ds_->dP(), ds_->dJ(), with_xc ? ds_->dXC() : nullptr, ds_->dscal()));
+    ds_->energy(enuc_, with_xc, mixed_);
+    bool ok = false;
+    ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
+    if (!ok) {
+      // the projection did not converge: this step by the host eigen-solver, then
+      // the build and the energy again
+      ++host_fallbacks;
+      ds_->download(P_.data(), ds_->dPold());
+      ds_->download(J_.data(), ds_->dJ());
+      if (with_xc) ds_->download(XC_.data(), ds_->dXC());
+      HostSCF::density_matrix();
+      ds_->upload(ds_->dP(), P_.data());
+      dg_throw(dg_build_device(grid_.handle(), ds_->dP(), ds_->dJ(), with_xc ? ds_->dXC() : nullptr,
+                               ds_->dscal()));
+      ds_->energy(enuc_, with_xc, mixed_);
+      ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
+    }
+    first_ = false;
+  }
+
   // set_density + correct_densities + Hartree J (+ XC) on the GPU
   void grid_build(bool with_xc) {
     const size_t nn = (size_t)n_ * n_;
@@ -757,6 +805,8 @@ class DFT : public HostSCF {
   }
 
   MolecularGrid grid_;
+  bool on_device_ = false;
+  std::unique_ptr<DeviceStep> ds_;
   double* dP_ = nullptr;
   double* hbuf_ = nullptr;
 };
@@ -859,6 +909,8 @@ bool check_scf_params(const dg_scf_params* p, const char* who) {
   else if (p->min_iter < 0) bad = "min_iter must be >= 0";
   else if (!std::isfinite(p->threshold) || p->threshold < 0.0) bad = "threshold must be finite and >= 0";
   else if (!(p->mixing >= 0.0 && p->mixing < 1.0)) bad = "mixing must lie in [0, 1)";
+  else if (p->density_step < DG_STEP_AUTO || p->density_step > DG_STEP_DEVICE)
+    bad = "density_step must be DG_STEP_AUTO, DG_STEP_HOST or DG_STEP_DEVICE";
   if (bad) dg::set_last_error(std::string(who) + ": " + bad);
   return bad == nullptr;
 }
@@ -906,7 +958,9 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     dftcxx::MoleculeData m = to_data(mol, basis);
     if (m.nelec() % 2 != 0) throw std::runtime_error("closed-shell SCF needs an even electron count");
     const auto t0 = std::chrono::steady_clock::now();
-    dftcxx::DFT dft(m, *grid);
+    const bool dev = scf->density_step == DG_STEP_DEVICE ||
+                     (scf->density_step == DG_STEP_AUTO && m.nbf() >= DG_STEP_AUTO_NBF);
+    dftcxx::DFT dft(m, *grid, dev);
     dft.mixing = scf->mixing;
     dft.construct_matrices();
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -914,6 +968,8 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     bool converged = false;
     const int it = dft.scf(scf->threshold, scf->min_iter, scf->max_iter, energies, &sec_iter, &converged, &delta);
     dft.fill(result, it, setup, sec_iter, converged, delta);
+    result->device_step = dev ? 1 : 0;
+    result->host_fallbacks = dft.host_fallbacks;
     if (P_out) std::memcpy(P_out, dft.P_.data(), sizeof(double) * dft.P_.size());
     return DG_OK;
   } catch (const std::exception& e) {

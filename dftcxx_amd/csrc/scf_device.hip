@@ -1,0 +1,332 @@
+// Device-resident density-matrix step of the SCF (src/dft.cpp:330-366, 441-447):
+// F = H + 2J + XC, F' = X^T F X, the projector onto the nocc lowest
+// eigenvectors of F', P = X P' X^T, the mixing with the previous P, and the
+// energy 2 tr(PH) + E_J + E_nuc + E_xc -- all on the device, so that an SCF
+// iteration moves no matrix between host and device.
+//
+// The reference diagonalises F' (Eigen::SelfAdjointEigenSolver) and forms
+// P = C_occ C_occ^T.  Only that projector enters the SCF, so here it is
+// computed by second-order spectral projection (SP2; A. M. N. Niklasson,
+// Phys. Rev. B 66, 155115 (2002)): starting from the Gershgorin-bounded linear
+// map P0 = (h_max I - F')/(h_max - h_min), each step replaces P by P^2 or by
+// 2P - P^2, whichever trace lies closer to nocc; the eigenvalues of P go to 1
+// below the HOMO-LUMO gap and to 0 above it.  Every step is one symmetric
+// FP64 matrix product on the matrix cores.  Without a gap (degenerate
+// frontier orbitals) the iteration stalls; the step then reports failure and
+// the caller falls back to the host eigen-solver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "scf_device.h"
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma_f64(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Control block of one projection: done, steps taken, failed; the idempotency
+// error tr(P) - tr(P^2) of the last two steps.
+struct Sp2Ctl {
+  int done, iter, fail, pad;
+  double err, err_prev, hmin, hmax;
+};
+
+// C = alpha op(A) op(B) + beta D (D may be NULL), column-major, M x N with inner
+// dimension K.  A work-group computes a 32 x 32 block of C, each of its four
+// waves a 16 x 16 tile on the FP64 16x16x4 MFMA: lane (g, c) supplies
+// op(A)(i0 + c, k + g) and op(B)(k + g, j0 + c), and result register v holds
+// C(i0 + 4v + g, j0 + c).  TA reads A transposed (A(k, i)), TB reads B
+// transposed (B(j, k)); for a symmetric B, TB = true gives the same product
+// with lane-contiguous loads.  The launch returns at once when ctl->done.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                              int lda, const double* __restrict__ B, int ldb, double beta,
+                                              const double* __restrict__ D, double* __restrict__ C, int ldc,
+                                              const Sp2Ctl* __restrict__ ctl) {
+  if (ctl && ctl->done) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int i0 = blockIdx.x * 32 + (wave & 1) * 16, j0 = blockIdx.y * 32 + (wave >> 1) * 16;
+  const int ia = i0 + c, jb = j0 + c;
+  const bool arow = ia < M, bcol = jb < N;
+  dbl4 d = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * u + g;
+      a[u] = (arow && k < K) ? (TA ? A[k + (long)ia * lda] : A[ia + (long)k * lda]) : 0.0;
+      b[u] = (bcol && k < K) ? (TB ? B[jb + (long)k * ldb] : B[k + (long)jb * ldb]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d = mfma_f64(a[u], b[u], d);
+  }
+  if (!bcol) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = i0 + 4 * v + g;
+    if (row >= M) continue;
+    const long e = row + (long)jb * ldc;
+    C[e] = alpha * d[v] + (D ? beta * D[e] : 0.0);
+  }
+}
+
+// F = H + 2J + XC (src/dft.cpp:334); XC may be NULL (the set-up step, before any XC).
+__global__ void k_fock(const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ XC,
+                       long nn, double* F) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nn) F[e] = H[e] + 2.0 * J[e] + (XC ? XC[e] : 0.0);
+}
+
+// Gershgorin bounds of the symmetric F' and P0 = (h_max I - F') / (h_max - h_min);
+// resets the control block.  One work-group.
+__global__ __launch_bounds__(1024) void k_sp2_init(const double* __restrict__ F, int n, double* P, Sp2Ctl* ctl) {
+  __shared__ double lo_s[1024], hi_s[1024];
+  double lo = 1e300, hi = -1e300;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double* col = F + (long)i * n;  // row i of the symmetric F'
+    double r = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) r += fabs(col[j]);
+    lo = fmin(lo, col[i] - r);
+    hi = fmax(hi, col[i] + r);
+  }
+  lo_s[threadIdx.x] = lo;
+  hi_s[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      lo_s[threadIdx.x] = fmin(lo_s[threadIdx.x], lo_s[threadIdx.x + s]);
+      hi_s[threadIdx.x] = fmax(hi_s[threadIdx.x], hi_s[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  const double hmin = lo_s[0], hmax = hi_s[0];
+  const double inv = 1.0 / fmax(hmax - hmin, 1e-300);
+  for (long e = threadIdx.x; e < (long)n * n; e += blockDim.x) {
+    const long i = e % n, j = e / n;
+    P[e] = ((i == j ? hmax : 0.0) - F[e]) * inv;
+  }
+  if (threadIdx.x == 0) {
+    ctl->done = ctl->iter = ctl->fail = 0;
+    ctl->err = ctl->err_prev = 1e300;
+    ctl->hmin = hmin;
+    ctl->hmax = hmax;
+  }
+}
+
+// One SP2 decision from tr(P) and tr(P^2) (fixed summation order): converged when
+// the idempotency error tr(P) - tr(P^2) falls below tol, or, once below 1e-9,
+// stops halving (the rounding floor); failed after maxit steps.  Else records
+// which update the next kernel applies (ctl->pad: 1 = P^2, 0 = 2P - P^2).
+__global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P, const double* __restrict__ P2,
+                                                    int n, int nocc, double tol, int maxit, Sp2Ctl* ctl) {
+  if (ctl->done) return;
+  __shared__ double t1s[256], t2s[256];
+  double t1 = 0.0, t2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    t1 += P[i + (long)i * n];
+    t2 += P2[i + (long)i * n];
+  }
+  t1s[threadIdx.x] = t1;
+  t2s[threadIdx.x] = t2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      t1s[threadIdx.x] += t1s[threadIdx.x + s];
+      t2s[threadIdx.x] += t2s[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double tr1 = t1s[0], tr2 = t2s[0];
+  const double err = fabs(tr1 - tr2);
+  ctl->err_prev = ctl->err;
+  ctl->err = err;
+  if (err < tol || (err < 1e-9 && err > 0.5 * ctl->err_prev)) {
+    ctl->done = 1;
+    return;
+  }
+  if (ctl->iter + 1 >= maxit) {
+    ctl->done = 1;
+    ctl->fail = 1;
+    return;
+  }
+  ctl->iter += 1;
+  // trace after P -> P^2 is tr2, after P -> 2P - P^2 it is 2 tr1 - tr2
+  ctl->pad = fabs(tr2 - nocc) < fabs(2.0 * tr1 - tr2 - nocc) ? 1 : 0;
+}
+
+__global__ void k_sp2_apply(double* P, const double* __restrict__ P2, long nn, const Sp2Ctl* __restrict__ ctl) {
+  if (ctl->done) return;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  P[e] = ctl->pad ? P2[e] : 2.0 * P[e] - P2[e];
+}
+
+// P = Pn on the first step, else (1 - m) Pn + m P (src/dft.cpp:331, 358).
+__global__ void k_mix(double* P, const double* __restrict__ Pn, long nn, double m, int first) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  P[e] = first ? Pn[e] : (1.0 - m) * Pn[e] + m * P[e];
+}
+
+// E = 2 tr(PH) + E_J + E_nuc + E_xc (src/dft.cpp:441-447) with E_J = 2 tr(PJ), or
+// the pointwise value scal[3] (use_point_ej); E_xc = scal[0] when use_exc.
+// out = {E_one, E_J, E_xc, total, N after correction}.  One work-group, fixed order.
+__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ P, const double* __restrict__ H,
+                                                const double* __restrict__ J, int n,
+                                                const double* __restrict__ scal, double enuc, int use_exc,
+                                                int use_point_ej, double* out) {
+  __shared__ double s1[256], s2[256];
+  double t1 = 0.0, t2 = 0.0;
+  const long nn = (long)n * n;
+  for (long e = threadIdx.x; e < nn; e += blockDim.x) {
+    t1 += P[e] * H[e];
+    t2 += P[e] * J[e];
+  }
+  s1[threadIdx.x] = t1;
+  s2[threadIdx.x] = t2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double e1 = 2.0 * s1[0];
+    const double ej = use_point_ej ? scal[3] : 2.0 * s2[0];
+    const double exc = use_exc ? scal[0] : 0.0;
+    out[0] = e1;
+    out[1] = ej;
+    out[2] = exc;
+    out[3] = e1 + ej + enuc + exc;
+    out[4] = scal[2];
+  }
+}
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+unsigned nb(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+namespace dftcxx {
+
+DeviceStep::DeviceStep(int n, int nocc, hipStream_t stream) : n_(n), nocc_(nocc), st_(stream) {
+  const size_t nn = (size_t)n * n;
+  // H, X, F (also P'), F', Pn, P2 (also X P'), P, J, XC, P before the step, scalars(4), energy(5)
+  check(hipMalloc((void**)&buf_, sizeof(double) * (10 * nn + 16)), "hipMalloc of the SCF buffers");
+  check(hipMalloc((void**)&ctl_, sizeof(Sp2Ctl)), "hipMalloc of the SCF control block");
+  check(hipHostMalloc((void**)&host_, sizeof(double) * 8 + sizeof(Sp2Ctl), hipHostMallocDefault),
+        "hipHostMalloc of the SCF staging");
+  H_ = buf_;
+  X_ = H_ + nn;
+  F_ = X_ + nn;
+  Fp_ = F_ + nn;
+  Pn_ = Fp_ + nn;
+  P2_ = Pn_ + nn;
+  P_ = P2_ + nn;
+  J_ = P_ + nn;
+  XC_ = J_ + nn;
+  Pold_ = XC_ + nn;
+  scal_ = Pold_ + nn;
+  eout_ = scal_ + 4;
+  check(hipMemsetAsync(buf_, 0, sizeof(double) * (10 * nn + 16), st_), "hipMemsetAsync");
+}
+
+DeviceStep::~DeviceStep() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  if (buf_) (void)hipFree(buf_);
+  if (ctl_) (void)hipFree(ctl_);
+  if (host_) (void)hipHostFree(host_);
+}
+
+void DeviceStep::upload_core(const double* H, const double* X) {
+  const size_t bytes = sizeof(double) * (size_t)n_ * n_;
+  check(hipMemcpyAsync(H_, H, bytes, hipMemcpyHostToDevice, st_), "copy of H");
+  check(hipMemcpyAsync(X_, X, bytes, hipMemcpyHostToDevice, st_), "copy of X");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+template <bool TA, bool TB>
+static void gemm(hipStream_t st, int n, const double* A, const double* B, double* C, const Sp2Ctl* ctl) {
+  const dim3 grid(nb(n, 32), nb(n, 32));
+  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, st, n, n, n, 1.0, A, n, B, n, 0.0, nullptr, C, n, ctl);
+}
+
+void DeviceStep::density(bool with_xc, bool first, double mixing) {
+  const long nn = (long)n_ * n_;
+  Sp2Ctl* ctl = static_cast<Sp2Ctl*>(ctl_);
+  hipLaunchKernelGGL(k_fock, dim3(nb(nn, 256)), dim3(256), 0, st_, H_, J_, with_xc ? XC_ : nullptr, nn, F_);
+  // F' = X^T F X: Pn <- X^T F (A transposed), F' <- Pn X
+  gemm<true, false>(st_, n_, X_, F_, Pn_, nullptr);
+  gemm<false, false>(st_, n_, Pn_, X_, Fp_, nullptr);
+  // SP2 on F' into F (as P'), P2 as the square
+  hipLaunchKernelGGL(k_sp2_init, dim3(1), dim3(1024), 0, st_, Fp_, n_, F_, ctl);
+  const double tol = 1e-13 * n_;
+  // steps in groups, the control block read back between groups (a converged
+  // projection makes the rest of a group return at once)
+  Sp2Ctl* hc = reinterpret_cast<Sp2Ctl*>(host_ + 8);
+  for (int it = 0; it < kMaxSteps;) {
+    const int group = it == 0 ? 24 : 8;
+    for (int k = 0; k < group; ++k) {
+      gemm<false, true>(st_, n_, F_, F_, P2_, ctl);  // P' symmetric: B read transposed
+      hipLaunchKernelGGL(k_sp2_decide, dim3(1), dim3(256), 0, st_, F_, P2_, n_, nocc_, tol, kMaxSteps, ctl);
+      hipLaunchKernelGGL(k_sp2_apply, dim3(nb(nn, 256)), dim3(256), 0, st_, F_, P2_, nn, ctl);
+    }
+    it += group;
+    check(hipMemcpyAsync(hc, ctl_, sizeof(Sp2Ctl), hipMemcpyDeviceToHost, st_), "copy of the SP2 state");
+    check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+    if (hc->done) break;
+  }
+  // Pn = X P' X^T: P2 <- X P', Pn <- P2 X^T
+  gemm<false, false>(st_, n_, X_, F_, P2_, nullptr);
+  gemm<false, true>(st_, n_, P2_, X_, Pn_, nullptr);
+  check(hipMemcpyAsync(Pold_, P_, sizeof(double) * nn, hipMemcpyDeviceToDevice, st_), "copy of P");
+  hipLaunchKernelGGL(k_mix, dim3(nb(nn, 256)), dim3(256), 0, st_, P_, Pn_, nn, mixing, first ? 1 : 0);
+  check(hipGetLastError(), "launch of the density-matrix step");
+  check(hipMemcpyAsync(host_ + 8, ctl_, sizeof(Sp2Ctl), hipMemcpyDeviceToHost, st_), "copy of the SP2 state");
+}
+
+void DeviceStep::energy(double enuc, bool with_xc, bool point_ej) {
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, st_, P_, H_, J_, n_, scal_, enuc, with_xc ? 1 : 0,
+                     point_ej ? 1 : 0, eout_);
+  check(hipGetLastError(), "launch of the energy");
+  check(hipMemcpyAsync(host_, eout_, 5 * sizeof(double), hipMemcpyDeviceToHost, st_), "copy of the energy");
+}
+
+void DeviceStep::finish(double* e_one, double* e_j, double* e_xc, double* total, double* nelec, bool* sp2_ok,
+                        int* sp2_steps) {
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+  *e_one = host_[0];
+  *e_j = host_[1];
+  *e_xc = host_[2];
+  *total = host_[3];
+  *nelec = host_[4];
+  const Sp2Ctl* c = reinterpret_cast<const Sp2Ctl*>(host_ + 8);
+  *sp2_ok = c->done && !c->fail;
+  if (sp2_steps) *sp2_steps = c->iter;
+}
+
+void DeviceStep::download(double* dst, const double* src) {
+  check(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n_ * n_, hipMemcpyDeviceToHost, st_), "copy to host");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+void DeviceStep::upload(double* dst, const double* src) {
+  check(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n_ * n_, hipMemcpyHostToDevice, st_), "copy to device");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+}  // namespace dftcxx

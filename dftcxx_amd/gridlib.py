@@ -63,14 +63,15 @@ class _GridParams(C.Structure):
 
 class _ScfParams(C.Structure):
     _fields_ = [("threshold", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
-                ("mixing", C.c_double)]
+                ("mixing", C.c_double), ("density_step", C.c_int)]
 
 
 class _ScfResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("energy", C.c_double), ("e_one", C.c_double),
                 ("e_j", C.c_double), ("e_xc", C.c_double), ("e_nuc", C.c_double),
                 ("nelec", C.c_double), ("seconds_setup", C.c_double),
-                ("seconds_iter", C.c_double), ("converged", C.c_int), ("delta", C.c_double)]
+                ("seconds_iter", C.c_double), ("converged", C.c_int), ("delta", C.c_double),
+                ("device_step", C.c_int), ("host_fallbacks", C.c_int)]
 
 
 def lib():
@@ -551,13 +552,17 @@ def eigh(A):
     return w, Z.T.copy()
 
 
+STEP_AUTO, STEP_HOST, STEP_DEVICE = 0, 1, 2
+
+
 def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5,
-            precision=PREC_FP64, screen=0.0):
-    """Reference SCF (src/dft.cpp:84-142) with the grid work on the GPU.
+            precision=PREC_FP64, screen=0.0, density_step=STEP_AUTO):
+    """Reference SCF (src/dft.cpp:84-142) with the grid work on the GPU; the
+    density-matrix step on the host or the GPU (density_step, see dg_scf_params).
 
     Returns (energies per iteration, result dict, P)."""
     m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision, screen)
-    sp = _ScfParams(threshold, min_iter, max_iter, mixing)
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing, density_step)
     res = _ScfResult()
     en = np.zeros(max(int(max_iter), 1))
     n = mol.nbf()
@@ -576,7 +581,7 @@ def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter
     ins = [_c_inputs(m, nrad, nang, lmax, PREC_FP64) for m in mols]
     ms = (_Molecule * n)(*[i[0] for i in ins])
     bs = (_Basis * n)(*[i[1] for i in ins])
-    sp = _ScfParams(threshold, min_iter, max_iter, mixing)
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing, STEP_HOST)
     res = (_ScfResult * n)()
     it = C.c_int()
     _check(lib().dg_batch_scf_run(n, ms, bs, C.byref(ins[0][2]), C.byref(sp), res, C.byref(it)))

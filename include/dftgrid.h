@@ -184,11 +184,23 @@ int dg_mfma_probe(const double* a, const double* b, double* d);
 int dg_mfma_probe_f32(const float* a, const float* b, float* d);
 
 /* --- SCF driver (host C++, src/dft.cpp:84-142, 160-227, 300-366, 441-447) --- */
+#define DG_STEP_AUTO 0   /* device step for bases of at least DG_STEP_AUTO_NBF functions */
+#define DG_STEP_HOST 1   /* density-matrix step on the host (eigen-solver) */
+#define DG_STEP_DEVICE 2 /* density-matrix step and energy on the GPU */
+#define DG_STEP_AUTO_NBF 128
+
 typedef struct {
   double threshold;  /* stop when |dE| <= threshold (after min_iter) */
   int min_iter;      /* reference: 3 */
   int max_iter;      /* reference: 100 */
   double mixing;     /* reference: 0.5 (weight of the old P) */
+  /* Where the density-matrix step runs (no counterpart in the reference, which
+   * diagonalises on the host): DG_STEP_HOST diagonalises X^T F X on the host
+   * and copies P to and J, XC from the device each iteration; DG_STEP_DEVICE
+   * keeps F, P and the energy on the GPU and forms P as the projector onto
+   * the occupied space by second-order spectral projection (the same P up to
+   * rounding; steps without a HOMO-LUMO gap fall back to the host). */
+  int density_step;
 } dg_scf_params;
 
 typedef struct {
@@ -206,6 +218,8 @@ typedef struct {
                          * "Stopping because maximum number of iterations has been
                          * reached", src/dft.cpp:126-131) */
   double delta;         /* |dE| of the last iteration */
+  int device_step;      /* 1 if the density-matrix steps ran on the GPU */
+  int host_fallbacks;   /* device steps redone on the host (no gap) */
 } dg_scf_result;
 
 /* Runs the reference SCF with the grid work on the GPU.  energies (may be NULL)

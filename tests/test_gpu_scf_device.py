@@ -1,0 +1,63 @@
+"""The density-matrix step on the GPU (dg_scf_params.density_step = DG_STEP_DEVICE)
+against the host eigen-solver step and the oracle SCF.
+
+The device step forms the occupied projector by SP2 purification instead of
+diagonalising; with a HOMO-LUMO gap it is the same P up to rounding, so the
+energy sequences agree to well below the 1e-8 Ha of the FP64 parity bar.
+Without a gap (a carbon atom: three degenerate 2p orbitals, one occupied) the
+projection cannot converge and every step falls back to the host.
+"""
+import numpy as np
+import pytest
+
+from dftcxx_amd import gridlib as G
+from dftcxx_amd import molecule as M
+
+pytestmark = pytest.mark.gpu
+
+
+def test_h2o_device_step_vs_host_and_oracle():
+    from oracle import oracle as O
+    mol = M.h2o("sto3g")
+    en_h, r_h, P_h = G.scf_run(mol, 50, 110, 8, density_step=G.STEP_HOST)
+    en_d, r_d, P_d = G.scf_run(mol, 50, 110, 8, density_step=G.STEP_DEVICE)
+    assert r_d["device_step"] == 1 and r_h["device_step"] == 0
+    assert r_d["host_fallbacks"] == 0
+    assert len(en_d) == len(en_h) and r_d["converged"] == 1
+    assert np.abs(en_d - en_h).max() < 1e-10
+    assert np.abs(P_d - P_h).max() < 1e-9
+    og = O.Grid(mol, 50, 110, 8)
+    en_o, terms_o, P_o = og.scf(threshold=1e-6)
+    assert len(en_d) == len(en_o)
+    assert np.abs(en_d - en_o).max() < 1e-8
+    assert abs(r_d["e_j"] - terms_o["J"]) < 1e-8 and abs(r_d["e_xc"] - terms_o["xc"]) < 1e-8
+
+
+@pytest.mark.parametrize("make,grid", [(M.benzene, (30, 110, 6)), (lambda: M.alkane(10), (20, 50, 4))],
+                         ids=["benzene", "c10h22"])
+def test_device_step_vs_host(make, grid):
+    mol = make()
+    print(f"device step on {mol.name}: {mol.nbf()} functions", flush=True)
+    en_h, r_h, P_h = G.scf_run(mol, *grid, max_iter=8, threshold=0.0, density_step=G.STEP_HOST)
+    en_d, r_d, P_d = G.scf_run(mol, *grid, max_iter=8, threshold=0.0, density_step=G.STEP_DEVICE)
+    assert r_d["host_fallbacks"] == 0
+    assert len(en_d) == len(en_h) == 8
+    assert np.abs(en_d - en_h).max() < 1e-9
+    assert np.abs(P_d - P_h).max() < 1e-8
+    assert abs(r_d["nelec"] - mol.nelec) < 1e-9
+
+
+def test_device_step_mixed_precision():
+    mol = M.alkane(4)
+    en_h, r_h, _ = G.scf_run(mol, 20, 50, 4, precision=G.PREC_MIXED, density_step=G.STEP_HOST)
+    en_d, r_d, _ = G.scf_run(mol, 20, 50, 4, precision=G.PREC_MIXED, density_step=G.STEP_DEVICE)
+    assert len(en_d) == len(en_h)
+    assert np.abs(en_d - en_h).max() < 1e-9
+
+
+def test_no_gap_falls_back_to_host():
+    mol = M.Molecule([6], [[0.0, 0.0, 0.0]], "sto3g", "C atom")
+    en_h, r_h, _ = G.scf_run(mol, 20, 50, 4, max_iter=6, threshold=0.0, density_step=G.STEP_HOST)
+    en_d, r_d, _ = G.scf_run(mol, 20, 50, 4, max_iter=6, threshold=0.0, density_step=G.STEP_DEVICE)
+    assert r_d["host_fallbacks"] >= 1
+    assert np.abs(en_d - en_h).max() < 1e-10
