@@ -1776,6 +1776,12 @@ __global__ void k_phi_grad(const double* __restrict__ atoms, int nshell, const i
   }
 }
 
+// Weights of the two-electron Fock part G = 2J + XC = Phi^T diag(2 uJ + uxc) Phi.
+__global__ void k_fock_weights(const double* __restrict__ uJ, const double* __restrict__ uxc, long n, double* uG) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) uG[i] = 2.0 * uJ[i] + uxc[i];
+}
+
 // x[i] = v for i < n.
 __global__ void k_fill(double* x, long n, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2242,6 +2248,7 @@ struct dg_handle {
   double *pr_alpha = nullptr, *pr_cn0 = nullptr, *pr_cn1 = nullptr;
   double *P = nullptr, *pfrag = nullptr, *rho = nullptr, *uxc = nullptr, *exd = nullptr;
   double *uJ = nullptr, *V = nullptr, *vcell = nullptr, *vpart = nullptr;
+  double* uG = nullptr;  // 2 uJ + uxc (dg_build_fock_device, allocated on first use)
   double *q = nullptr, *qc = nullptr, *exc_part = nullptr, *scalars = nullptr;
   double* qpart = nullptr;  // split cell populations (cell_split > 1)
   int cell_split = 1;
@@ -2297,7 +2304,7 @@ struct dg_handle {
                       V,     vcell, vpart, q,      qc,    exc_part, scalars, ylm, minvT, shc,
                       knots, invh,  KD,    rho_lm, U,     tab,  cpart, J,   XC,
                       adj_ub, adj_ab, adj_udp, adj_T, adj_frag, adj_part, adj_mot, adj_lam, adj_out, adj_dP,
-                      mol_z, mol_fac, mol_qraw, atom_exc, atom_ej, qpart};
+                      mol_z, mol_fac, mol_qraw, atom_exc, atom_ej, qpart, uG};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, plen, ash_start, ash_list,
@@ -3455,6 +3462,23 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
   DG_TRY(stage_hartree(h));
   const ScalarArgs sa = scalar_args(h, true, dscalars);
   DG_TRY(stage_contract(h, h->uxc, h->uJ, dXC ? dXC : h->XC, dJ ? dJ : h->J, &sa));
+  h->has_hartree = h->has_xc = true;
+  return DG_OK;
+}
+
+int dg_build_fock_device(dg_handle* h, const double* dP, double* dG, double* dscalars) {
+  if (!h || !dP || !dG) return fail(DG_ERR_INVALID, "dg_build_fock_device: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_build_fock_device: not for a batch handle");
+  if (!h->uG) DG_TRY(dalloc(&h->uG, (size_t)h->npts_pad));
+  DG_TRY(stage_density(h, dP));
+  DG_TRY(stage_correct_lda(h));
+  DG_TRY(stage_hartree(h));
+  DG_TRY(run(h, "fock_weights", [&] {
+    hipLaunchKernelGGL(k_fock_weights, dim3(nblocks(h->npts_pad, 256)), dim3(256), 0, h->stream, h->uJ, h->uxc,
+                       h->npts_pad, h->uG);
+  }));
+  const ScalarArgs sa = scalar_args(h, true, dscalars);
+  DG_TRY(stage_contract(h, h->uG, nullptr, dG, nullptr, &sa));
   h->has_hartree = h->has_xc = true;
   return DG_OK;
 }

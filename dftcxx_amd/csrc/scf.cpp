@@ -760,10 +760,12 @@ class DFT : public HostSCF {
 
  private:
   // One iteration with everything on the device: density step, grid build, energy.
+  // The set-up build (no XC) forms J alone; every later build forms
+  // G = 2J + XC in one contraction (dg_build_fock_device), the only part of
+  // them the next density step and the energy (E_J pointwise) use.
   void device_iteration(bool with_xc) {
-    ds_->density(with_xc, first_, mixing);
-    dg_throw(dg_build_device(grid_.handle(), ds_->dP(), ds_->dJ(), with_xc ? ds_->dXC() : nullptr, ds_->dscal()));
-    ds_->energy(enuc_, with_xc, mixed_);
+    ds_->density(with_xc, first_, mixing, fused_);
+    device_build(with_xc);
     bool ok = false;
     ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
     if (!ok) {
@@ -771,16 +773,28 @@ class DFT : public HostSCF {
       // the build and the energy again
       ++host_fallbacks;
       ds_->download(P_.data(), ds_->dPold());
-      ds_->download(J_.data(), ds_->dJ());
-      if (with_xc) ds_->download(XC_.data(), ds_->dXC());
+      if (fused_) {  // dXC holds G = 2J + XC
+        std::fill(J_.begin(), J_.end(), 0.0);
+        ds_->download(XC_.data(), ds_->dXC());
+      } else {
+        ds_->download(J_.data(), ds_->dJ());
+        if (with_xc) ds_->download(XC_.data(), ds_->dXC());
+      }
       HostSCF::density_matrix();
       ds_->upload(ds_->dP(), P_.data());
-      dg_throw(dg_build_device(grid_.handle(), ds_->dP(), ds_->dJ(), with_xc ? ds_->dXC() : nullptr,
-                               ds_->dscal()));
-      ds_->energy(enuc_, with_xc, mixed_);
+      device_build(with_xc);
       ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
     }
     first_ = false;
+    fused_ = with_xc;
+  }
+
+  void device_build(bool with_xc) {
+    if (with_xc)
+      dg_throw(dg_build_fock_device(grid_.handle(), ds_->dP(), ds_->dXC(), ds_->dscal()));
+    else
+      dg_throw(dg_build_device(grid_.handle(), ds_->dP(), ds_->dJ(), nullptr, ds_->dscal()));
+    ds_->energy(enuc_, with_xc, with_xc || mixed_);
   }
 
   // set_density + correct_densities + Hartree J (+ XC) on the GPU
@@ -806,6 +820,7 @@ class DFT : public HostSCF {
 
   MolecularGrid grid_;
   bool on_device_ = false;
+  bool fused_ = false;  // the last device build formed G = 2J + XC
   std::unique_ptr<DeviceStep> ds_;
   double* dP_ = nullptr;
   double* hbuf_ = nullptr;

@@ -21,9 +21,10 @@ class DeviceStep {
   DeviceStep& operator=(const DeviceStep&) = delete;
 
   void upload_core(const double* H, const double* X);
-  // F = H + 2J (+ XC when with_xc); P <- projector of X^T F X, mixed into P
-  // unless first.  Asynchronous; the projection's outcome comes with finish().
-  void density(bool with_xc, bool first, double mixing);
+  // F = H + 2J (+ XC when with_xc), or F = H + G when fused (dXC() holding
+  // G = 2J + XC); P <- projector of X^T F X, mixed into P unless first.
+  // Asynchronous; the projection's outcome comes with finish().
+  void density(bool with_xc, bool first, double mixing, bool fused);
   // energy of the current P with the last build's J and scalars (asynchronous)
   void energy(double enuc, bool with_xc, bool point_ej);
   // waits for the stream; energy terms of energy() and the state of the last density()
@@ -34,7 +35,7 @@ class DeviceStep {
 
   double* dP() { return P_; }
   double* dJ() { return J_; }
-  double* dXC() { return XC_; }
+  double* dXC() { return XC_; }  // XC, or G = 2J + XC of a fused build
   double* dscal() { return scal_; }
   double* dPold() { return Pold_; }  // P before the last density()
 

@@ -77,11 +77,12 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha,
   }
 }
 
-// F = H + 2J + XC (src/dft.cpp:334); XC may be NULL (the set-up step, before any XC).
+// F = H + 2J + XC (src/dft.cpp:334), or F = H + G with G = 2J + XC from one
+// contraction (J NULL); XC may be NULL (the set-up step, before any XC).
 __global__ void k_fock(const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ XC,
                        long nn, double* F) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < nn) F[e] = H[e] + 2.0 * J[e] + (XC ? XC[e] : 0.0);
+  if (e < nn) F[e] = H[e] + (J ? 2.0 * J[e] : 0.0) + (XC ? XC[e] : 0.0);
 }
 
 // Gershgorin bounds of the symmetric F' and P0 = (h_max I - F') / (h_max - h_min);
@@ -265,10 +266,13 @@ static void gemm(hipStream_t st, int n, const double* A, const double* B, double
   hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, st, n, n, n, 1.0, A, n, B, n, 0.0, nullptr, C, n, ctl);
 }
 
-void DeviceStep::density(bool with_xc, bool first, double mixing) {
+void DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
   const long nn = (long)n_ * n_;
   Sp2Ctl* ctl = static_cast<Sp2Ctl*>(ctl_);
-  hipLaunchKernelGGL(k_fock, dim3(nb(nn, 256)), dim3(256), 0, st_, H_, J_, with_xc ? XC_ : nullptr, nn, F_);
+  if (fused)  // XC_ holds G = 2J + XC of the last build
+    hipLaunchKernelGGL(k_fock, dim3(nb(nn, 256)), dim3(256), 0, st_, H_, nullptr, XC_, nn, F_);
+  else
+    hipLaunchKernelGGL(k_fock, dim3(nb(nn, 256)), dim3(256), 0, st_, H_, J_, with_xc ? XC_ : nullptr, nn, F_);
   // F' = X^T F X: Pn <- X^T F (A transposed), F' <- Pn X
   gemm<true, false>(st_, n_, X_, F_, Pn_, nullptr);
   gemm<false, false>(st_, n_, Pn_, X_, Fp_, nullptr);

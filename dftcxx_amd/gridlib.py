@@ -35,7 +35,7 @@ EXPORTS = (
     "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
     "dg_batch_autotune", "dg_batch_scf_run", "dg_input_read", "dg_input_destroy", "dg_input_molecule",
     "dg_input_basis", "dg_input_grid", "dg_input_value", "dg_get_amplitude_gradients",
-    "dg_get_density_gradients", "dg_density_at_points",
+    "dg_get_density_gradients", "dg_density_at_points", "dg_build_fock_device",
 )
 
 
@@ -108,6 +108,7 @@ def lib():
         L.dg_get_xc_parts.argtypes = [C.c_void_p, _D, _D]
         L.dg_xc_adjoint.argtypes = [C.c_void_p, C.c_int, _D, _D]
         L.dg_build_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dg_build_fock_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.dg_profile_read.argtypes = [C.c_void_p, C.c_int, C.c_char_p, _D, _I, _I]
         L.dg_profile_reset.argtypes = [C.c_void_p]
@@ -370,6 +371,11 @@ class MolecularGrid:
         self._need()
         _check(lib().dg_build_device(self.h, C.c_void_p(dP), C.c_void_p(dJ), C.c_void_p(dXC),
                                      C.c_void_p(dscalars)))
+
+    def build_fock_device(self, dP, dG, dscalars=0):
+        """Enqueue one build of G = 2J + XC on device pointers (ints), see dg_build_fock_device."""
+        self._need()
+        _check(lib().dg_build_fock_device(self.h, C.c_void_p(dP), C.c_void_p(dG), C.c_void_p(dscalars)))
 
     def profile_enable(self, on=True):
         self._need()

@@ -164,6 +164,13 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad);
  * sum_p w_p V_p rho_p / (2c)}.  Any output pointer may be NULL. */
 int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, double* dscalars);
 
+/* The same build for an SCF step that needs only the two-electron part of the
+ * Fock matrix F = H + 2J + XC (src/dft.cpp:334): dG (device, nbf*nbf) receives
+ * G = 2J + XC from one contraction, Phi^T diag(w V + w v_xc) Phi, instead of
+ * two; dscalars as dg_build_device (E_J evaluated pointwise).  No counterpart
+ * in the reference, which forms J and XC separately. */
+int dg_build_fock_device(dg_handle* h, const double* dP, double* dG, double* dscalars);
+
 /* --- profiling ------------------------------------------------------------- */
 /* When enabled, every kernel of dg_build_device is bracketed by HIP events on
  * the handle's stream and its time accumulated per kernel name. */

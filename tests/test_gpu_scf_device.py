@@ -61,3 +61,30 @@ def test_no_gap_falls_back_to_host():
     en_d, r_d, _ = G.scf_run(mol, 20, 50, 4, max_iter=6, threshold=0.0, density_step=G.STEP_DEVICE)
     assert r_d["host_fallbacks"] >= 1
     assert np.abs(en_d - en_h).max() < 1e-10
+
+
+@pytest.mark.parametrize("make,prec", [(M.benzene, G.PREC_FP64), (lambda: M.alkane(10), G.PREC_FP64),
+                                       (lambda: M.alkane(10), G.PREC_MIXED)], ids=["benzene", "c10h22", "c10h22_mixed"])
+def test_fock_build_is_2j_plus_xc(make, prec):
+    import torch
+    from systems import random_density
+    mol = make()
+    g = G.MolecularGrid(mol, 20, 50, 4, precision=prec)
+    n = mol.nbf()
+    dev = torch.device("cuda:0")
+    P = torch.tensor(random_density(n, 7).T.copy(), dtype=torch.float64, device=dev)
+    J = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    XC = torch.zeros_like(J)
+    Gm = torch.zeros_like(J)
+    s1 = torch.zeros(4, dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s1)
+    torch.cuda.synchronize()
+    g.build_device(P.data_ptr(), J.data_ptr(), XC.data_ptr(), s1.data_ptr())
+    g.build_fock_device(P.data_ptr(), Gm.data_ptr(), s2.data_ptr())
+    g.close()
+    torch.cuda.synchronize()
+    ref = (2 * J + XC).cpu().numpy()
+    got = Gm.cpu().numpy()
+    tol = 1e-12 if prec == G.PREC_FP64 else 1e-7
+    assert np.abs(got - ref).max() < tol * np.abs(ref).max()
+    assert np.array_equal(s1.cpu().numpy(), s2.cpu().numpy())
