@@ -3813,3 +3813,4 @@ int dg_selftest_mfma(int* ok) {
 }  // extern "C"
 
 #include "batch.inl"
+#include "noalloc.inl"

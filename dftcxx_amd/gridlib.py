@@ -35,7 +35,7 @@ EXPORTS = (
     "dg_batch_handle", "dg_batch_set_stream", "dg_batch_build_device", "dg_batch_build",
     "dg_batch_autotune", "dg_batch_scf_run", "dg_input_read", "dg_input_destroy", "dg_input_molecule",
     "dg_input_basis", "dg_input_grid", "dg_input_value", "dg_get_amplitude_gradients",
-    "dg_get_density_gradients", "dg_density_at_points", "dg_build_fock_device",
+    "dg_get_density_gradients", "dg_density_at_points", "dg_density_at_points_noalloc", "dg_build_fock_device",
 )
 
 
@@ -133,6 +133,8 @@ def lib():
                                        C.POINTER(_GridParams), C.POINTER(_ScfParams),
                                        C.POINTER(_ScfResult), _I]
         L.dg_density_at_points.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.c_long, _D, _D, _D, _D]
+        L.dg_density_at_points_noalloc.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.c_long, _D, _D, _D,
+                                                   _D]
         L.dg_input_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
         L.dg_input_destroy.argtypes = [C.c_void_p]
         L.dg_input_molecule.argtypes = [C.c_void_p, C.POINTER(_Molecule)]
@@ -512,14 +514,16 @@ def one_electron(mol):
     return S.T.copy(), T.T.copy(), V.T.copy()
 
 
-def density_at_points(mol, xyz, P):
-    """(rho [n], grad rho [n, 3]) at arbitrary points (bohr) on the GPU (dg_density_at_points)."""
+def density_at_points(mol, xyz, P, noalloc=False):
+    """(rho [n], grad rho [n, 3]) at arbitrary points (bohr) on the GPU (dg_density_at_points,
+    or dg_density_at_points_noalloc, which allocates no device memory)."""
     m, b, g, keep = _c_inputs(mol, 5, 6, 0, PREC_FP64)
     xyz = _f64(np.asarray(xyz).reshape(-1, 3))
     n = len(xyz)
     Pc = _f64(np.asarray(P).T)
     rho, grad = np.empty(n), np.empty((n, 3))
-    _check(lib().dg_density_at_points(C.byref(m), C.byref(b), n, _d(xyz), _d(Pc), _d(rho), _d(grad)))
+    fn = lib().dg_density_at_points_noalloc if noalloc else lib().dg_density_at_points
+    _check(fn(C.byref(m), C.byref(b), n, _d(xyz), _d(Pc), _d(rho), _d(grad)))
     return rho, grad
 
 

@@ -140,6 +140,20 @@ int dg_get_density_gradients(dg_handle* h, double* grad);
  * synchronises. */
 int dg_density_at_points(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz,
                          const double* P, double* rho, double* grad);
+/* The same evaluation without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * runs beside HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its work buffers live in a 32 MiB array in device memory that
+ * the library holds on every device it runs on until the process ends; the
+ * call allocates nothing on the host heap either on its success path.  Same
+ * arguments and results as dg_density_at_points; bases of at most 512
+ * functions, 512 shells, 4096 primitives and 1024 atoms (DG_ERR_UNSUPPORTED
+ * beyond), points in passes that fit the array.  The array keeps the last
+ * call's contents; calls are serialised within the process (calls that
+ * overlapped on one device would race on its array).  Runs on the calling
+ * thread's default stream (hipStreamPerThread) and synchronises. */
+int dg_density_at_points_noalloc(const dg_molecule* mol, const dg_basis* basis, long npts, const double* xyz,
+                                 const double* P, double* rho, double* grad);
 
 /* --- adjoint of E_xc ------------------------------------------------------ */
 /* No counterpart in the reference: its GTO gradient (src/cgf.cpp:67-94,
