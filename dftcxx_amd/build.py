@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdftgrid.so")
 
-SOURCES = ["dftgrid.hip", "scf_device.hip", "grid_tables.cpp", "scf.cpp", "input.cpp"]
+SOURCES = ["dftgrid.hip", "scf_device.hip", "eri.hip", "grid_tables.cpp", "scf.cpp", "input.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

@@ -3466,6 +3466,17 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
   return DG_OK;
 }
 
+int dg_build_xc_device(dg_handle* h, const double* dP, double* dXC, double* dscalars) {
+  if (!h || !dP || !dXC) return fail(DG_ERR_INVALID, "dg_build_xc_device: null argument");
+  if (h->nmol) return fail(DG_ERR_STATE, "dg_build_xc_device: not for a batch handle");
+  DG_TRY(stage_density(h, dP));
+  DG_TRY(stage_correct_lda(h));
+  const ScalarArgs sa = scalar_args(h, false, dscalars);
+  DG_TRY(stage_contract(h, h->uxc, nullptr, dXC, nullptr, &sa));
+  h->has_xc = true;
+  return DG_OK;
+}
+
 int dg_build_fock_device(dg_handle* h, const double* dP, double* dG, double* dscalars) {
   if (!h || !dP || !dG) return fail(DG_ERR_INVALID, "dg_build_fock_device: null argument");
   if (h->nmol) return fail(DG_ERR_STATE, "dg_build_fock_device: not for a batch handle");

@@ -652,7 +652,8 @@ class HostSCF {
     e_one_ = 2.0 * t1;
     // In reduced precision J carries FP32-product errors that would enter 2 tr(PJ)
     // at first order; the same quantity evaluated pointwise in FP64 does not.
-    e_j_ = mixed_ ? ej_point_ : 2.0 * t2;
+    // A J from the two-electron integrals is FP64 in either mode.
+    e_j_ = (mixed_ && !eri_mode_) ? ej_point_ : 2.0 * t2;
     et_ = e_one_ + e_j_ + enuc_ + exc_;
   }
 
@@ -673,6 +674,15 @@ class HostSCF {
     r->seconds_iter = sec_iter;
   }
 
+  // for a step formed elsewhere (the batched device step): later steps mix
+  void mark_started() { first_ = false; }
+  bool first() const { return first_; }
+  int nocc() const { return nelec_ / 2; }
+  const std::vector<double>& H() const { return H_; }
+  const std::vector<double>& X() const { return X_; }
+  std::vector<double>& J() { return J_; }
+  std::vector<double>& XC() { return XC_; }
+
   double mixing = 0.5;
   double et_ = 0.0, e_one_ = 0.0, e_j_ = 0.0, exc_ = 0.0, enuc_ = 0.0, nelec_grid_ = 0.0;
   std::vector<double> P_;
@@ -681,6 +691,7 @@ class HostSCF {
  protected:
   MoleculeData mol_;
   bool mixed_ = false;
+  bool eri_mode_ = false;  // J from the analytic two-electron integrals
   double ej_point_ = 0.0;
   int nelec_ = 0;
   bool first_ = true;
@@ -695,19 +706,26 @@ class HostSCF {
 // does not converge (no HOMO-LUMO gap) is redone by the host eigen-solver.
 class DFT : public HostSCF {
  public:
-  DFT(const MoleculeData& mol, const dg_grid_params& grid, bool on_device)
+  DFT(const MoleculeData& mol, const dg_grid_params& grid, bool on_device, bool eri)
       : HostSCF(mol, grid.precision == DG_PREC_MIXED), grid_(mol), on_device_(on_device) {
+    eri_mode_ = eri;
     grid_.set_grid_points(grid.n_radial, grid.n_angular, grid.lmax, grid.precision, grid.screen);
     grid_.create_grid();
   }
   ~DFT() {
     ds_.reset();
+    if (eri_) dg_eri_destroy(eri_);
     if (dP_) (void)hipFree(dP_);
     if (hbuf_) (void)hipHostFree(hbuf_);
   }
 
   void construct_matrices() {
     HostSCF::construct_matrices();
+    if (eri_mode_) {  // DFT::construct_matrices computes every ERI once (src/dft.cpp:204-211)
+      dg_molecule m = mol_.c_molecule();
+      dg_basis b = mol_.c_basis();
+      dg_throw(dg_eri_create(&m, &b, &eri_));
+    }
     if (on_device_) {
       void* sp = nullptr;
       dg_throw(dg_get_stream(grid_.handle(), &sp));
@@ -764,15 +782,12 @@ class DFT : public HostSCF {
   // G = 2J + XC in one contraction (dg_build_fock_device), the only part of
   // them the next density step and the energy (E_J pointwise) use.
   void device_iteration(bool with_xc) {
-    ds_->density(with_xc, first_, mixing, fused_);
-    device_build(with_xc);
-    bool ok = false;
-    ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
-    if (!ok) {
-      // the projection did not converge: this step by the host eigen-solver, then
-      // the build and the energy again
+    if (!ds_->density(with_xc, first_, mixing, fused_)) {
+      // no gap (degenerate frontier orbitals): this step by the host eigen-solver
+      // from the same F, whose choice within the degenerate space is the one the
+      // host path makes
       ++host_fallbacks;
-      ds_->download(P_.data(), ds_->dPold());
+      ds_->download(P_.data(), ds_->dP());
       if (fused_) {  // dXC holds G = 2J + XC
         std::fill(J_.begin(), J_.end(), 0.0);
         ds_->download(XC_.data(), ds_->dXC());
@@ -782,14 +797,23 @@ class DFT : public HostSCF {
       }
       HostSCF::density_matrix();
       ds_->upload(ds_->dP(), P_.data());
-      device_build(with_xc);
-      ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_, &ok, nullptr);
     }
+    device_build(with_xc);
+    ds_->finish(&e_one_, &e_j_, &exc_, &et_, &nelec_grid_);
     first_ = false;
-    fused_ = with_xc;
+    fused_ = with_xc && !eri_mode_;
   }
 
   void device_build(bool with_xc) {
+    if (eri_mode_) {  // J from the integrals, the grid for rho and XC
+      void* sp = nullptr;
+      dg_throw(dg_get_stream(grid_.handle(), &sp));
+      dg_throw(dg_eri_coulomb_device(eri_, ds_->dP(), ds_->dJ(), sp));
+      dg_throw(dg_build_xc_device(grid_.handle(), ds_->dP(), ds_->dXC(), ds_->dscal()));
+      if (!with_xc) ds_->clear_xc();
+      ds_->energy(enuc_, with_xc, false);
+      return;
+    }
     if (with_xc)
       dg_throw(dg_build_fock_device(grid_.handle(), ds_->dP(), ds_->dXC(), ds_->dscal()));
     else
@@ -811,7 +835,12 @@ class DFT : public HostSCF {
     std::memcpy(hP, P_.data(), nn * 8);
     if (hipMemcpyAsync(dP_, hP, nn * 8, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("copy of P failed");
-    dg_throw(dg_build_device(grid_.handle(), dP_, dJ, dXC, dS));
+    if (eri_mode_) {
+      dg_throw(dg_eri_coulomb_device(eri_, dP_, dJ, st));
+      dg_throw(dg_build_xc_device(grid_.handle(), dP_, dXC, dS));
+    } else {
+      dg_throw(dg_build_device(grid_.handle(), dP_, dJ, dXC, dS));
+    }
     if (hipMemcpyAsync(hJ, dJ, (2 * nn + 4) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       throw std::runtime_error("copy of grid results failed");
@@ -821,6 +850,7 @@ class DFT : public HostSCF {
   MolecularGrid grid_;
   bool on_device_ = false;
   bool fused_ = false;  // the last device build formed G = 2J + XC
+  dg_eri* eri_ = nullptr;
   std::unique_ptr<DeviceStep> ds_;
   double* dP_ = nullptr;
   double* hbuf_ = nullptr;
@@ -895,6 +925,95 @@ int batch_scf(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& 
   return iteration;
 }
 
+// The batched SCF with every molecule's density-matrix step and energy on the
+// GPU (BatchStep, one work-group per molecule): per iteration the step, one
+// batch build (a graph launch) and the energies, with only the energies and the
+// per-molecule projection flags read back; a molecule whose projection has no
+// gap has that step formed on the host.  Same loop and convergence rule as
+// batch_scf.
+int batch_scf_device(std::vector<HostSCF>& mols, dg_batch* batch, const dg_scf_params& prm, int* iterations,
+                     double* sec_iter, std::vector<char>& converged, std::vector<double>& diff, int* fallbacks) {
+  const int nm = (int)mols.size();
+  std::vector<int> nbf(nm), nocc(nm);
+  std::vector<long> off(nm + 1);
+  dg_throw(dg_batch_layout(batch, nbf.data(), off.data(), nullptr));
+  for (int m = 0; m < nm; ++m) nocc[m] = mols[m].nocc();
+  dg_handle* h = nullptr;
+  dg_throw(dg_batch_handle(batch, &h));
+  void* sp = nullptr;
+  dg_throw(dg_get_stream(h, &sp));
+  BatchStep bs(nm, nbf.data(), off.data(), nocc.data(), (hipStream_t)sp);
+  {
+    std::vector<double> Hp(off[nm]), Xp(off[nm]), en(nm);
+    for (int m = 0; m < nm; ++m) {
+      std::memcpy(&Hp[off[m]], mols[m].H().data(), sizeof(double) * nbf[m] * nbf[m]);
+      std::memcpy(&Xp[off[m]], mols[m].X().data(), sizeof(double) * nbf[m] * nbf[m]);
+      en[m] = mols[m].enuc_;
+    }
+    bs.upload_core(Hp.data(), Xp.data(), en.data());
+  }
+  std::vector<char> active(nm, 1);
+  bs.set_active(active.data());
+  *fallbacks = 0;
+  auto step = [&](bool with_xc, bool first) {
+    if (bs.density(with_xc, first, mols[0].mixing) > 0) {
+      const int* f = bs.failed();
+      for (int m = 0; m < nm; ++m) {
+        if (!active[m] || !f[m]) continue;
+        ++*fallbacks;
+        const size_t nn = (size_t)nbf[m] * nbf[m];
+        bs.download(mols[m].P_.data(), bs.dP() + off[m], nn);
+        bs.download(mols[m].J().data(), bs.dJ() + off[m], nn);
+        if (with_xc) bs.download(mols[m].XC().data(), bs.dXC() + off[m], nn);
+        mols[m].density_matrix();
+        bs.upload(bs.dP() + off[m], mols[m].P_.data(), nn);
+      }
+    }
+    for (HostSCF& hm : mols) hm.mark_started();
+    dg_throw(dg_batch_build_device(batch, bs.dP(), bs.dJ(), bs.dXC(), bs.dscal()));
+    if (!with_xc) bs.clear_xc();  // the set-up step has no XC yet (src/dft.cpp:218-226)
+    bs.energy(with_xc);
+    bs.finish();
+    const double* e = bs.energies();
+    for (int m = 0; m < nm; ++m) {
+      if (!active[m]) continue;
+      mols[m].e_one_ = e[5 * m];
+      mols[m].e_j_ = e[5 * m + 1];
+      mols[m].exc_ = e[5 * m + 2];
+      mols[m].et_ = e[5 * m + 3];
+      mols[m].nelec_grid_ = e[5 * m + 4];
+    }
+  };
+  step(false, true);
+  std::vector<double> old(nm);
+  diff.assign(nm, 1.0);
+  converged.assign(nm, 0);
+  for (int m = 0; m < nm; ++m) old[m] = mols[m].et_;
+  int iteration = 0, done = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  while (done < nm) {
+    iteration++;
+    step(true, false);
+    bool changed = false;
+    for (int m = 0; m < nm; ++m) {
+      if (!active[m]) continue;
+      diff[m] = std::fabs(mols[m].et_ - old[m]);
+      old[m] = mols[m].et_;
+      iterations[m] = iteration;
+      converged[m] = diff[m] <= prm.threshold && iteration >= prm.min_iter;
+      if (converged[m] || iteration >= prm.max_iter) {
+        active[m] = 0;
+        changed = true;
+        ++done;
+      }
+    }
+    if (changed) bs.set_active(active.data());
+  }
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (sec_iter) *sec_iter = iteration ? dt / iteration : 0.0;
+  return iteration;
+}
+
 }  // namespace
 }  // namespace dftcxx
 
@@ -926,6 +1045,8 @@ bool check_scf_params(const dg_scf_params* p, const char* who) {
   else if (!(p->mixing >= 0.0 && p->mixing < 1.0)) bad = "mixing must lie in [0, 1)";
   else if (p->density_step < DG_STEP_AUTO || p->density_step > DG_STEP_DEVICE)
     bad = "density_step must be DG_STEP_AUTO, DG_STEP_HOST or DG_STEP_DEVICE";
+  else if (p->hartree != DG_HARTREE_BECKE && p->hartree != DG_HARTREE_ERI)
+    bad = "hartree must be DG_HARTREE_BECKE or DG_HARTREE_ERI";
   if (bad) dg::set_last_error(std::string(who) + ": " + bad);
   return bad == nullptr;
 }
@@ -975,7 +1096,7 @@ extern "C" int dg_scf_run(const dg_molecule* mol, const dg_basis* basis, const d
     const auto t0 = std::chrono::steady_clock::now();
     const bool dev = scf->density_step == DG_STEP_DEVICE ||
                      (scf->density_step == DG_STEP_AUTO && m.nbf() >= DG_STEP_AUTO_NBF);
-    dftcxx::DFT dft(m, *grid, dev);
+    dftcxx::DFT dft(m, *grid, dev, scf->hartree == DG_HARTREE_ERI);
     dft.mixing = scf->mixing;
     dft.construct_matrices();
     const double setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1001,6 +1122,10 @@ extern "C" int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basi
     return DG_ERR_INVALID;
   }
   if (!check_scf_params(scf, "dg_batch_scf_run")) return DG_ERR_INVALID;
+  if (scf->hartree != DG_HARTREE_BECKE) {
+    dg::set_last_error("dg_batch_scf_run: batches use the Becke-grid Hartree path");
+    return DG_ERR_UNSUPPORTED;
+  }
   dg_batch* batch = nullptr;
   try {
     const auto t0 = std::chrono::steady_clock::now();
@@ -1019,8 +1144,15 @@ extern "C" int dg_batch_scf_run(int nmol, const dg_molecule* mols, const dg_basi
     double sec_iter = 0.0;
     std::vector<char> conv;
     std::vector<double> delta;
-    const int total = dftcxx::batch_scf(dfts, batch, *scf, its.data(), &sec_iter, conv, delta);
-    for (int m = 0; m < nmol; ++m) dfts[m].fill(&results[m], its[m], setup, sec_iter, conv[m] != 0, delta[m]);
+    const bool dev = scf->density_step != DG_STEP_HOST;
+    int fallbacks = 0;
+    const int total = dev ? dftcxx::batch_scf_device(dfts, batch, *scf, its.data(), &sec_iter, conv, delta, &fallbacks)
+                          : dftcxx::batch_scf(dfts, batch, *scf, its.data(), &sec_iter, conv, delta);
+    for (int m = 0; m < nmol; ++m) {
+      dfts[m].fill(&results[m], its[m], setup, sec_iter, conv[m] != 0, delta[m]);
+      results[m].device_step = dev ? 1 : 0;
+      results[m].host_fallbacks = m == 0 ? fallbacks : 0;
+    }
     if (iterations) *iterations = total;
     dg_batch_destroy(batch);
     return DG_OK;

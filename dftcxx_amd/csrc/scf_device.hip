@@ -122,10 +122,13 @@ __global__ __launch_bounds__(1024) void k_sp2_init(const double* __restrict__ F,
   }
 }
 
-// One SP2 decision from tr(P) and tr(P^2) (fixed summation order): converged when
-// the idempotency error tr(P) - tr(P^2) falls below tol, or, once below 1e-9,
-// stops halving (the rounding floor); failed after maxit steps.  Else records
-// which update the next kernel applies (ctl->pad: 1 = P^2, 0 = 2P - P^2).
+// One SP2 decision from tr(P) and tr(P^2) (fixed summation order): the
+// projection stops once the idempotency error tr(P) - tr(P^2) falls below tol
+// (the eigenvalues of P are then within about tol of 0 or 1; two McWeeny steps
+// follow, k_mcweeny); it fails after maxit steps.  Else records which update
+// the next kernel applies (ctl->pad: 1 = P^2, 0 = 2P - P^2).  (A test on the
+// error's decrease from one step to the next would not do: a step squares the
+// eigenvalue errors on one side of the gap but doubles them on the other.)
 __global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P, const double* __restrict__ P2,
                                                     int n, int nocc, double tol, int maxit, Sp2Ctl* ctl) {
   if (ctl->done) return;
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P
   const double err = fabs(tr1 - tr2);
   ctl->err_prev = ctl->err;
   ctl->err = err;
-  if (err < tol || (err < 1e-9 && err > 0.5 * ctl->err_prev)) {
+  if (err < tol) {
     ctl->done = 1;
     return;
   }
@@ -162,6 +165,14 @@ __global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P
   ctl->iter += 1;
   // trace after P -> P^2 is tr2, after P -> 2P - P^2 it is 2 tr1 - tr2
   ctl->pad = fabs(tr2 - nocc) < fabs(2.0 * tr1 - tr2 - nocc) ? 1 : 0;
+}
+
+// dst = src elementwise: the result Q = 3 P^2 - 2 P^2 P of a McWeeny step
+// (which squares the eigenvalue errors on both sides of the gap, once the
+// projection has brought them within 1e-6 of 0 or 1) copied back over P.
+__global__ void k_copy(double* dst, const double* __restrict__ src, long nn) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nn) dst[e] = src[e];
 }
 
 __global__ void k_sp2_apply(double* P, const double* __restrict__ P2, long nn, const Sp2Ctl* __restrict__ ctl) {
@@ -214,6 +225,152 @@ __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ P, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched step: one work-group per molecule of a batch (n <= 32 functions),
+// every matrix in LDS.  The same algebra as the single-molecule step: F, X^T F X,
+// SP2 to an idempotency error of 1e-6 and two McWeeny steps, X P' X^T, mixing.
+// Matrices are packed column-major at off[m] (the batch layout).
+// ---------------------------------------------------------------------------
+constexpr int kBatchN = 32;
+constexpr int kBatchMaxSteps = 100;
+
+// C = A op(B) for n x n LDS matrices (column-major, leading dimension n);
+// TB reads B transposed.  All 256 threads; ends with a barrier.
+template <bool TA, bool TB>
+__device__ void lds_mm(int n, const double* A, const double* B, double* C) {
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += (TA ? A[k + i * n] : A[i + k * n]) * (TB ? B[j + k * n] : B[k + j * n]);
+    C[e] = s;
+  }
+  __syncthreads();
+}
+
+// sum over the work-group of v (fixed order); every thread gets the result
+__device__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// One density-matrix step of every active molecule; fail[m] = 1, with P_m left
+// as it was, when its projection did not reach the tolerance (no gap: the
+// caller forms that molecule's step on the host).
+__global__ __launch_bounds__(256) void k_batch_density(
+    const int* __restrict__ nbf, const long* __restrict__ off, const int* __restrict__ nocc,
+    const char* __restrict__ active, const double* __restrict__ H, const double* __restrict__ X,
+    const double* __restrict__ J, const double* __restrict__ XC, int with_xc, int first, double mixing, double* P,
+    int* fail) {
+  __shared__ double A[kBatchN * kBatchN], B[kBatchN * kBatchN], Cm[kBatchN * kBatchN], Xs[kBatchN * kBatchN];
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  if (!active[m]) return;
+  const int n = nbf[m], nn = n * n;
+  const long o = off[m];
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+    A[e] = H[o + e] + 2.0 * J[o + e] + (with_xc ? XC[o + e] : 0.0);
+    Xs[e] = X[o + e];
+  }
+  __syncthreads();
+  lds_mm<true, false>(n, Xs, A, B);   // X^T F
+  lds_mm<false, false>(n, B, Xs, A);  // F' = X^T F X
+  // Gershgorin bounds of F'
+  double lo = 1e300, hi = -1e300;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double r = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) r += fabs(A[j + i * n]);
+    lo = fmin(lo, A[i + i * n] - r);
+    hi = fmax(hi, A[i + i * n] + r);
+  }
+  red[threadIdx.x] = lo;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double hmin = red[0];
+  __syncthreads();
+  red[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double hmax = red[0];
+  __syncthreads();
+  const double inv = 1.0 / fmax(hmax - hmin, 1e-300);
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) B[e] = ((e % n == e / n ? hmax : 0.0) - A[e]) * inv;
+  __syncthreads();
+  // SP2 on B (P'), squares in A
+  const int no = nocc[m];
+  bool ok = false;
+  for (int it = 0; it < kBatchMaxSteps; ++it) {
+    lds_mm<false, true>(n, B, B, A);
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      t1 += B[i + i * n];
+      t2 += A[i + i * n];
+    }
+    const double tr1 = block_sum(t1, red), tr2 = block_sum(t2, red);
+    if (fabs(tr1 - tr2) < 1e-6) {
+      ok = true;
+      break;
+    }
+    const bool sq = fabs(tr2 - no) < fabs(2.0 * tr1 - tr2 - no);
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) B[e] = sq ? A[e] : 2.0 * B[e] - A[e];
+    __syncthreads();
+  }
+  // two McWeeny steps P <- 3P^2 - 2P^3
+  for (int k = 0; k < 2; ++k) {
+    lds_mm<false, true>(n, B, B, A);
+    lds_mm<false, true>(n, A, B, Cm);
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) B[e] = 3.0 * A[e] - 2.0 * Cm[e];
+    __syncthreads();
+  }
+  lds_mm<false, false>(n, Xs, B, A);  // X P'
+  lds_mm<false, true>(n, A, Xs, Cm);  // Pn = X P' X^T
+  if (ok)
+    for (int e = threadIdx.x; e < nn; e += blockDim.x)
+      P[o + e] = first ? Cm[e] : (1.0 - mixing) * Cm[e] + mixing * P[o + e];
+  if (threadIdx.x == 0) fail[m] = ok ? 0 : 1;
+}
+
+// Energy of every active molecule: out[5m..5m+4] = {E_one, E_J = 2 tr(PJ), E_xc
+// (scal[4m], when with_xc), total, N after correction (scal[4m+2])}.
+__global__ __launch_bounds__(256) void k_batch_energy(const int* __restrict__ nbf, const long* __restrict__ off,
+                                                      const char* __restrict__ active, const double* __restrict__ P,
+                                                      const double* __restrict__ H, const double* __restrict__ J,
+                                                      const double* __restrict__ scal,
+                                                      const double* __restrict__ enuc, int with_xc, double* out) {
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  if (!active[m]) return;
+  const int nn = nbf[m] * nbf[m];
+  const long o = off[m];
+  double t1 = 0.0, t2 = 0.0;
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+    t1 += P[o + e] * H[o + e];
+    t2 += P[o + e] * J[o + e];
+  }
+  const double s1 = block_sum(t1, red), s2 = block_sum(t2, red);
+  if (threadIdx.x == 0) {
+    const double exc = with_xc ? scal[4 * m] : 0.0;
+    out[5 * m] = 2.0 * s1;
+    out[5 * m + 1] = 2.0 * s2;
+    out[5 * m + 2] = exc;
+    out[5 * m + 3] = 2.0 * s1 + 2.0 * s2 + enuc[m] + exc;
+    out[5 * m + 4] = scal[4 * m + 2];
+  }
+}
+
 void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -226,8 +383,8 @@ namespace dftcxx {
 
 DeviceStep::DeviceStep(int n, int nocc, hipStream_t stream) : n_(n), nocc_(nocc), st_(stream) {
   const size_t nn = (size_t)n * n;
-  // H, X, F (also P'), F', Pn, P2 (also X P'), P, J, XC, P before the step, scalars(4), energy(5)
-  check(hipMalloc((void**)&buf_, sizeof(double) * (10 * nn + 16)), "hipMalloc of the SCF buffers");
+  // H, X, F (also P'), F', Pn, P2 (also X P'), P, J, XC, scalars(4), energy(5)
+  check(hipMalloc((void**)&buf_, sizeof(double) * (9 * nn + 16)), "hipMalloc of the SCF buffers");
   check(hipMalloc((void**)&ctl_, sizeof(Sp2Ctl)), "hipMalloc of the SCF control block");
   check(hipHostMalloc((void**)&host_, sizeof(double) * 8 + sizeof(Sp2Ctl), hipHostMallocDefault),
         "hipHostMalloc of the SCF staging");
@@ -240,10 +397,9 @@ DeviceStep::DeviceStep(int n, int nocc, hipStream_t stream) : n_(n), nocc_(nocc)
   P_ = P2_ + nn;
   J_ = P_ + nn;
   XC_ = J_ + nn;
-  Pold_ = XC_ + nn;
-  scal_ = Pold_ + nn;
+  scal_ = XC_ + nn;
   eout_ = scal_ + 4;
-  check(hipMemsetAsync(buf_, 0, sizeof(double) * (10 * nn + 16), st_), "hipMemsetAsync");
+  check(hipMemsetAsync(buf_, 0, sizeof(double) * (9 * nn + 16), st_), "hipMemsetAsync");
 }
 
 DeviceStep::~DeviceStep() {
@@ -266,7 +422,7 @@ static void gemm(hipStream_t st, int n, const double* A, const double* B, double
   hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, st, n, n, n, 1.0, A, n, B, n, 0.0, nullptr, C, n, ctl);
 }
 
-void DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
+bool DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
   const long nn = (long)n_ * n_;
   Sp2Ctl* ctl = static_cast<Sp2Ctl*>(ctl_);
   if (fused)  // XC_ holds G = 2J + XC of the last build
@@ -278,7 +434,7 @@ void DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
   gemm<false, false>(st_, n_, Pn_, X_, Fp_, nullptr);
   // SP2 on F' into F (as P'), P2 as the square
   hipLaunchKernelGGL(k_sp2_init, dim3(1), dim3(1024), 0, st_, Fp_, n_, F_, ctl);
-  const double tol = 1e-13 * n_;
+  const double tol = 1e-6;
   // steps in groups, the control block read back between groups (a converged
   // projection makes the rest of a group return at once)
   Sp2Ctl* hc = reinterpret_cast<Sp2Ctl*>(host_ + 8);
@@ -294,13 +450,22 @@ void DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
     check(hipStreamSynchronize(st_), "hipStreamSynchronize");
     if (hc->done) break;
   }
+  // no gap: P is left as it was, the caller forms this step on the host
+  if (!hc->done || hc->fail) return false;
+  // two McWeeny steps: eigenvalue errors 1e-6 -> 1e-12 -> rounding
+  for (int k = 0; k < 2; ++k) {
+    gemm<false, true>(st_, n_, F_, F_, P2_, nullptr);
+    const dim3 grid(nb(n_, 32), nb(n_, 32));
+    hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, st_, n_, n_, n_, -2.0, P2_, n_, F_, n_, 3.0, P2_,
+                       Fp_, n_, (const Sp2Ctl*)nullptr);
+    hipLaunchKernelGGL(k_copy, dim3(nb(nn, 256)), dim3(256), 0, st_, F_, Fp_, nn);
+  }
   // Pn = X P' X^T: P2 <- X P', Pn <- P2 X^T
   gemm<false, false>(st_, n_, X_, F_, P2_, nullptr);
   gemm<false, true>(st_, n_, P2_, X_, Pn_, nullptr);
-  check(hipMemcpyAsync(Pold_, P_, sizeof(double) * nn, hipMemcpyDeviceToDevice, st_), "copy of P");
   hipLaunchKernelGGL(k_mix, dim3(nb(nn, 256)), dim3(256), 0, st_, P_, Pn_, nn, mixing, first ? 1 : 0);
   check(hipGetLastError(), "launch of the density-matrix step");
-  check(hipMemcpyAsync(host_ + 8, ctl_, sizeof(Sp2Ctl), hipMemcpyDeviceToHost, st_), "copy of the SP2 state");
+  return true;
 }
 
 void DeviceStep::energy(double enuc, bool with_xc, bool point_ej) {
@@ -310,17 +475,17 @@ void DeviceStep::energy(double enuc, bool with_xc, bool point_ej) {
   check(hipMemcpyAsync(host_, eout_, 5 * sizeof(double), hipMemcpyDeviceToHost, st_), "copy of the energy");
 }
 
-void DeviceStep::finish(double* e_one, double* e_j, double* e_xc, double* total, double* nelec, bool* sp2_ok,
-                        int* sp2_steps) {
+void DeviceStep::finish(double* e_one, double* e_j, double* e_xc, double* total, double* nelec) {
   check(hipStreamSynchronize(st_), "hipStreamSynchronize");
   *e_one = host_[0];
   *e_j = host_[1];
   *e_xc = host_[2];
   *total = host_[3];
   *nelec = host_[4];
-  const Sp2Ctl* c = reinterpret_cast<const Sp2Ctl*>(host_ + 8);
-  *sp2_ok = c->done && !c->fail;
-  if (sp2_steps) *sp2_steps = c->iter;
+}
+
+void DeviceStep::clear_xc() {
+  check(hipMemsetAsync(XC_, 0, sizeof(double) * (size_t)n_ * n_, st_), "hipMemsetAsync");
 }
 
 void DeviceStep::download(double* dst, const double* src) {
@@ -330,6 +495,91 @@ void DeviceStep::download(double* dst, const double* src) {
 
 void DeviceStep::upload(double* dst, const double* src) {
   check(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n_ * n_, hipMemcpyHostToDevice, st_), "copy to device");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+BatchStep::BatchStep(int nmol, const int* nbf, const long* off, const int* nocc, hipStream_t stream)
+    : nmol_(nmol), packed_(off[nmol]), st_(stream) {
+  for (int m = 0; m < nmol; ++m)
+    if (nbf[m] > kBatchN) throw std::runtime_error("batched device step: more than 32 functions in a molecule");
+  const size_t np = (size_t)packed_;
+  // H, X, P, J, XC, scalars (4/mol), energies (5/mol), enuc
+  check(hipMalloc((void**)&buf_, sizeof(double) * (5 * np + 10 * (size_t)nmol)), "hipMalloc of the batch SCF");
+  check(hipMalloc((void**)&ibuf_, sizeof(long) * (nmol + 1) + sizeof(int) * 3 * (size_t)nmol + nmol),
+        "hipMalloc of the batch SCF");
+  check(hipHostMalloc((void**)&host_, sizeof(double) * 5 * (size_t)nmol + sizeof(int) * (size_t)nmol,
+                      hipHostMallocDefault),
+        "hipHostMalloc of the batch SCF");
+  H_ = buf_;
+  X_ = H_ + np;
+  P_ = X_ + np;
+  J_ = P_ + np;
+  XC_ = J_ + np;
+  scal_ = XC_ + np;
+  eout_ = scal_ + 4 * (size_t)nmol;
+  enuc_ = eout_ + 5 * (size_t)nmol;
+  off_ = reinterpret_cast<long*>(ibuf_);
+  nbf_ = reinterpret_cast<int*>(off_ + nmol + 1);
+  nocc_ = nbf_ + nmol;
+  fail_ = nocc_ + nmol;
+  active_ = reinterpret_cast<char*>(fail_ + nmol);
+  check(hipMemsetAsync(buf_, 0, sizeof(double) * (5 * np + 10 * (size_t)nmol), st_), "hipMemsetAsync");
+  check(hipMemcpyAsync(off_, off, sizeof(long) * (nmol + 1), hipMemcpyHostToDevice, st_), "copy");
+  check(hipMemcpyAsync(nbf_, nbf, sizeof(int) * nmol, hipMemcpyHostToDevice, st_), "copy");
+  check(hipMemcpyAsync(nocc_, nocc, sizeof(int) * nmol, hipMemcpyHostToDevice, st_), "copy");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+BatchStep::~BatchStep() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  if (buf_) (void)hipFree(buf_);
+  if (ibuf_) (void)hipFree(ibuf_);
+  if (host_) (void)hipHostFree(host_);
+}
+
+void BatchStep::upload_core(const double* H, const double* X, const double* enuc) {
+  check(hipMemcpyAsync(H_, H, sizeof(double) * packed_, hipMemcpyHostToDevice, st_), "copy of H");
+  check(hipMemcpyAsync(X_, X, sizeof(double) * packed_, hipMemcpyHostToDevice, st_), "copy of X");
+  check(hipMemcpyAsync(enuc_, enuc, sizeof(double) * nmol_, hipMemcpyHostToDevice, st_), "copy of E_nuc");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+void BatchStep::set_active(const char* active) {
+  check(hipMemcpyAsync(active_, active, nmol_, hipMemcpyHostToDevice, st_), "copy of the active flags");
+}
+
+int BatchStep::density(bool with_xc, bool first, double mixing) {
+  hipLaunchKernelGGL(k_batch_density, dim3(nmol_), dim3(256), 0, st_, nbf_, off_, nocc_, active_, H_, X_, J_, XC_,
+                     with_xc ? 1 : 0, first ? 1 : 0, mixing, P_, fail_);
+  check(hipGetLastError(), "launch of the batched density step");
+  int* hf = reinterpret_cast<int*>(host_ + 5 * (size_t)nmol_);
+  check(hipMemcpyAsync(hf, fail_, sizeof(int) * nmol_, hipMemcpyDeviceToHost, st_), "copy of the projection flags");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+  int nf = 0;
+  for (int m = 0; m < nmol_; ++m) nf += hf[m] != 0;
+  return nf;
+}
+
+void BatchStep::clear_xc() {
+  check(hipMemsetAsync(XC_, 0, sizeof(double) * (size_t)packed_, st_), "hipMemsetAsync");
+}
+
+void BatchStep::energy(bool with_xc) {
+  hipLaunchKernelGGL(k_batch_energy, dim3(nmol_), dim3(256), 0, st_, nbf_, off_, active_, P_, H_, J_, scal_, enuc_,
+                     with_xc ? 1 : 0, eout_);
+  check(hipGetLastError(), "launch of the batched energy");
+  check(hipMemcpyAsync(host_, eout_, sizeof(double) * 5 * nmol_, hipMemcpyDeviceToHost, st_), "copy of energies");
+}
+
+void BatchStep::finish() { check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+
+void BatchStep::download(double* dst, const double* src, size_t n) {
+  check(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, st_), "copy to host");
+  check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
+
+void BatchStep::upload(double* dst, const double* src, size_t n) {
+  check(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, st_), "copy to device");
   check(hipStreamSynchronize(st_), "hipStreamSynchronize");
 }
 

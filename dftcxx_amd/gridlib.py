@@ -36,6 +36,8 @@ EXPORTS = (
     "dg_batch_autotune", "dg_batch_scf_run", "dg_input_read", "dg_input_destroy", "dg_input_molecule",
     "dg_input_basis", "dg_input_grid", "dg_input_value", "dg_get_amplitude_gradients",
     "dg_get_density_gradients", "dg_density_at_points", "dg_density_at_points_noalloc", "dg_build_fock_device",
+    "dg_build_xc_device", "dg_eri_create", "dg_eri_destroy", "dg_eri_count", "dg_eri_get", "dg_eri_coulomb_device",
+    "dg_eri_coulomb",
 )
 
 
@@ -63,7 +65,7 @@ class _GridParams(C.Structure):
 
 class _ScfParams(C.Structure):
     _fields_ = [("threshold", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
-                ("mixing", C.c_double), ("density_step", C.c_int)]
+                ("mixing", C.c_double), ("density_step", C.c_int), ("hartree", C.c_int)]
 
 
 class _ScfResult(C.Structure):
@@ -109,6 +111,13 @@ def lib():
         L.dg_xc_adjoint.argtypes = [C.c_void_p, C.c_int, _D, _D]
         L.dg_build_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dg_build_fock_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dg_build_xc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dg_eri_create.argtypes = [C.POINTER(_Molecule), C.POINTER(_Basis), C.POINTER(C.c_void_p)]
+        L.dg_eri_destroy.argtypes = [C.c_void_p]
+        L.dg_eri_count.argtypes = [C.c_void_p, C.POINTER(C.c_long), _I]
+        L.dg_eri_get.argtypes = [C.c_void_p, _D]
+        L.dg_eri_coulomb_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dg_eri_coulomb.argtypes = [C.c_void_p, _D, _D]
         L.dg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.dg_profile_read.argtypes = [C.c_void_p, C.c_int, C.c_char_p, _D, _I, _I]
         L.dg_profile_reset.argtypes = [C.c_void_p]
@@ -566,13 +575,15 @@ STEP_AUTO, STEP_HOST, STEP_DEVICE = 0, 1, 2
 
 
 def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5,
-            precision=PREC_FP64, screen=0.0, density_step=STEP_AUTO):
+            precision=PREC_FP64, screen=0.0, density_step=STEP_AUTO, hartree=0):
     """Reference SCF (src/dft.cpp:84-142) with the grid work on the GPU; the
-    density-matrix step on the host or the GPU (density_step, see dg_scf_params).
+    density-matrix step on the host or the GPU (density_step, see dg_scf_params);
+    J from the Becke-grid Poisson path or the two-electron integrals (hartree =
+    HARTREE_BECKE or HARTREE_ERI).
 
     Returns (energies per iteration, result dict, P)."""
     m, b, g, keep = _c_inputs(mol, nrad, nang, lmax, precision, screen)
-    sp = _ScfParams(threshold, min_iter, max_iter, mixing, density_step)
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing, density_step, hartree)
     res = _ScfResult()
     en = np.zeros(max(int(max_iter), 1))
     n = mol.nbf()
@@ -582,8 +593,11 @@ def scf_run(mol, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, m
     return en[:res.iterations], r, P.T.copy()
 
 
-def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5):
-    """The reference SCF for every molecule of a batch, one batched grid build per iteration.
+def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5,
+                  density_step=STEP_AUTO):
+    """The reference SCF for every molecule of a batch, one batched grid build per iteration;
+    the molecules' density-matrix steps on the GPU (STEP_AUTO, STEP_DEVICE) or on host threads
+    (STEP_HOST).
 
     Returns (list of result dicts, number of batch iterations)."""
     mols = list(mols)
@@ -591,7 +605,7 @@ def batch_scf_run(mols, nrad, nang, lmax=8, threshold=1e-6, min_iter=3, max_iter
     ins = [_c_inputs(m, nrad, nang, lmax, PREC_FP64) for m in mols]
     ms = (_Molecule * n)(*[i[0] for i in ins])
     bs = (_Basis * n)(*[i[1] for i in ins])
-    sp = _ScfParams(threshold, min_iter, max_iter, mixing, STEP_HOST)
+    sp = _ScfParams(threshold, min_iter, max_iter, mixing, density_step, 0)
     res = (_ScfResult * n)()
     it = C.c_int()
     _check(lib().dg_batch_scf_run(n, ms, bs, C.byref(ins[0][2]), C.byref(sp), res, C.byref(it)))
@@ -680,3 +694,54 @@ class InputFile:
 def read_input(path, basis_dir=None):
     """Parse a reference input file; see :class:`InputFile`."""
     return InputFile(path, basis_dir)
+
+
+class Eri:
+    """Analytic two-electron integrals of a basis on the GPU (dg_eri_*): every unique
+    (ij|kl) once (src/dft.cpp:254-288), J = sum_kl P_kl (ij|kl) from them
+    (src/dft.cpp:452-477)."""
+
+    def __init__(self, mol):
+        self.e = None
+        m, b, g, keep = _c_inputs(mol, 5, 6, 0, PREC_FP64)
+        h = C.c_void_p()
+        _check(lib().dg_eri_create(C.byref(m), C.byref(b), C.byref(h)))
+        self.e = h
+        n, nb = C.c_long(), C.c_int()
+        _check(lib().dg_eri_count(self.e, C.byref(n), C.byref(nb)))
+        self.nints, self.nbf = n.value, nb.value
+
+    def close(self):
+        if self.e is not None:
+            lib().dg_eri_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def integrals(self):
+        """Packed unique integrals in the reference's teindex order."""
+        out = np.empty(self.nints)
+        _check(lib().dg_eri_get(self.e, _d(out)))
+        return out
+
+    def coulomb(self, P):
+        Pc = _f64(np.asarray(P).T)
+        J = np.empty((self.nbf, self.nbf))
+        _check(lib().dg_eri_coulomb(self.e, _d(Pc), _d(J)))
+        return J.T.copy()
+
+
+def teindex(i, j, k, l):
+    """Integrator::teindex (src/integrals.cpp:470-486)."""
+    if i < j:
+        i, j = j, i
+    if k < l:
+        k, l = l, k
+    ij, kl = i * (i + 1) // 2 + j, k * (k + 1) // 2 + l
+    if ij < kl:
+        ij, kl = kl, ij
+    return ij * (ij + 1) // 2 + kl

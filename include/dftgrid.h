@@ -185,6 +185,31 @@ int dg_build_device(dg_handle* h, const double* dP, double* dJ, double* dXC, dou
  * in the reference, which forms J and XC separately. */
 int dg_build_fock_device(dg_handle* h, const double* dP, double* dG, double* dscalars);
 
+/* The build without the Hartree stages, for the SCF whose J comes from the
+ * two-electron integrals (dg_eri_*): rho, its renormalisation, the functional
+ * and XC (device, nbf*nbf); dscalars as dg_build_device with E_J = 0. */
+int dg_build_xc_device(dg_handle* h, const double* dP, double* dXC, double* dscalars);
+
+/* --- analytic two-electron integrals ------------------------------------------ */
+/* The reference's other Hartree path (hartree_evaluation = two_electron_integrals,
+ * src/dft.cpp:378-385): every unique (ij|kl) over the basis, computed once on
+ * the current HIP device and kept there, packed at
+ * IJ(IJ+1)/2 + KL with IJ = i(i+1)/2 + j, i >= j, IJ >= KL
+ * (DFT::calculate_two_electron_integrals and Integrator::teindex,
+ * src/dft.cpp:254-288, src/integrals.cpp:470-486). */
+typedef struct dg_eri dg_eri;
+int dg_eri_create(const dg_molecule* mol, const dg_basis* basis, dg_eri** out);
+int dg_eri_destroy(dg_eri* e);
+int dg_eri_count(const dg_eri* e, long* nints, int* nbf);
+/* the packed integrals, nints doubles (host) */
+int dg_eri_get(dg_eri* e, double* ints);
+/* DFT::calculate_hartree_potential_te_int (src/dft.cpp:452-477):
+ * J_ij = sum_kl P_kl (ij|kl), device pointers (nbf*nbf, column-major),
+ * enqueued on stream (a hipStream_t; NULL = the object's own). */
+int dg_eri_coulomb_device(dg_eri* e, const double* dP, double* dJ, void* stream);
+/* the same from and to host buffers (synchronous) */
+int dg_eri_coulomb(dg_eri* e, const double* P, double* J);
+
 /* --- profiling ------------------------------------------------------------- */
 /* When enabled, every kernel of dg_build_device is bracketed by HIP events on
  * the handle's stream and its time accumulated per kernel name. */
@@ -222,6 +247,10 @@ typedef struct {
    * the occupied space by second-order spectral projection (the same P up to
    * rounding; steps without a HOMO-LUMO gap fall back to the host). */
   int density_step;
+  /* DG_HARTREE_BECKE (the Becke-grid Poisson path) or DG_HARTREE_ERI (J from
+   * the analytic two-electron integrals, hartree_evaluation =
+   * two_electron_integrals; the grid then serves the XC part only). */
+  int hartree;
 } dg_scf_params;
 
 typedef struct {

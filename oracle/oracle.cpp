@@ -1433,16 +1433,27 @@ struct ScfCore {
 
 extern "C" {
 
-int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
-           const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
-           const double* coef, double threshold, int min_iter, int max_iter, double mixing, double* energies,
-           double* terms, double* P_out) {
+// DFT::scf with J from the Becke-grid Poisson path (eri = false) or from the
+// two-electron integrals (eri = true: hartree_evaluation = two_electron_integrals,
+// src/dft.cpp:378-385, 452-477); rho is set and corrected every iteration in
+// both (src/dft.cpp:362-365) since XC needs it.
+static int scf_loop(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+                    const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+                    const double* coef, double threshold, int min_iter, int max_iter, double mixing,
+                    double* energies, double* terms, double* P_out, bool eri) {
   ScfCore s(g, natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef);
   s.mixing = mixing;
+  auto coulomb = [&] {
+    if (eri)
+      or_eri_J(natoms, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef, s.P.data(),
+               s.J.data());
+    else
+      or_grid_hartree(g, s.J.data());
+  };
   // DFT::DFT -> construct_matrices: the core guess, J of it, the energy (E_xc = 0, Q3)
   s.density_matrix();
   s.grid_density(0.0);
-  or_grid_hartree(g, s.J.data());
+  coulomb();
   s.energy();
   // DFT::scf, src/dft.cpp:84-142
   double old_energy = s.et, difference = 1.0;
@@ -1451,7 +1462,7 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, 
     iteration++;
     s.density_matrix();
     s.grid_density(0.0);
-    or_grid_hartree(g, s.J.data());
+    coulomb();
     or_grid_xc(g, s.XC.data(), &s.exc);
     s.energy();
     if (energies) energies[iteration - 1] = s.et;
@@ -1462,6 +1473,22 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, 
   if (terms) s.terms(terms);
   if (P_out) std::memcpy(P_out, s.P.data(), sizeof(double) * s.nn);
   return iteration;
+}
+
+int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+           const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+           const double* coef, double threshold, int min_iter, int max_iter, double mixing, double* energies,
+           double* terms, double* P_out) {
+  return scf_loop(g, natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef,
+                  threshold, min_iter, max_iter, mixing, energies, terms, P_out, false);
+}
+
+int or_scf_eri(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+               const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+               const double* coef, double threshold, int min_iter, int max_iter, double mixing,
+               double* energies, double* terms, double* P_out) {
+  return scf_loop(g, natoms, Z, xyz, nshell, shell_atom, shell_L, shell_nprim, shell_offset, alpha, coef,
+                  threshold, min_iter, max_iter, mixing, energies, terms, P_out, true);
 }
 
 int or_scf_evaluate(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,

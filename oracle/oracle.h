@@ -108,6 +108,12 @@ int or_scf(or_grid* g, int natoms, const int* Z, const double* xyz,
            const double* alpha, const double* coef,
            double threshold, int min_iter, int max_iter, double mixing,
            double* energies, double* terms, double* P_out);
+/* The same SCF with J from the two-electron integrals (or_eri_J each iteration),
+ * the reference's hartree_evaluation = two_electron_integrals (src/dft.cpp:378-385). */
+int or_scf_eri(or_grid* g, int natoms, const int* Z, const double* xyz, int nshell, const int* shell_atom,
+               const int* shell_L, const int* shell_nprim, const int* shell_offset, const double* alpha,
+               const double* coef, double threshold, int min_iter, int max_iter, double mixing,
+               double* energies, double* terms, double* P_out);
 /* One evaluation of the SCF's grid work and energy at a given P, by the same
  * code or_scf runs each iteration: set_density, correct_densities (or, with
  * hold_c > 0, rho scaled by hold_c instead), Hartree J, XC, calculate_energy.

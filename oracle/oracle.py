@@ -74,6 +74,8 @@ def lib():
         L.or_scf.argtypes = [C.c_void_p, C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
                              C.c_double, C.c_int, C.c_int, C.c_double, _D, _D, _D]
         L.or_scf.restype = C.c_int
+        L.or_scf_eri.argtypes = L.or_scf.argtypes
+        L.or_scf_eri.restype = C.c_int
         L.or_scf_evaluate.argtypes = [C.c_void_p, C.c_int, _I, _D, C.c_int, _I, _I, _I, _I, _D, _D,
                                       _D, C.c_double, _D, _D, _D, _D, _D]
         L.or_scf_evaluate.restype = C.c_int
@@ -200,15 +202,17 @@ class Grid:
         lib().or_grid_xc_parts(self.h, _d(exc), _d(wva))
         return exc, wva
 
-    def scf(self, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5):
-        """Run the reference SCF on this grid; returns (energies, terms, P)."""
+    def scf(self, threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5, eri=False):
+        """Run the reference SCF on this grid (J from the grid, or from the
+        two-electron integrals with eri=True); returns (energies, terms, P)."""
         Z, xyz, at, L, npr, off, al, co = self._keep
         en = np.zeros(max_iter)
         terms = np.zeros(5)
         P = np.zeros((self.nbf, self.nbf))
-        it = lib().or_scf(self.h, self.mol.natoms, _i(Z), _d(xyz), len(at), _i(at), _i(L), _i(npr),
-                          _i(off), _d(al), _d(co), threshold, min_iter, max_iter, mixing,
-                          _d(en), _d(terms), _d(P))
+        fn = lib().or_scf_eri if eri else lib().or_scf
+        it = fn(self.h, self.mol.natoms, _i(Z), _d(xyz), len(at), _i(at), _i(L), _i(npr),
+                _i(off), _d(al), _d(co), threshold, min_iter, max_iter, mixing,
+                _d(en), _d(terms), _d(P))
         return en[:it], dict(zip(("total", "one", "J", "xc", "nuc"), terms)), P
 
     def evaluate(self, P, hold_c=0.0):

@@ -139,11 +139,13 @@ def test_batch_rejects_single_molecule_calls():
         assert rc == 4 and "batch handle" in L.dg_last_error().decode()
 
 
-def test_batch_scf_matches_single_scf():
+@pytest.mark.parametrize("step", [G.STEP_HOST, G.STEP_DEVICE], ids=["host_step", "device_step"])
+def test_batch_scf_matches_single_scf(step):
     mols = M.batch(count=6, seed=21)
     grid = (30, 50, 6)
-    print("[case] batch SCF of 6 at 30x50", flush=True)
-    res, total = G.batch_scf_run(mols, *grid, threshold=1e-6)
+    print(f"[case] batch SCF of 6 at 30x50, step {step}", flush=True)
+    res, total = G.batch_scf_run(mols, *grid, threshold=1e-6, density_step=step)
+    assert res[0]["device_step"] == (1 if step == G.STEP_DEVICE else 0)
     assert total == max(r["iterations"] for r in res)
     for m, r in zip(mols, res):
         _, ref, _ = G.scf_run(m, *grid, threshold=1e-6)

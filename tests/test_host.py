@@ -280,15 +280,17 @@ def test_host_integrals_independent_of_thread_count():
     ("mixing", -0.1, "mixing"),
     ("density_step", 3, "density_step must be"),
     ("density_step", -1, "density_step must be"),
+    ("hartree", 2, "hartree must be"),
 ])
 def test_scf_rejects_invalid_params(field, value, msg):
     # the SCF parameters are checked before any device call, so this holds without a GPU
     import ctypes as C
     mol = M.h2o()
     m, b, g, keep = G._c_inputs(mol, 20, 26, 4, G.PREC_FP64)
-    kw = dict(threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5, density_step=0)
+    kw = dict(threshold=1e-6, min_iter=3, max_iter=100, mixing=0.5, density_step=0, hartree=0)
     kw[field] = value
-    sp = G._ScfParams(kw["threshold"], kw["min_iter"], kw["max_iter"], kw["mixing"], kw["density_step"])
+    sp = G._ScfParams(kw["threshold"], kw["min_iter"], kw["max_iter"], kw["mixing"], kw["density_step"],
+                      kw["hartree"])
     res = G._ScfResult()
     rc = G.lib().dg_scf_run(C.byref(m), C.byref(b), C.byref(g), C.byref(sp), C.byref(res), None, None)
     assert rc == 1 and msg in G.lib().dg_last_error().decode()
