@@ -69,6 +69,18 @@ constexpr double kPiD = 3.14159265358979323846;
 // Setup kernels (once per geometry)
 // ---------------------------------------------------------------------------
 
+// Position R_a + r_i Omega_j of a grid point (explicit fused multiply-adds: the
+// grid kernel and the cross-potential kernel, which re-forms the coordinates
+// from the indices, see the same values).
+__device__ __forceinline__ void grid_xyz(const double* __restrict__ atoms, const double* __restrict__ rad,
+                                         const double* __restrict__ leb, int a, int i, int j, double& x, double& y,
+                                         double& z) {
+  const double r = rad[i];
+  x = __fma_rn(leb[4 * j], r, atoms[4 * a]);
+  y = __fma_rn(leb[4 * j + 1], r, atoms[4 * a + 1]);
+  z = __fma_rn(leb[4 * j + 2], r, atoms[4 * a + 2]);
+}
+
 // Grid points: atom a, radial node i, Lebedev point j of p = (a*nrad + i)*nang + j.
 // Position R_a + r_i*Omega_j, weight w_i * w_j * |r_i Omega_j|^2 * 4 pi
 // (src/atomicgrid.cpp:50-87).
@@ -89,9 +101,7 @@ __global__ void k_grid(const double* __restrict__ atoms, const double* __restric
   const int a = (int)(ai / nrad);
   const double r = rad[i];
   const double ox = leb[4 * j] * r, oy = leb[4 * j + 1] * r, oz = leb[4 * j + 2] * r;
-  px[p] = atoms[4 * a] + ox;
-  py[p] = atoms[4 * a + 1] + oy;
-  pz[p] = atoms[4 * a + 2] + oz;
+  grid_xyz(atoms, rad, leb, a, i, j, px[p], py[p], pz[p]);
   double wp = wrad[i] * leb[4 * j + 3];
   wp *= (ox * ox + oy * oy + oz * oz) * 4.0 * kPiD;
   w[p] = wp;
@@ -249,27 +259,35 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // the next live tile's fragments are copied from global memory straight into
 // the other buffer (global_load_lds, 1 KB per wave instruction) while the
 // current tile's MFMAs run; one wait and barrier per tile.
-// NT = ceil(nbf/16).  With screening, mask (one word per 64-point block, bit T
-// for basis tile T) drops the tiles whose amplitudes are negligible on the
-// whole block; the mask is uniform over the work-group, so every branch on it is too.
-template <int NT>
-__global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi, int ldphi,
-                                                 const double* __restrict__ frag, double* rho,
-                                                 const unsigned* __restrict__ mask) {
+// NT = ceil(nbf/16).  W = 8 waves (128 points) share the fragments, two per
+// SIMD: one issues MFMAs while the other waits on LDS or the tile barrier.
+// With screening, mask (one word per 64-point block, bit T for basis tile T)
+// drops the tiles whose amplitudes are negligible on the whole block: a wave
+// skips the MFMAs of the tiles off its block's word, and the work-group walks
+// (and fetches) the tiles on either block's word, so every branch on the walk
+// is uniform over the work-group.
+template <int NT, int W>
+__global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ phi, int ldphi,
+                                                    const double* __restrict__ frag, double* rho,
+                                                    const unsigned* __restrict__ mask) {
   __shared__ double pl[2][4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const long p0 = (long)blockIdx.x * 16 * W + wave * 16;
   const double* row = phi + (p0 + c) * ldphi;
-  const unsigned mk = (mask ? mask[blockIdx.x] : 0xffffffffu) & (NT >= 32 ? 0xffffffffu : ((1u << NT) - 1u));
+  constexpr unsigned all = NT >= 32 ? 0xffffffffu : ((1u << NT) - 1u);
+  const long blk0 = (long)blockIdx.x * W / 4;  // first 64-point block of the work-group
+  const unsigned mine = (mask ? mask[blk0 + wave / 4] : 0xffffffffu) & all;
+  unsigned mk = 0;
+  for (int b = 0; b < W / 4; ++b) mk |= (mask ? mask[blk0 + b] : 0xffffffffu) & all;
   double f[4 * NT];
 #pragma unroll
-  for (int s = 0; s < 4 * NT; ++s) f[s] = ((mk >> (s >> 2)) & 1u) ? row[16 * (s >> 2) + 4 * g + (s & 3)] : 0.0;
+  for (int s = 0; s < 4 * NT; ++s) f[s] = ((mine >> (s >> 2)) & 1u) ? row[16 * (s >> 2) + 4 * g + (s & 3)] : 0.0;
   // tile b's fragments, 256 (b + 1) doubles from offset 128 b (b + 1): chunks of
-  // 128 doubles, chunk q copied by wave q % 4, 16 bytes per lane
+  // 128 doubles, chunk q copied by wave q % W, 16 bytes per lane
   auto fetch = [&](int b, double* dst) {
     const double* src = frag + 128L * b * (b + 1);
-    for (int q = wave; q < 2 * (b + 1); q += 4)
+    for (int q = wave; q < 2 * (b + 1); q += W)
       __builtin_amdgcn_global_load_lds((const void*)(src + q * 128 + lane * 2),
                                        (__attribute__((address_space(3))) void*)(dst + q * 128), 16, 0, 0);
   };
@@ -304,12 +322,12 @@ __global__ __launch_bounds__(256) void k_density(const double* __restrict__ phi,
       if (k + 1 <= b)
 #pragma unroll
         for (int j = 0; j < 4; ++j) nxt[j] = cur[(4 * (k + 1) + j) * 64 + lane];
-      if (k <= b && ((mk >> k) & 1u))
+      if (k <= b && ((mine >> k) & 1u))
 #pragma unroll
         for (int j = 0; j < 4; ++j) d = mfma16x16x4(av[j], f[4 * k + j], d);
     }
     // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]
-    acc += d[0] * r01.x + d[1] * r01.y + d[2] * r23.x + d[3] * r23.y;
+    if ((mine >> b) & 1u) acc += d[0] * r01.x + d[1] * r01.y + d[2] * r23.x + d[3] * r23.y;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
     __syncthreads();
     buf ^= 1;
@@ -786,9 +804,8 @@ __global__ __launch_bounds__(256) void k_vcell(const double* __restrict__ U, con
 }
 
 struct CrossArgs {
-  const double* px;
-  const double* py;
-  const double* pz;
+  const double* rad;    // radial nodes (nrad)
+  const double* leb;    // Lebedev directions and weights (4 nang)
   const double* atoms;
   const double* knots;  // ascending, nrad
   const double* invh;   // 1/(x_t - x_{t-1})
@@ -799,7 +816,8 @@ struct CrossArgs {
   const int* plen;             // per item, the padded length of its perm range
   double* vpart;        // partial potentials, atom k's at vpart + vbase[k] + p
   const long* vbase;
-  int nitems, nrad, nlm, l0, l1, accumulate;
+  int nitems, nrad, nlm, l0, l1, accumulate, nang;
+  double inv_nang, inv_nrad;  // 1/nang, 1/nrad: grid indices of a point
 };
 
 // Distance r of a point from an atom centre and its distance rxy from the
@@ -953,7 +971,10 @@ __global__ __launch_bounds__(1024) void k_sort_items(const int* __restrict__ ite
 // evaluated once (linear in the per-lm coefficients).  The Q = 2 slots of a
 // thread share their interval (a pad slot repeats its partner's point and is
 // not written), taken from the sort (pkey), so each table entry is read from
-// LDS once for both.
+// LDS once for both.  A slot's coordinates are formed from its grid indices
+// (grid_xyz), not read: the sorted order scatters reads over the item's range.
+// The item's potentials are collected in LDS (vbuf, by offset in the item) and
+// written to vpart in point order once the item is done.
 template <int LMAX, int Q>
 __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   static_assert(Q == 2, "slots are paired by the item sort");
@@ -963,6 +984,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   double* kn = lds;
   double* ih = lds + n;
   double* tab = lds + 2 * n;
+  double* vbuf = tab + (long)n * nlmc * 2;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     kn[i] = A.knots[i];
     ih[i] = A.invh[i];
@@ -982,13 +1004,13 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       __syncthreads();
       cur = k;
     }
-    const long pb = A.items[4 * it + 1];
+    const int pb = A.items[4 * it + 1], npt = A.items[4 * it + 2] - pb;
     const int len = A.plen[it];
     const unsigned short* pm = A.perm + A.items[4 * it + 3];
     const unsigned short* pk = A.pkey + A.items[4 * it + 3];
     const double Ax = A.atoms[4 * k], Ay = A.atoms[4 * k + 1], Az = A.atoms[4 * k + 2];
     for (int q0 = threadIdx.x * Q; q0 < len; q0 += blockDim.x * Q) {
-      long pq[Q];
+      int eq[Q];
       double rinv[Q], ct[Q], st[Q], cp[Q], sp[Q], tq[Q];
       bool valid[Q];
       unsigned short e0 = pm[q0];
@@ -1002,8 +1024,16 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
         // a pad slot or a missing tail slot repeats the thread's first point
         const unsigned short e = (q0 + q < len) ? pm[q0 + q] : kPadSlot;
         valid[q] = e != kPadSlot;
-        pq[q] = pb + (valid[q] ? e : e0);
-        const double dx = A.px[pq[q]] - Ax, dy = A.py[pq[q]] - Ay, dz = A.pz[pq[q]] - Az;
+        eq[q] = valid[q] ? e : e0;
+        // p = (a nrad + i) nang + j
+        const int p = pb + eq[q];
+        const int ai = (int)(((double)p + 0.5) * A.inv_nang);
+        const int j = p - ai * A.nang;
+        const int a = (int)(((double)ai + 0.5) * A.inv_nrad);
+        const int i = ai - a * n;
+        double x, y, z;
+        grid_xyz(A.atoms, A.rad, A.leb, a, i, j, x, y, z);
+        const double dx = x - Ax, dy = y - Ay, dz = z - Az;
         double rxy, r;
         point_polar(dx, dy, dz, rxy, r);
         rinv[q] = 1.0 / r;
@@ -1117,13 +1147,14 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
           const double d = hi_ * hi_ * (-2.0 * dy + sd0[q] + sd1[q]);
           val = sy0[q] + sd0[q] * t + c * t * t + d * t * t * t;
         }
-        double* out = A.vpart + A.vbase[k] + pq[q];
-        if (A.accumulate)
-          *out += val * rinv[q];
-        else
-          *out = val * rinv[q];
+        vbuf[eq[q]] = val * rinv[q];
       }
     }
+    // every point of the item has its slot: write the potentials in point order
+    __syncthreads();
+    double* out = A.vpart + A.vbase[k] + pb;
+    for (int e = threadIdx.x; e < npt; e += blockDim.x) out[e] = A.accumulate ? out[e] + vbuf[e] : vbuf[e];
+    __syncthreads();
   }
 }
 
@@ -2398,7 +2429,10 @@ int sort_items(dg_handle* h) {
 
 template <int NT>
 void launch_density_nt(dg_handle* h) {
-  hipLaunchKernelGGL(k_density<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
+  // two waves per SIMD while the amplitude rows (8 NT registers) leave room for
+  // both (NT <= 25: 400 functions); npts_pad is a multiple of 256
+  constexpr int W = NT <= 25 ? 8 : 4;
+  hipLaunchKernelGGL((k_density<NT, W>), dim3(nblocks(h->npts_pad, 16 * W)), dim3(64 * W), 0, h->stream, h->phi,
                      h->ldphi, h->pfrag, h->rho, h->bmask);
 }
 
@@ -2586,9 +2620,11 @@ int stage_hartree(dg_handle* h) {
   }));
   if (h->natoms > 1) {
     CrossArgs a;
-    a.px = h->px;
-    a.py = h->py;
-    a.pz = h->pz;
+    a.rad = h->rad;
+    a.leb = h->leb;
+    a.nang = h->nang;
+    a.inv_nang = 1.0 / h->nang;
+    a.inv_nrad = 1.0 / h->nrad;
     a.atoms = h->atoms;
     a.knots = h->knots;
     a.invh = h->invh;
@@ -3122,11 +3158,11 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     }
   h->ncb = (int)cbs.size() / 4;
   h->nlda_blocks = h->npts_pad / 256;
-  // cross kernel: l ranges whose spline table fits in LDS
+  // cross kernel: l ranges whose spline table fits in LDS beside an item's potentials
   {
     const size_t limit = 150 * 1024;
     auto bytes = [&](int l0, int l1) {
-      return (size_t)(2 * h->nrad + (size_t)h->nrad * (l1 * l1 - l0 * l0) * 2) * sizeof(double);
+      return (size_t)(2 * h->nrad + (size_t)h->nrad * (l1 * l1 - l0 * l0) * 2 + kMaxItem) * sizeof(double);
     };
     int l0 = 0;
     size_t maxb = 0;

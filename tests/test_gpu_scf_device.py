@@ -116,6 +116,8 @@ def test_fock_build_is_2j_plus_xc(make, prec):
     torch.cuda.synchronize()
     ref = (2 * J + XC).cpu().numpy()
     got = Gm.cpu().numpy()
-    tol = 1e-12 if prec == G.PREC_FP64 else 1e-7
+    # mixed: one FP32 contraction against two, each FP32 sum flushed to FP64 every 64
+    # points, so the two differ at a few FP32 ulps of the larger terms
+    tol = 1e-12 if prec == G.PREC_FP64 else 5e-7
     assert np.abs(got - ref).max() < tol * np.abs(ref).max()
     assert np.array_equal(s1.cpu().numpy(), s2.cpu().numpy())
