@@ -1228,31 +1228,35 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // waves per SIMD); its register v holds row 4g + v and is stored in the FP64 order.
 // Work-group order: the hardware deals work-groups round-robin to the kXcds
 // XCDs, each with its own L2.  Work-group L (1-D grid, padded to a multiple of
-// kXcds) takes virtual id v = (L % kXcds) per + L / kXcds, split v / ncb, tile
-// block v % ncb: an XCD runs whole splits, all tile blocks of a split together,
-// so the amplitude rows of a point range come from HBM into one L2 once.
+// kXcds) takes virtual id v = (L % kXcds) per + L / kXcds and order[v] = {tile
+// block, split}: consecutive v are the tile blocks of one split, so an XCD runs
+// whole splits, and within an XCD's range the blocks with the most MFMAs come
+// first, so that the short diagonal blocks fill the last round.
 constexpr int kXcds = 8;
-template <typename T, int NOUT>
+template <typename T, int NOUT, bool DENSE>
 __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(const T* __restrict__ phi, int ldphi,
                                                     const double* __restrict__ u0,
                                                     const double* __restrict__ u1,
                                                     const int* __restrict__ cblocks,
                                                     const int* __restrict__ blist,
-                                                    const int* __restrict__ loff, int ngroups, int nsplit,
+                                                    const int* __restrict__ loff,
+                                                    const int2* __restrict__ order, int ngroups, int nsplit,
                                                     double* part) {
   constexpr bool f32 = sizeof(T) == 4;
   __shared__ double red[(f32 ? NOUT : 1) * 8 * 256];
   const int ncb = ngroups / 4, per = gridDim.x / kXcds;
   const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (v >= ncb * nsplit) return;
-  const int cbi = v % ncb, split = v / ncb;
+  const int cbi = order[v].x, split = order[v].y;
   const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
-  const int* bl = blist + l0 + cnt * split / nsplit;
-  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - cnt * split / nsplit);
-  auto point = [&](int t) { return 64L * bl[t >> 2] + 16 * (t & 3) + 4 * wave + g; };
+  const int s0 = (int)(cnt * split / nsplit);  // the split's first list entry
+  const int* bl = blist + l0 + s0;
+  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
+  // DENSE: every block's list is all 64-point blocks, entry i = block i
+  auto point = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave + g; };
   const int ia = 16 * I0 + c, jb = 16 * J0 + c;
   typedef typename std::conditional<f32, flt4, dbl4>::type accT;
   accT acc[NOUT][4][2];
@@ -2265,6 +2269,7 @@ struct dg_handle {
   int cus = 256;  // compute units of the device
   // 4 x 2 tile blocks of k_contract2d: {I0, nI, J0, nJ}, their reduction groups and point lists
   int *cb = nullptr, *cb_groups = nullptr, *cblist = nullptr, *cbloff = nullptr;
+  int2* cborder = nullptr;  // per virtual work-group id, {tile block, split}
   int ncb = 0, nsplit2d = 0;
   long nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
@@ -2342,6 +2347,7 @@ struct dg_handle {
                     atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0,
                     cb, cb_groups, cblist, cbloff};
     if (bmask) (void)hipFree(bmask);
+    if (cborder) (void)hipFree(cborder);
     if (vbase) (void)hipFree(vbase);
     if (perm) (void)hipFree(perm);
     if (pkey) (void)hipFree(pkey);
@@ -2753,6 +2759,16 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
   }
 }
 
+template <typename T, int NOUT>
+void launch_contract2d(dg_handle* h, dim3 grid, const T* phi, const double* u0, const double* u1) {
+  if (h->screen == 0.0)
+    hipLaunchKernelGGL((k_contract2d<T, NOUT, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, h->cb,
+                       h->cblist, h->cbloff, h->cborder, 4 * h->ncb, h->nsplit2d, h->cpart);
+  else
+    hipLaunchKernelGGL((k_contract2d<T, NOUT, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, h->cb,
+                       h->cblist, h->cbloff, h->cborder, 4 * h->ncb, h->nsplit2d, h->cpart);
+}
+
 // sa (optional): the build's scalars, evaluated by one more work-group of the reduction
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1,
                    const ScalarArgs* sa = nullptr) {
@@ -2768,19 +2784,15 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
     } else if (mx) {
       const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
-        hipLaunchKernelGGL((k_contract2d<float, 2>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u1, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
+        launch_contract2d<float, 2>(h, g2, h->phf, u0, u1);
       else
-        hipLaunchKernelGGL((k_contract2d<float, 1>), g2, dim3(256), 0, h->stream, h->phf, h->ldphi, u0, u0, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
+        launch_contract2d<float, 1>(h, g2, h->phf, u0, u0);
     } else {
       const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
-        hipLaunchKernelGGL((k_contract2d<double, 2>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u1, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
+        launch_contract2d<double, 2>(h, g2, h->phi, u0, u1);
       else
-        hipLaunchKernelGGL((k_contract2d<double, 1>), g2, dim3(256), 0, h->stream, h->phi, h->ldphi, u0, u0, h->cb,
-                           h->cblist, h->cbloff, 4 * h->ncb, h->nsplit2d, h->cpart);
+        launch_contract2d<double, 1>(h, g2, h->phi, u0, u0);
     }
   }));
   const bool two_d = !small;
@@ -3215,11 +3227,26 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     const size_t lds = h->cross_lds;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
-    // 2-D contraction: splits per tile block such that all work-groups fit in two
-    // rounds of the resident slots (3 FP64 / 4 FP32 work-groups per CU), the
-    // second round as full as the tile-block count allows
+    // 2-D contraction: splits per tile block such that the work-groups make about
+    // three rounds of the resident slots (3 FP64 / 4 FP32 work-groups per CU); with
+    // the longest blocks first, the short ones even out the last round (C30H62: one
+    // round 14.3 ms, two 12.8, three 12.35, four 12.4)
     const long slots = (long)cus * (h->precision == DG_PREC_MIXED ? 4 : 3);
-    h->nsplit2d = (int)std::max(1L, std::min(std::min(512L, 2 * slots / std::max(h->ncb, 1)), h->npts_pad / 64));
+    h->nsplit2d = (int)std::max(1L, std::min(std::min(512L, 3 * slots / std::max(h->ncb, 1)), h->npts_pad / 64));
+    // virtual work-group order of k_contract2d: split-major, each XCD's range
+    // re-ordered by decreasing tile pairs of the block (stable)
+    const long nv = (long)h->ncb * h->nsplit2d, per = round_up(nv, kXcds) / kXcds;
+    std::vector<int> work(std::max(h->ncb, 1), 0);
+    for (int b = 0; b < h->ncb; ++b)
+      for (int r = 0; r < cbs[4 * b + 1]; ++r)
+        for (int t = 0; t < cbs[4 * b + 3]; ++t) work[b] += (cbs[4 * b] + r <= cbs[4 * b + 2] + t) ? 1 : 0;
+    std::vector<int2> order(std::max<long>(nv, 1));
+    for (long v = 0; v < nv; ++v) order[v] = make_int2((int)(v % std::max(h->ncb, 1)), (int)(v / std::max(h->ncb, 1)));
+    for (long x0 = 0; x0 < nv; x0 += per)
+      std::stable_sort(order.begin() + x0, order.begin() + std::min(nv, x0 + per),
+                       [&](const int2& a, const int2& b) { return work[a.x] > work[b.x]; });
+    DG_TRY(dalloc(&h->cborder, order.size()));
+    HIP_TRY(hipMemcpy(h->cborder, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
   }
 
   // allocations
