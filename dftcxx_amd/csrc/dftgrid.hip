@@ -699,6 +699,16 @@ __global__ __launch_bounds__(64 * kRhoWaves) void k_rho_lm(const double* __restr
   }
 }
 
+// c_lm of the scaled associated-Legendre recurrence: the standard recurrence
+// P_l = ((2l-1) ct P_{l-1} - (l+m-1) P_{l-2}) / (l-m) over Q_l = P_l / c_lm, with
+// c_mm = 1, c_{m+1,m} = 2m+1, c_lm = c_{l-1,m} (2l-1)/(l-m), needs one multiply
+// fewer per step (Q_{m+1} = ct Q_m, Q_l = ct Q_{l-1} - beta_l Q_{l-2}).
+__host__ __device__ inline double legendre_scale(int l, int m) {
+  double c = l >= m + 1 ? 2.0 * m + 1.0 : 1.0;
+  for (int k = m + 2; k <= l; ++k) c *= (double)(2 * k - 1) / (double)(k - m);
+  return c;
+}
+
 // Radial Poisson solve and spline slopes (src/atomicgrid.cpp:395-432,
 // src/cspline.cpp:66-127, src/atomicgrid.cpp:535-556): per l, with the columns
 // (a, m) of all atoms, U_i = sum_k KU_l[k][i] g_k and D_t = sum_k KD_l[k][t] g_k,
@@ -706,8 +716,9 @@ __global__ __launch_bounds__(64 * kRhoWaves) void k_rho_lm(const double* __restr
 // g_{N+1} = 0, KU_l = rows 1..N of the inverse FD matrix and KD_l = S KU_l (the
 // not-a-knot slope map S over the ascending knots).  Block x = (l, row tile,
 // column tile, output U or D); tiles past the l's columns exit.
-// tab[a][t][lm] = N_lm {U at knot t, D_t}; knot t is radial node N-1-t, and
-// N_lm (the harmonic's normalisation, shc) is folded in for the cross kernel.
+// tab[a][t][lm] = N_lm c_lm {U at knot t, D_t}; knot t is radial node N-1-t, and
+// N_lm c_lm (the harmonic's normalisation and the recurrence scale, shc) is folded
+// in for the cross kernel.
 // (Four waves per work-group, one tile each: tile 4 blockIdx.x + wave of the
 // ntiles; small work-groups would cap the waves resident per CU.)
 __global__ __launch_bounds__(256) void k_poisson(const double* __restrict__ rho_lm, const double* __restrict__ q,
@@ -981,6 +992,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
   extern __shared__ double lds[];
   const int n = A.nrad;
   const int lmb = A.l0 * A.l0, nlmc = A.l1 * A.l1 - lmb;
+  const unsigned lbits = __builtin_amdgcn_readfirstlane(((1u << A.l1) - 1u) & ~((1u << A.l0) - 1u));
   double* kn = lds;
   double* ih = lds + n;
   double* tab = lds + 2 * n;
@@ -1019,6 +1031,11 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       const bool clamp = key == 0 || key == n;
       const int s1 = clamp ? (key == 0 ? 0 : n - 1) : key;
       const int s0 = clamp ? s1 : key - 1;
+      unsigned lb = lbits;
+      asm volatile("" : "+s"(lb));
+      // knot rows s0 and s1 of the table, indexed by lm from 0 (entries below lmb are not read)
+      const double* T0 = tab + (s0 * nlmc - lmb) * 2;
+      const double* T1 = tab + (s1 * nlmc - lmb) * 2;
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         // a pad slot or a missing tail slot repeats the thread's first point
@@ -1072,10 +1089,12 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
         for (int q = 0; q < Q; ++q) p1[q] = p2[q] = 0.0;
 #pragma unroll
         for (int l = m; l <= LMAX; ++l) {
-          // recurrence coefficients (2l-1)/(l-m), (l+m-1)/(l-m) fold to constants
-          // after unrolling (a global read of them waited on its full latency)
-          const double ra = (l >= m + 2) ? (double)(2 * l - 1) / (double)(l - m) : 0.0;
-          const double rb = (l >= m + 2) ? (double)(l + m - 1) / (double)(l - m) : 0.0;
+          // scaled recurrence Q_l = P_l / c_lm (legendre_scale; c_lm is folded into
+          // the table): Q_l = ct Q_{l-1} - beta Q_{l-2}, beta = (l+m-1)/(l-m) c_{l-2,m} / c_lm,
+          // a constant after unrolling (a global read of it waited on its full latency)
+          const double beta = (l >= m + 2) ? (double)(l + m - 1) / (double)(l - m) * legendre_scale(l - 2, m) /
+                                                 legendre_scale(l, m)
+                                           : 0.0;
           double yp[Q], yn[Q];
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
@@ -1083,22 +1102,23 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             if (l == m)
               pl = pmm[q];
             else if (l == m + 1)
-              pl = ct[q] * (double)(2 * m + 1) * pmm[q];
+              pl = ct[q] * pmm[q];
             else
-              pl = ra * ct[q] * p1[q] - rb * p2[q];
+              pl = ct[q] * p1[q] - beta * p2[q];
             p2[q] = p1[q];
             p1[q] = pl;
-            const double base = pl;  // N_lm is folded into the table
+            const double base = pl;  // N_lm c_lm is folded into the table
             yp[q] = (m == 0) ? base : base * cm[q];
             yn[q] = base * sm[q];
           }
-          // (a run-time l range: its branches also keep the table reads from being
-          // hoisted en masse, which would spill at 1024 threads)
-          if (l >= A.l0 && l < A.l1) {
-            const int ip = (l * l + l + m - lmb) * 2;
-            const int in = (l * l + l - m - lmb) * 2;
-            const double* T0 = tab + s0 * nlmc * 2;
-            const double* T1 = tab + s1 * nlmc * 2;
+          // (a run-time l range, one bit per l of a scalar, re-read for every slot
+          // pair so that the 45 tests are not hoisted as masks: its branches keep the
+          // table reads from being hoisted en masse, which would spill at 1024
+          // threads; the rows are addressed from lm = 0, so every read's offset is a
+          // compile-time immediate)
+          if ((lb >> l) & 1u) {
+            const int ip = (l * l + l + m) * 2;
+            const int in = (l * l + l - m) * 2;
             // 16-byte {y, D} pairs (the table and its rows start on 16-byte boundaries)
             const double2 t0p = *reinterpret_cast<const double2*>(T0 + ip);
             const double2 t1p = *reinterpret_cast<const double2*>(T1 + ip);
@@ -3134,12 +3154,13 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
         kd[t] = acc;
       }
     }
-  // spherical-harmonic recurrence constants per (l, m >= 0): {N_lm, (2l-1)/(l-m), (l+m-1)/(l-m)}
+  // spherical-harmonic recurrence constants per (l, m >= 0): {N_lm c_lm (the table
+  // scale for the cross kernel's scaled recurrence), (2l-1)/(l-m), (l+m-1)/(l-m)}
   std::vector<double> shc(3 * (kMaxL + 1) * (kMaxL + 2) / 2, 0.0);
   for (int l = 0; l <= kMaxL; ++l)
     for (int m = 0; m <= l; ++m) {
       const int ci = 3 * (l * (l + 1) / 2 + m);
-      shc[ci] = dg::sh_prefactor(l, m);
+      shc[ci] = dg::sh_prefactor(l, m) * legendre_scale(l, m);
       if (l >= m + 2) {
         shc[ci + 1] = (double)(2 * l - 1) / (double)(l - m);
         shc[ci + 2] = (double)(l + m - 1) / (double)(l - m);
