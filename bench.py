@@ -32,6 +32,22 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 HARTREE_FLOPS_PER_TERM = 10.0
+
+
+def traffic_key(label, names):
+    """The rocprofv3 kernel name (with template arguments) that runs build stage
+    `label` in an FP64 dg_build_device, among `names`; None if absent."""
+    for k in sorted(names):
+        base = k.split("<")[0]
+        args = [a.strip() for a in k[k.find("<") + 1:k.rfind(">")].split(",")] if "<" in k else []
+        if label == "contract" and ((base == "contract2d" and args[:2] == ["double", "2"]) or
+                                    (base == "contract_small" and len(args) > 1 and args[1] == "2")):
+            return k
+        if label == "density" and base in ("density", "density_pl"):
+            return k
+        if label == "hartree_cross" and base == "hartree_cross":
+            return k
+    return None
 CPU_SAMPLE_MAX_POINTS = 300000
 
 
@@ -174,14 +190,14 @@ def bench_batch(args, G, M, np, torch):
         res, its = G.batch_scf_run(mols, nrad, nang, lmax, threshold=1e-6)
         conv = [r["converged"] == 1 for r in res]
         its_conv = [r["iterations"] for r, c in zip(res, conv) if c]
-        # wall time per batch iteration up to the last molecule that converged
-        # (iterations after it only carry molecules that stopped at max_iter)
-        out["scf"] = {"batch_iterations": its, "s_per_iter": res[0]["seconds_iter"],
+        # s_per_batch_iter: wall time of one batch iteration (every molecule advances)
+        out["scf"] = {"batch_iterations": its, "s_per_batch_iter": res[0]["seconds_iter"],
                       "setup_s": res[0]["seconds_setup"], "converged": int(sum(conv)),
                       "not_converged": [{"molecule": m, "name": mols[m].name, "basis": mols[m].basis,
                                          "iterations": res[m]["iterations"], "last_dE": res[m]["delta"]}
                                         for m in range(len(res)) if not conv[m]],
-                      "max_converged_iterations": max(its_conv) if its_conv else None}
+                      "max_converged_iterations": max(its_conv) if its_conv else None,
+                      "s_until_last_converged": (max(its_conv) * res[0]["seconds_iter"]) if its_conv else None}
     print(json.dumps(out))
 
 
@@ -327,6 +343,8 @@ def main():
             roof.update({"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": None})
         roof["avg_launch_ms"] = avg_s * 1e3
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # summary of this workload (keyed by kernel name)
         roof["traffic"] = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
         if args.precision != "fp64" or args.screen != 0.0:
@@ -334,7 +352,10 @@ def main():
         if os.path.exists(tpath):
             with open(tpath) as f:
                 t = json.load(f)
-            roof["traffic"] = t.get(dom)
+            kname = traffic_key(dom, t.keys())
+            if kname is not None:
+                roof["traffic"] = t[kname]
+                roof["traffic_source"] = f"profiles/traffic_{args.workload}.json[{kname}]"
         out["roofline"] = roof
 
     # the reference's call sequence with host buffers (P in, J and XC out over PCIe,
@@ -405,6 +426,8 @@ def main():
             del gv, bv
 
     if not args.no_scf:
+        # the reference SCF (plain 0.5 damping, |dE| < 1e-6, at most 100 iterations);
+        # density_step AUTO forms P on the device for 128 functions and more
         scf = {}
         for name, pr, sc in (("fp64", G.PREC_FP64, 0.0), ("fp64_screened", G.PREC_FP64, args.variant_screen),
                              ("mixed", G.PREC_MIXED, 0.0),
@@ -414,10 +437,15 @@ def main():
             en, res, _ = G.scf_run(mol, nrad, nang, lmax, threshold=1e-6, precision=pr, screen=sc)
             scf[name] = {"energy_Ha": res["energy"], "iterations": res["iterations"],
                          "converged": bool(res["converged"]), "last_dE": res["delta"],
-                         "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"]}
+                         "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"],
+                         "device_step": bool(res["device_step"]), "host_fallbacks": res["host_fallbacks"]}
         for name in ("fp64_screened", "mixed", "mixed_screened"):
             if name in scf:
                 scf[name]["energy_error_vs_fp64_Ha"] = scf[name]["energy_Ha"] - scf["fp64"]["energy_Ha"]
+        # the same iterations with the host eigen-solver step (s/iteration comparison)
+        _, rh, _ = G.scf_run(mol, nrad, nang, lmax, max_iter=8, threshold=0.0, density_step=G.STEP_HOST)
+        scf["fp64_host_step_8_iterations"] = {"s_per_iter": rh["seconds_iter"], "device_step": False}
+        scf["grid_build_ms"] = ms_per_step
         out["scf"] = scf
 
     if not args.no_cpu_baseline:

@@ -254,11 +254,13 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // One wave handles 16 points; lane (c = lane%16, g = lane/16) keeps
 // phi[p0+c][kk(s,g)] for every reduction step s in registers (B operand).
 // For output tile b the wave forms T_b[p][16b+4v+g] (result register v) with
-// 4(b+1) MFMAs, then dots it with phi[p][16b+4v+g].  The P fragments of one
-// tile are staged in LDS and shared by the block's 4 waves, double-buffered:
-// the next live tile's fragments are copied from global memory straight into
-// the other buffer (global_load_lds, 1 KB per wave instruction) while the
-// current tile's MFMAs run; one wait and barrier per tile.
+// 4(b+1) MFMAs in two accumulation chains (steps j even / odd), then dots it
+// with phi[p][16b+4v+g], which is its own B-operand register f[4b+v] (the tile
+// loop is unrolled, so every register index is a constant).  The P fragments
+// of one tile are staged in LDS and shared by the work-group's waves,
+// double-buffered: the next live tile's fragments are copied from global
+// memory straight into the other buffer (global_load_lds, 1 KB per wave
+// instruction) while the current tile's MFMAs run; one wait and barrier per tile.
 // NT = ceil(nbf/16).  W = 8 waves (128 points) share the fragments, two per
 // SIMD: one issues MFMAs while the other waits on LDS or the tile barrier.
 // With screening, mask (one word per 64-point block, bit T for basis tile T)
@@ -277,7 +279,8 @@ __global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ p
   const double* row = phi + (p0 + c) * ldphi;
   constexpr unsigned all = NT >= 32 ? 0xffffffffu : ((1u << NT) - 1u);
   const long blk0 = (long)blockIdx.x * W / 4;  // first 64-point block of the work-group
-  const unsigned mine = (mask ? mask[blk0 + wave / 4] : 0xffffffffu) & all;
+  // wave-uniform (the compiler cannot see that from threadIdx)
+  const unsigned mine = __builtin_amdgcn_readfirstlane((mask ? mask[blk0 + wave / 4] : 0xffffffffu) & all);
   unsigned mk = 0;
   for (int b = 0; b < W / 4; ++b) mk |= (mask ? mask[blk0 + b] : 0xffffffffu) & all;
   double f[4 * NT];
@@ -296,42 +299,42 @@ __global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ p
     return rest ? b + 1 + __builtin_ctz(rest) : NT;
   };
   double acc = 0.0;
-  int b = next_live(-1), buf = 0;
-  if (b < NT) fetch(b, pl[0]);
+  int buf = 0;
+  {
+    const int b0 = next_live(-1);
+    if (b0 < NT) fetch(b0, pl[0]);
+  }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  while (b < NT) {
+  // tiles in compile-time order (each skipped when off the mask), so that the
+  // amplitudes of the final dot are the B-operand registers f[4b + v]
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+    if (!((mk >> b) & 1u)) continue;
     const int nb = next_live(b);
-    // phi[p][16b + 4g + v], v < 4, for the final dot; loaded ahead of the copies
-    // and MFMAs (16-byte aligned: ldphi and the row offset are even)
-    const double2 r01 = *reinterpret_cast<const double2*>(row + 16 * b + 4 * g);
-    const double2 r23 = *reinterpret_cast<const double2*>(row + 16 * b + 4 * g + 2);
     if (nb < NT) fetch(nb, pl[buf ^ 1]);
     const double* cur = pl[buf];
-    dbl4 d = {0.0, 0.0, 0.0, 0.0};
-    // reduction tiles k <= b, four steps each, one branch per tile; the LDS
-    // operands of tile k + 1 are read before the MFMAs of tile k issue
-    double nxt[4];
+    // two accumulation chains (steps j even / odd) over the reduction tiles k <= b
+    dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) nxt[j] = cur[j * 64 + lane];
+    for (int k = 0; k <= b; ++k) {
+      if ((mine >> k) & 1u) {
+        double av[4];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      double av[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) av[j] = nxt[j];
-      if (k + 1 <= b)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) nxt[j] = cur[(4 * (k + 1) + j) * 64 + lane];
-      if (k <= b && ((mine >> k) & 1u))
-#pragma unroll
-        for (int j = 0; j < 4; ++j) d = mfma16x16x4(av[j], f[4 * k + j], d);
+        for (int j = 0; j < 4; ++j) av[j] = cur[(4 * k + j) * 64 + lane];
+        d0 = mfma16x16x4(av[0], f[4 * k], d0);
+        d1 = mfma16x16x4(av[1], f[4 * k + 1], d1);
+        d0 = mfma16x16x4(av[2], f[4 * k + 2], d0);
+        d1 = mfma16x16x4(av[3], f[4 * k + 3], d1);
+      }
     }
-    // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]
-    if ((mine >> b) & 1u) acc += d[0] * r01.x + d[1] * r01.y + d[2] * r23.x + d[3] * r23.y;
+    // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]; f[4b + v] = phi[p0+c][16b + 4g + v]
+    if ((mine >> b) & 1u)
+      acc += (d0[0] + d1[0]) * f[4 * b] + (d0[1] + d1[1]) * f[4 * b + 1] + (d0[2] + d1[2]) * f[4 * b + 2] +
+             (d0[3] + d1[3]) * f[4 * b + 3];
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
     __syncthreads();
     buf ^= 1;
-    b = nb;
   }
   acc += __shfl_xor(acc, 16);
   acc += __shfl_xor(acc, 32);
