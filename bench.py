@@ -440,8 +440,11 @@ def main():
                          "s_per_iter": res["seconds_iter"], "setup_s": res["seconds_setup"],
                          "device_step": bool(res["device_step"]), "host_fallbacks": res["host_fallbacks"]}
         for name in ("fp64_screened", "mixed", "mixed_screened"):
+            # a difference of energies means something only when both runs converged
+            # (the reference SCF oscillates without settling on the long alkanes)
             if name in scf:
-                scf[name]["energy_error_vs_fp64_Ha"] = scf[name]["energy_Ha"] - scf["fp64"]["energy_Ha"]
+                both = scf[name]["converged"] and scf["fp64"]["converged"]
+                scf[name]["energy_error_vs_fp64_Ha"] = (scf[name]["energy_Ha"] - scf["fp64"]["energy_Ha"]) if both else None
         # the same iterations with the host eigen-solver step (s/iteration comparison)
         _, rh, _ = G.scf_run(mol, nrad, nang, lmax, max_iter=8, threshold=0.0, density_step=G.STEP_HOST)
         scf["fp64_host_step_8_iterations"] = {"s_per_iter": rh["seconds_iter"], "device_step": False}
