@@ -1034,8 +1034,7 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
       const bool clamp = key == 0 || key == n;
       const int s1 = clamp ? (key == 0 ? 0 : n - 1) : key;
       const int s0 = clamp ? s1 : key - 1;
-      unsigned lb = lbits;
-      asm volatile("" : "+s"(lb));
+
       // knot rows s0 and s1 of the table, indexed by lm from 0 (entries below lmb are not read)
       const double* T0 = tab + (s0 * nlmc - lmb) * 2;
       const double* T1 = tab + (s1 * nlmc - lmb) * 2;
@@ -1114,12 +1113,14 @@ __global__ __launch_bounds__(1024) void k_hartree_cross(CrossArgs A) {
             yp[q] = (m == 0) ? base : base * cm[q];
             yn[q] = base * sm[q];
           }
-          // (a run-time l range, one bit per l of a scalar, re-read for every slot
-          // pair so that the 45 tests are not hoisted as masks: its branches keep the
-          // table reads from being hoisted en masse, which would spill at 1024
-          // threads; the rows are addressed from lm = 0, so every read's offset is a
-          // compile-time immediate)
-          if ((lb >> l) & 1u) {
+          // (a run-time l range, one bit per l of a scalar, tested where it is used:
+          // the opaque copy keeps the tests from being hoisted and carried as lane
+          // masks, and the branches keep the table reads from being hoisted en masse,
+          // which would spill at 1024 threads; the rows are addressed from lm = 0, so
+          // every read's offset is a compile-time immediate)
+          unsigned lb = lbits;
+          asm volatile("" : "+s"(lb));
+          if ((__builtin_amdgcn_readfirstlane(lb) >> l) & 1u) {
             const int ip = (l * l + l + m) * 2;
             const int in = (l * l + l - m) * 2;
             // 16-byte {y, D} pairs (the table and its rows start on 16-byte boundaries)
@@ -1664,31 +1665,50 @@ __global__ void k_pfrag_full(const double* __restrict__ P, int nbf, int nt, doub
 }
 
 // T = Phi P (T[p][j] = sum_k phi_pk P_kj) with the FP64 MFMA, same operand
-// arrangement as k_density; T is written [npts_pad][ldphi].
-template <int NT>
-__global__ __launch_bounds__(256) void k_tmat(const double* __restrict__ phi, int ldphi,
-                                              const double* __restrict__ frag, double* tmat) {
-  __shared__ double pl[4 * NT * 64];
+// arrangement as k_density; T is written [npts_pad][ldphi].  W waves (16
+// points each) share each column tile's fragments in LDS, double-buffered: the
+// next tile's are copied by global_load_lds while the current tile's MFMAs run
+// in four accumulation chains; one wait and barrier per tile.
+template <int NT, int W>
+__global__ __launch_bounds__(64 * W) void k_tmat(const double* __restrict__ phi, int ldphi,
+                                                 const double* __restrict__ frag, double* tmat) {
+  __shared__ double pl[2][4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const long p0 = (long)blockIdx.x * 16 * W + wave * 16;
   const double* row = phi + (p0 + c) * ldphi;
   double f[4 * NT];
 #pragma unroll
   for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
-  for (int b = 0; b < NT; ++b) {
-    __syncthreads();
+  // column tile b's fragments: 256 NT doubles from 256 NT b, chunks of 128
+  // doubles, chunk q copied by wave q % W, 16 bytes per lane
+  auto fetch = [&](int b, double* dst) {
     const double* src = frag + 256L * NT * b;
-    for (int e = threadIdx.x; e < 4 * NT * 64; e += 256) pl[e] = src[e];
-    __syncthreads();
-    dbl4 d = {0.0, 0.0, 0.0, 0.0};
+    for (int q = wave; q < 2 * NT; q += W)
+      __builtin_amdgcn_global_load_lds((const void*)(src + q * 128 + lane * 2),
+                                       (__attribute__((address_space(3))) void*)(dst + q * 128), 16, 0, 0);
+  };
+  fetch(0, pl[0]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int b = 0; b < NT; ++b) {
+    if (b + 1 < NT) fetch(b + 1, pl[(b + 1) & 1]);
+    const double* cur = pl[b & 1];
+    dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0, d3 = d0;
 #pragma unroll
-    for (int s = 0; s < 4 * NT; ++s) d = mfma16x16x4(pl[s * 64 + lane], f[s], d);
+    for (int k = 0; k < NT; ++k) {
+      d0 = mfma16x16x4(cur[(4 * k) * 64 + lane], f[4 * k], d0);
+      d1 = mfma16x16x4(cur[(4 * k + 1) * 64 + lane], f[4 * k + 1], d1);
+      d2 = mfma16x16x4(cur[(4 * k + 2) * 64 + lane], f[4 * k + 2], d2);
+      d3 = mfma16x16x4(cur[(4 * k + 3) * 64 + lane], f[4 * k + 3], d3);
+    }
     double* out = tmat + (p0 + c) * ldphi + 16 * b + g;
-    out[0] = d[0];
-    out[4] = d[1];
-    out[8] = d[2];
-    out[12] = d[3];
+    out[0] = (d0[0] + d1[0]) + (d2[0] + d3[0]);
+    out[4] = (d0[1] + d1[1]) + (d2[1] + d3[1]);
+    out[8] = (d0[2] + d1[2]) + (d2[2] + d3[2]);
+    out[12] = (d0[3] + d1[3]) + (d2[3] + d3[3]);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
+    __syncthreads();
   }
 }
 
@@ -2506,7 +2526,15 @@ int launch_density(dg_handle* h, const double* dP) {
 
 template <int NT>
 void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat) {
-  hipLaunchKernelGGL(k_tmat<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
+  // eight waves (two per SIMD) when the rows come in 128s and the amplitude
+  // registers (8 NT) leave room for both
+  if constexpr (NT <= 25) {
+    if (npts_pad % 128 == 0) {
+      hipLaunchKernelGGL((k_tmat<NT, 8>), dim3(npts_pad / 128), dim3(512), 0, st, phi, ldphi, frag, tmat);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tmat<NT, 4>), dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
 }
 
 // T = Phi P on npts_pad (a multiple of 64) rows, from full P fragments of nt tiles.
