@@ -1238,9 +1238,10 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // ---------------------------------------------------------------------------
 // Contractions  C_o = Phi^T diag(u_o) Phi  (src/dft.cpp:424-432, src/atomicgrid.cpp:471-488)
 // ---------------------------------------------------------------------------
-// Contraction for larger bases with 4 x 2 tile blocks: block {I0, nI, J0, nJ}
-// covers the tiles (I0 + r, J0 + t), r < nI, t < nJ, I <= J, of both outputs,
-// so a wave loads 4 + 2 amplitude tiles per step (4 points) for 16 MFMAs.  The
+// Contraction for larger bases with 4 x NJ tile blocks: block {I0, nI, J0, nJ}
+// covers the tiles (I0 + r, J0 + t), r < nI, t < nJ <= NJ, I <= J, of the NOUT
+// outputs, so a wave loads 4 + NJ amplitude tiles per step (4 points) for
+// 4 NJ NOUT MFMAs: NJ = 2 with two outputs (J and XC), NJ = 4 with one.  The
 // block's 64-point blocks (its list: all, or with screening those where its
 // tiles are significant) are split evenly over nsplit work-groups (split-K);
 // the 4 waves interleave the 16 points of each block, the next step's operands
@@ -1257,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // whole splits, and within an XCD's range the blocks with the most MFMAs come
 // first, so that the short diagonal blocks fill the last round.
 constexpr int kXcds = 8;
-template <typename T, int NOUT, bool DENSE>
+template <typename T, int NOUT, int NJ, bool DENSE>
 __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(const T* __restrict__ phi, int ldphi,
                                                     const double* __restrict__ u0,
                                                     const double* __restrict__ u1,
@@ -1267,7 +1268,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
                                                     const int2* __restrict__ order, int ngroups, int nsplit,
                                                     double* part) {
   constexpr bool f32 = sizeof(T) == 4;
-  __shared__ double red[(f32 ? NOUT : 1) * 8 * 256];
+  __shared__ double red[(f32 ? NOUT : 1) * 4 * NJ * 256];
   const int ncb = ngroups / 4, per = gridDim.x / kXcds;
   const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (v >= ncb * nsplit) return;
@@ -1283,26 +1284,26 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
   auto point = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave + g; };
   const int ia = 16 * I0 + c, jb = 16 * J0 + c;
   typedef typename std::conditional<f32, flt4, dbl4>::type accT;
-  accT acc[NOUT][4][2];
+  accT acc[NOUT][4][NJ];
 #pragma unroll
   for (int o = 0; o < NOUT; ++o)
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int t = 0; t < 2; ++t) acc[o][r][t] = accT{0, 0, 0, 0};
+      for (int t = 0; t < NJ; ++t) acc[o][r][t] = accT{0, 0, 0, 0};
   if constexpr (f32) {
-    for (int e = threadIdx.x; e < NOUT * 8 * 256; e += 256) red[e] = 0.0;
+    for (int e = threadIdx.x; e < NOUT * 4 * NJ * 256; e += 256) red[e] = 0.0;
     __syncthreads();
   }
   // operands of the next PF steps in flight
   constexpr int PF = 1;
-  T a[PF][4], bb[PF][2], w0[PF], w1[PF];
+  T a[PF][4], bb[PF][NJ], w0[PF], w1[PF];
   auto load = [&](int k, long p) {
     const T* row = phi + p * ldphi;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[k][r] = (r < nI) ? row[ia + 16 * r] : (T)0;
+    for (int r = 0; r < 4; ++r) a[k][r] = row[ia + 16 * r];  // tiles r >= nI, t >= nJ: read, never used
 #pragma unroll
-    for (int t = 0; t < 2; ++t) bb[k][t] = (t < nJ) ? row[jb + 16 * t] : (T)0;
+    for (int t = 0; t < NJ; ++t) bb[k][t] = row[jb + 16 * t];
     w0[k] = (T)u0[p];
     w1[k] = (NOUT == 2) ? (T)u1[p] : (T)0;
   };
@@ -1310,18 +1311,18 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
   for (int k = 0; k < PF; ++k)
     if (k < nst) load(k, point(k));
   for (int st = 0; st < nst; ++st) {
-    T ca[4], cb[2];
+    T ca[4], cb[NJ];
 #pragma unroll
     for (int r = 0; r < 4; ++r) ca[r] = a[0][r];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) cb[t] = bb[0][t];
+    for (int t = 0; t < NJ; ++t) cb[t] = bb[0][t];
     const T cw0 = w0[0], cw1 = w1[0];
 #pragma unroll
     for (int k = 0; k + 1 < PF; ++k) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[k][r] = a[k + 1][r];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) bb[k][t] = bb[k + 1][t];
+      for (int t = 0; t < NJ; ++t) bb[k][t] = bb[k + 1][t];
       w0[k] = w0[k + 1];
       w1[k] = w1[k + 1];
     }
@@ -1337,7 +1338,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
       }
     };
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < NJ; ++t) {
       const T b0 = cw0 * cb[t], b1 = cw1 * cb[t];
 #pragma unroll
       for (int r = 0; r < 4; ++r)
@@ -1353,9 +1354,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
 #pragma unroll
               for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
+                for (int t = 0; t < NJ; ++t) {
 #pragma unroll
-                  for (int v = 0; v < 4; ++v) red[(o * 8 + r * 2 + t) * 256 + g * 64 + c + 16 * v] += (double)acc[o][r][t][v];
+                  for (int v = 0; v < 4; ++v)
+                    red[((o * 4 + r) * NJ + t) * 256 + g * 64 + c + 16 * v] += (double)acc[o][r][t][v];
                   acc[o][r][t] = accT{0, 0, 0, 0};
                 }
           }
@@ -1367,9 +1369,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
     __syncthreads();
 #pragma unroll
     for (int o = 0; o < NOUT; ++o)
-      for (int e = threadIdx.x; e < 8 * 256; e += 256) {
-        const int r = e >> 9, t = (e >> 8) & 1, rem = e & 255;
-        part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[o * 8 * 256 + e];
+      for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
+        const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
+        part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[o * 4 * NJ * 256 + e];
       }
     return;
   }
@@ -1381,19 +1383,19 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int t = 0; t < 2; ++t)
+          for (int t = 0; t < NJ; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               // register v of lane (c, g) = row 4v + g
-              double* q = red + (r * 2 + t) * 256 + v * 64 + lane;
+              double* q = red + (r * NJ + t) * 256 + v * 64 + lane;
               const double x = (double)acc[o][r][t][v];
               *q = (wv == 0) ? x : *q + x;
             }
       }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 8 * 256; e += 256) {
-      const int r = e >> 9, t = (e >> 8) & 1, rem = e & 255;
+    for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
+      const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
       part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[e];
     }
   }
@@ -2302,6 +2304,17 @@ struct ProfStat {
   int n = 0;
 };
 
+// Tile blocks of k_contract2d for one block width NJ: {I0, nI, J0, nJ} per
+// block, its four reduction groups (I0 + r, J0, nJ) in the k_contract_reduce
+// layout, the 64-point block list of each block (all blocks, or with screening
+// those where its tiles are significant), and per virtual work-group id the
+// {tile block, split} it runs.
+struct TileBlocks {
+  int *cb = nullptr, *groups = nullptr, *list = nullptr, *loff = nullptr;
+  int2* order = nullptr;
+  int ncb = 0, nsplit = 0;
+};
+
 struct dg_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -2310,10 +2323,8 @@ struct dg_handle {
   long npts = 0, npts_pad = 0, cell = 0;
   int ldphi = 0, ntb = 0, ngroups = 0, nsplit_small = 0;
   int cus = 256;  // compute units of the device
-  // 4 x 2 tile blocks of k_contract2d: {I0, nI, J0, nJ}, their reduction groups and point lists
-  int *cb = nullptr, *cb_groups = nullptr, *cblist = nullptr, *cbloff = nullptr;
-  int2* cborder = nullptr;  // per virtual work-group id, {tile block, split}
-  int ncb = 0, nsplit2d = 0;
+  // tile blocks of k_contract2d: [0] 4 x 2 (two outputs), [1] 4 x 4 (one output)
+  TileBlocks tb[2];
   long nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
   VwnConst vwn{};
@@ -2388,9 +2399,11 @@ struct dg_handle {
       if (b) (void)hipFree(b);
     int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, plen, ash_start, ash_list,
                     atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0,
-                    cb, cb_groups, cblist, cbloff};
+                    tb[0].cb, tb[0].groups, tb[0].list, tb[0].loff, tb[1].cb, tb[1].groups, tb[1].list,
+                    tb[1].loff};
     if (bmask) (void)hipFree(bmask);
-    if (cborder) (void)hipFree(cborder);
+    for (TileBlocks& t : tb)
+      if (t.order) (void)hipFree(t.order);
     if (vbase) (void)hipFree(vbase);
     if (perm) (void)hipFree(perm);
     if (pkey) (void)hipFree(pkey);
@@ -2810,14 +2823,18 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
   }
 }
 
+// two outputs on 4 x 2 tile blocks, one output on 4 x 4 (the same 16 MFMAs per step)
 template <typename T, int NOUT>
-void launch_contract2d(dg_handle* h, dim3 grid, const T* phi, const double* u0, const double* u1) {
+void launch_contract2d(dg_handle* h, const T* phi, const double* u0, const double* u1) {
+  constexpr int NJ = NOUT == 2 ? 2 : 4;
+  const TileBlocks& t = h->tb[NOUT == 2 ? 0 : 1];
+  const dim3 grid(round_up((long)t.ncb * t.nsplit, kXcds));
   if (h->screen == 0.0)
-    hipLaunchKernelGGL((k_contract2d<T, NOUT, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, h->cb,
-                       h->cblist, h->cbloff, h->cborder, 4 * h->ncb, h->nsplit2d, h->cpart);
+    hipLaunchKernelGGL((k_contract2d<T, NOUT, NJ, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, t.cb,
+                       t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
   else
-    hipLaunchKernelGGL((k_contract2d<T, NOUT, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, h->cb,
-                       h->cblist, h->cbloff, h->cborder, 4 * h->ncb, h->nsplit2d, h->cpart);
+    hipLaunchKernelGGL((k_contract2d<T, NOUT, NJ, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1,
+                       t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
 }
 
 // sa (optional): the build's scalars, evaluated by one more work-group of the reduction
@@ -2833,26 +2850,25 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
     if (small) {
       launch_contract_small(h, u0, u1, nout);
     } else if (mx) {
-      const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
-        launch_contract2d<float, 2>(h, g2, h->phf, u0, u1);
+        launch_contract2d<float, 2>(h, h->phf, u0, u1);
       else
-        launch_contract2d<float, 1>(h, g2, h->phf, u0, u0);
+        launch_contract2d<float, 1>(h, h->phf, u0, u0);
     } else {
-      const dim3 g2(round_up((long)h->ncb * h->nsplit2d, kXcds));
       if (nout == 2)
-        launch_contract2d<double, 2>(h, g2, h->phi, u0, u1);
+        launch_contract2d<double, 2>(h, h->phi, u0, u1);
       else
-        launch_contract2d<double, 1>(h, g2, h->phi, u0, u0);
+        launch_contract2d<double, 1>(h, h->phi, u0, u0);
     }
   }));
   const bool two_d = !small;
-  const int ng = two_d ? 4 * h->ncb : h->ngroups;
+  const TileBlocks& tb = h->tb[nout == 2 ? 0 : 1];
+  const int ng = two_d ? 4 * tb.ncb : h->ngroups;
   ScalarArgs none{};
   const unsigned nb = nblocks((long)ng * nout * 1024, 16) + (sa ? 1 : 0);
   return run(h, "contract_reduce", [&] {
     hipLaunchKernelGGL(k_contract_reduce, dim3(nb), dim3(256), 0, h->stream, h->cpart,
-                       small ? h->nsplit_small : h->nsplit2d, ng, nout, two_d ? h->cb_groups : h->blocks, h->nbf,
+                       small ? h->nsplit_small : tb.nsplit, ng, nout, two_d ? tb.groups : h->blocks, h->nbf,
                        out0, out1, sa ? *sa : none);
   });
 }
@@ -3213,14 +3229,17 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       blocks.push_back(std::min(4, h->ntb - J0));
     }
   h->ngroups = (int)blocks.size() / 3;
-  std::vector<int> cbs, cbg;
-  for (int I0 = 0; I0 < h->ntb; I0 += 4)
-    for (int J0 = I0; J0 < h->ntb; J0 += 2) {
-      const int nI = std::min(4, h->ntb - I0), nJ = std::min(2, h->ntb - J0);
-      cbs.insert(cbs.end(), {I0, nI, J0, nJ});
-      for (int r = 0; r < 4; ++r) cbg.insert(cbg.end(), {I0 + r, J0, r < nI ? nJ : 0});
-    }
-  h->ncb = (int)cbs.size() / 4;
+  std::vector<int> cbs[2], cbg[2];
+  for (int q = 0; q < 2; ++q) {
+    const int nj = q ? 4 : 2;
+    for (int I0 = 0; I0 < h->ntb; I0 += 4)
+      for (int J0 = I0; J0 < h->ntb; J0 += nj) {
+        const int nI = std::min(4, h->ntb - I0), nJ = std::min(nj, h->ntb - J0);
+        cbs[q].insert(cbs[q].end(), {I0, nI, J0, nJ});
+        for (int r = 0; r < 4; ++r) cbg[q].insert(cbg[q].end(), {I0 + r, J0, r < nI ? nJ : 0});
+      }
+    h->tb[q].ncb = (int)cbs[q].size() / 4;
+  }
   h->nlda_blocks = h->npts_pad / 256;
   // cross kernel: l ranges whose spline table fits in LDS beside an item's potentials
   {
@@ -3284,21 +3303,26 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     // the longest blocks first, the short ones even out the last round (C30H62: one
     // round 14.3 ms, two 12.8, three 12.35, four 12.4)
     const long slots = (long)cus * (h->precision == DG_PREC_MIXED ? 4 : 3);
-    h->nsplit2d = (int)std::max(1L, std::min(std::min(512L, 3 * slots / std::max(h->ncb, 1)), h->npts_pad / 64));
-    // virtual work-group order of k_contract2d: split-major, each XCD's range
-    // re-ordered by decreasing tile pairs of the block (stable)
-    const long nv = (long)h->ncb * h->nsplit2d, per = round_up(nv, kXcds) / kXcds;
-    std::vector<int> work(std::max(h->ncb, 1), 0);
-    for (int b = 0; b < h->ncb; ++b)
-      for (int r = 0; r < cbs[4 * b + 1]; ++r)
-        for (int t = 0; t < cbs[4 * b + 3]; ++t) work[b] += (cbs[4 * b] + r <= cbs[4 * b + 2] + t) ? 1 : 0;
-    std::vector<int2> order(std::max<long>(nv, 1));
-    for (long v = 0; v < nv; ++v) order[v] = make_int2((int)(v % std::max(h->ncb, 1)), (int)(v / std::max(h->ncb, 1)));
-    for (long x0 = 0; x0 < nv; x0 += per)
-      std::stable_sort(order.begin() + x0, order.begin() + std::min(nv, x0 + per),
-                       [&](const int2& a, const int2& b) { return work[a.x] > work[b.x]; });
-    DG_TRY(dalloc(&h->cborder, order.size()));
-    HIP_TRY(hipMemcpy(h->cborder, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+    for (int q = 0; q < 2; ++q) {
+      TileBlocks& t = h->tb[q];
+      const std::vector<int>& c = cbs[q];
+      t.nsplit = (int)std::max(1L, std::min(std::min(512L, 3 * slots / std::max(t.ncb, 1)), h->npts_pad / 64));
+      // virtual work-group order of k_contract2d: split-major, each XCD's range
+      // re-ordered by decreasing tile pairs of the block (stable)
+      const long nv = (long)t.ncb * t.nsplit, per = round_up(nv, kXcds) / kXcds;
+      std::vector<int> work(std::max(t.ncb, 1), 0);
+      for (int b = 0; b < t.ncb; ++b)
+        for (int r = 0; r < c[4 * b + 1]; ++r)
+          for (int u = 0; u < c[4 * b + 3]; ++u) work[b] += (c[4 * b] + r <= c[4 * b + 2] + u) ? 1 : 0;
+      std::vector<int2> order(std::max<long>(nv, 1));
+      for (long v = 0; v < nv; ++v)
+        order[v] = make_int2((int)(v % std::max(t.ncb, 1)), (int)(v / std::max(t.ncb, 1)));
+      for (long x0 = 0; x0 < nv; x0 += per)
+        std::stable_sort(order.begin() + x0, order.begin() + std::min(nv, x0 + per),
+                         [&](const int2& a, const int2& b) { return work[a.x] > work[b.x]; });
+      DG_TRY(dalloc(&t.order, order.size()));
+      HIP_TRY(hipMemcpy(t.order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
   }
 
   // allocations
@@ -3310,7 +3334,9 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->leb, 4 * (size_t)h->nang));
   for (double** b : {&h->px, &h->py, &h->pz, &h->w, &h->wb, &h->rho, &h->uxc, &h->exd, &h->uJ, &h->V, &h->vcell})
     DG_TRY(dalloc(b, np));
-  DG_TRY(dalloc(&h->phi, (size_t)np * h->ldphi));
+  // one row of slack: k_contract2d reads whole 4 x NJ tile blocks, whose tiles
+  // past the last function tile reach up to 48 values beyond a row
+  DG_TRY(dalloc(&h->phi, (size_t)(np + 1) * h->ldphi));
   DG_TRY(dalloc(&h->sh_atom, h->nshell));
   DG_TRY(dalloc(&h->sh_L, h->nshell));
   DG_TRY(dalloc(&h->sh_nprim, h->nshell));
@@ -3350,13 +3376,17 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->plen, h->items_cap / 4));
   HIP_TRY(hipMalloc((void**)&h->perm, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
   HIP_TRY(hipMalloc((void**)&h->pkey, std::max<long>(h->npairs, 1) * sizeof(unsigned short)));
-  DG_TRY(dalloc(&h->cpart, std::max((size_t)h->nsplit_small * h->ngroups,
-                                     (size_t)h->nsplit2d * h->ncb * 4) * 2 * 1024));
-  DG_TRY(dalloc(&h->cb, cbs.size()));
-  DG_TRY(dalloc(&h->cb_groups, cbg.size()));
+  // partials: two outputs of the 4 x 2 blocks, one of the 4 x 4 blocks
+  DG_TRY(dalloc(&h->cpart, std::max({(size_t)h->nsplit_small * h->ngroups * 2 * 1024,
+                                      (size_t)h->tb[0].nsplit * h->tb[0].ncb * 4 * 2 * 1024,
+                                      (size_t)h->tb[1].nsplit * h->tb[1].ncb * 4 * 1024})));
+  for (int q = 0; q < 2; ++q) {
+    DG_TRY(dalloc(&h->tb[q].cb, cbs[q].size()));
+    DG_TRY(dalloc(&h->tb[q].groups, cbg[q].size()));
+  }
   DG_TRY(dalloc(&h->ej_part, h->npts_pad / 256));
   if (h->precision == DG_PREC_MIXED) {
-    HIP_TRY(hipMalloc((void**)&h->phf, (size_t)np * h->ldphi * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&h->phf, (size_t)(np + 1) * h->ldphi * sizeof(float)));
   }
   DG_TRY(dalloc(&h->J, (size_t)h->packed));
   DG_TRY(dalloc(&h->XC, (size_t)h->packed));
@@ -3386,10 +3416,12 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   HIP_TRY(up(h->invh, invh.data(), invh.size() * 8));
   HIP_TRY(up(h->KD, KD.data(), KD.size() * 8));
   HIP_TRY(up(h->blocks, blocks.data(), blocks.size() * 4));
-  HIP_TRY(up(h->cb, cbs.data(), cbs.size() * 4));
-  HIP_TRY(up(h->cb_groups, cbg.data(), cbg.size() * 4));
+  for (int q = 0; q < 2; ++q) {
+    HIP_TRY(up(h->tb[q].cb, cbs[q].data(), cbs[q].size() * 4));
+    HIP_TRY(up(h->tb[q].groups, cbg[q].data(), cbg[q].size() * 4));
+  }
   if (!items.empty()) HIP_TRY(up(h->items, items.data(), items.size() * 4));
-  HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)np * h->ldphi * 8, h->stream));
+  HIP_TRY(hipMemsetAsync(h->phi, 0, (size_t)(np + 1) * h->ldphi * 8, h->stream));
   HIP_TRY(up(h->vbase, vbase.data(), vbase.size() * 8));
   HIP_TRY(hipMemsetAsync(h->vpart, 0, (size_t)vtotal * 8, h->stream));
   HIP_TRY(hipMemsetAsync(h->rho, 0, (size_t)np * 8, h->stream));
@@ -3442,13 +3474,13 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
       HIP_TRY(hipMemcpyAsync(mk.data(), h->bmask, nblk * 4, hipMemcpyDeviceToHost, h->stream));
       DG_TRY(sync(h));
     }
-    {
+    for (int q = 0; q < 2; ++q) {
       std::vector<int> l2, o2;
-      tile_block_lists(mk, cbs, l2, o2);
-      DG_TRY(dalloc(&h->cblist, std::max<size_t>(l2.size(), 1)));
-      DG_TRY(dalloc(&h->cbloff, o2.size()));
-      HIP_TRY(hipMemcpy(h->cblist, l2.data(), l2.size() * 4, hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(h->cbloff, o2.data(), o2.size() * 4, hipMemcpyHostToDevice));
+      tile_block_lists(mk, cbs[q], l2, o2);
+      DG_TRY(dalloc(&h->tb[q].list, std::max<size_t>(l2.size(), 1)));
+      DG_TRY(dalloc(&h->tb[q].loff, o2.size()));
+      HIP_TRY(hipMemcpy(h->tb[q].list, l2.data(), l2.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(h->tb[q].loff, o2.data(), o2.size() * 4, hipMemcpyHostToDevice));
     }
   }
   DG_TRY(sync(h));
