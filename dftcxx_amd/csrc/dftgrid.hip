@@ -1401,6 +1401,129 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(cons
   }
 }
 
+// s_waitcnt immediate that waits until at most n of this wave's vector-memory
+// operations are outstanding (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// The FP64 k_contract2d with its operands staged through LDS: each wave copies
+// the amplitude rows and weights of step st + D - 1 into slot (st + D - 1) % D
+// of its own ring (global_load_lds, 16 bytes per lane, so the copies hold no
+// registers) while the MFMAs of step st read slot st % D; a step's copy is
+// NI vector-memory instructions, so waiting for vmcnt <= NI (D - 1) leaves the
+// later steps in flight.  Slot layout (doubles): the 4 points' rows of tiles
+// I0..I0+3 (point-major, 64 each), of tiles J0..J0+NJ-1 (16 NJ each), then
+// u0 and u1 of the 4 points.  Same blocks, order, splits and partial layout as
+// k_contract2d; the epilogue's LDS reduction reuses the ring.
+template <int NOUT, int NJ, int D, bool DENSE>
+__global__ __launch_bounds__(256, 3) void k_contract2d_lds(const double* __restrict__ phi, int ldphi,
+                                                          const double* __restrict__ u0,
+                                                          const double* __restrict__ u1,
+                                                          const int* __restrict__ cblocks,
+                                                          const int* __restrict__ blist,
+                                                          const int* __restrict__ loff,
+                                                          const int2* __restrict__ order, int ngroups, int nsplit,
+                                                          double* part) {
+  constexpr int SA = 256, SB = 64 * NJ, SW = 8, SLOT = SA + SB + SW;  // doubles per slot
+  constexpr int NI = 2 + NJ / 2 + 1;                                    // copy instructions per step
+  static_assert(4 * SLOT * D >= 4 * NJ * 256, "the epilogue reduction fits in the ring");
+  __shared__ __attribute__((aligned(16))) double ring[4 * SLOT * D];
+  double* red = ring;
+  const int ncb = ngroups / 4, per = gridDim.x / kXcds;
+  const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (v >= ncb * nsplit) return;
+  const int cbi = order[v].x, split = order[v].y;
+  const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
+  const int s0 = (int)(cnt * split / nsplit);
+  const int* bl = blist + l0 + s0;
+  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
+  // first of the 4 points of step t (this wave's; points p4 .. p4 + 3)
+  auto point4 = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave; };
+  double* my = ring + wave * SLOT * D;
+  // copy lane roles: A pair k = 64 i + lane -> point k / 32, columns 2 (k % 32), +1
+  const int ag = lane >> 5, acol = 16 * I0 + 2 * (lane & 31);
+  const int bg = NJ == 2 ? lane >> 4 : lane >> 5, bcol = 16 * J0 + 2 * (NJ == 2 ? (lane & 15) : (lane & 31));
+  auto copy = [&](int t, int slot) {
+    const long p4 = point4(t);
+    double* dst = my + slot * SLOT;
+    typedef __attribute__((address_space(3))) void* lptr;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + 2 * i + ag) * ldphi + acol),
+                                       (lptr)(dst + i * 128), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < NJ / 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + (NJ == 2 ? 0 : 2 * i) + bg) * ldphi + bcol),
+                                       (lptr)(dst + SA + i * 128), 16, 0, 0);
+    // weights: lanes 0, 1 u0 of points 0-1, 2-3; lanes 2, 3 u1 (unused with one output)
+    if (lane < 4) {
+      const double* src = (lane < 2 ? u0 : u1) + p4 + 2 * (lane & 1);
+      __builtin_amdgcn_global_load_lds((const void*)src, (lptr)(dst + SA + SB), 16, 0, 0);
+    }
+  };
+  dbl4 acc[NOUT][4][NJ];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) acc[o][r][t] = dbl4{0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < D - 1; ++k)
+    if (k < nst) copy(k, k);
+  for (int st = 0; st < nst; ++st) {
+    if (st + D - 1 < nst) {
+      copy(st + D - 1, (st + D - 1) % D);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NI * (D - 1)));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    const double* cur = my + (st % D) * SLOT;
+    double ca[4], cb[NJ];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ca[r] = cur[g * 64 + 16 * r + c];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) cb[t] = cur[SA + g * 16 * NJ + 16 * t + c];
+    const double cw0 = cur[SA + SB + g], cw1 = NOUT == 2 ? cur[SA + SB + 4 + g] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const double b0 = cw0 * cb[t], b1 = cw1 * cb[t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < nI && t < nJ && I0 + r <= J0 + t) {
+          acc[0][r][t] = mfma16x16x4(ca[r], b0, acc[0][r][t]);
+          if (NOUT == 2) acc[NOUT - 1][r][t] = mfma16x16x4(ca[r], b1, acc[NOUT - 1][r][t]);
+        }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    for (int wv = 0; wv < 4; ++wv) {
+      __syncthreads();
+      if (wave == wv) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int t = 0; t < NJ; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              // register q of lane (c, g) = row 4q + g
+              double* x = red + (r * NJ + t) * 256 + q * 64 + lane;
+              *x = (wv == 0) ? acc[o][r][t][q] : *x + acc[o][r][t][q];
+            }
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
+      const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
+      part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[e];
+    }
+  }
+}
+
 // Contraction for small bases (NT = ceil(nbf/16) <= 5 tiles, dense): a wave
 // accumulates all NT(NT+1)/2 upper tiles of one output, so each amplitude is
 // read from HBM once per build and serves as both operands.  Work-group s
@@ -2829,6 +2952,16 @@ void launch_contract2d(dg_handle* h, const T* phi, const double* u0, const doubl
   constexpr int NJ = NOUT == 2 ? 2 : 4;
   const TileBlocks& t = h->tb[NOUT == 2 ? 0 : 1];
   const dim3 grid(round_up((long)t.ncb * t.nsplit, kXcds));
+  if constexpr (sizeof(T) == 8) {
+    constexpr int D = NJ == 2 ? 4 : 3;  // ring slots: 3 work-groups per CU within 160 KB of LDS
+    if (h->screen == 0.0)
+      hipLaunchKernelGGL((k_contract2d_lds<NOUT, NJ, D, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1,
+                         t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+    else
+      hipLaunchKernelGGL((k_contract2d_lds<NOUT, NJ, D, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0,
+                         u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+    return;
+  }
   if (h->screen == 0.0)
     hipLaunchKernelGGL((k_contract2d<T, NOUT, NJ, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, t.cb,
                        t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
