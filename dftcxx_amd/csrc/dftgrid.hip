@@ -257,22 +257,26 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // 4(b+1) MFMAs in two accumulation chains (steps j even / odd), then dots it
 // with phi[p][16b+4v+g], which is its own B-operand register f[4b+v] (the tile
 // loop is unrolled, so every register index is a constant).  The P fragments
-// of one tile are staged in LDS and shared by the work-group's waves,
-// double-buffered: the next live tile's fragments are copied from global
-// memory straight into the other buffer (global_load_lds, 1 KB per wave
-// instruction) while the current tile's MFMAs run; one wait and barrier per tile.
-// NT = ceil(nbf/16).  W = 8 waves (128 points) share the fragments, two per
-// SIMD: one issues MFMAs while the other waits on LDS or the tile barrier.
+// of one tile are staged in LDS and shared by the work-group's W = 4 waves
+// (64 points), copied from global memory straight into LDS (global_load_lds,
+// 1 KB per wave instruction).  NT = ceil(nbf/16).
+// NBUF = 1 (NT <= 25, 400 functions): one tile buffer (<= 50 KB), so that two
+// work-groups share a CU, one wave of each per SIMD: while one work-group
+// copies a tile, waits at its barriers or loads its amplitude rows, the other
+// issues MFMAs.  NBUF = 2 (larger bases, whose amplitude registers leave room for
+// one wave per SIMD only): the next live tile is copied into the other buffer
+// while the current tile's MFMAs run.
 // With screening, mask (one word per 64-point block, bit T for basis tile T)
 // drops the tiles whose amplitudes are negligible on the whole block: a wave
 // skips the MFMAs of the tiles off its block's word, and the work-group walks
-// (and fetches) the tiles on either block's word, so every branch on the walk
-// is uniform over the work-group.
-template <int NT, int W>
-__global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ phi, int ldphi,
-                                                    const double* __restrict__ frag, double* rho,
-                                                    const unsigned* __restrict__ mask) {
-  __shared__ double pl[2][4 * NT * 64];
+// (and fetches) the tiles on the block's word, so every branch on the walk is
+// uniform over the work-group.
+template <int NT, int W, int NBUF>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(NBUF == 1 ? 2 : 1))) void k_density(
+    const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* rho,
+    const unsigned* __restrict__ mask) {
+  static_assert(W % 4 == 0 && (NBUF == 1 || NBUF == 2), "whole 64-point blocks; one or two tile buffers");
+  __shared__ double pl[NBUF][4 * NT * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long p0 = (long)blockIdx.x * 16 * W + wave * 16;
@@ -300,19 +304,25 @@ __global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ p
   };
   double acc = 0.0;
   int buf = 0;
-  {
+  if (NBUF == 2) {
     const int b0 = next_live(-1);
     if (b0 < NT) fetch(b0, pl[0]);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
   // tiles in compile-time order (each skipped when off the mask), so that the
   // amplitudes of the final dot are the B-operand registers f[4b + v]
 #pragma unroll
   for (int b = 0; b < NT; ++b) {
     if (!((mk >> b) & 1u)) continue;
-    const int nb = next_live(b);
-    if (nb < NT) fetch(nb, pl[buf ^ 1]);
+    if (NBUF == 2) {
+      const int nb = next_live(b);
+      if (nb < NT) fetch(nb, pl[buf ^ 1]);
+    } else {
+      fetch(b, pl[0]);
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+      __syncthreads();
+    }
     const double* cur = pl[buf];
     // two accumulation chains (steps j even / odd) over the reduction tiles k <= b
     dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
@@ -332,9 +342,13 @@ __global__ __launch_bounds__(64 * W) void k_density(const double* __restrict__ p
     if ((mine >> b) & 1u)
       acc += (d0[0] + d1[0]) * f[4 * b] + (d0[1] + d1[1]) * f[4 * b + 1] + (d0[2] + d1[2]) * f[4 * b + 2] +
              (d0[3] + d1[3]) * f[4 * b + 3];
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
-    __syncthreads();
-    buf ^= 1;
+    if (NBUF == 2) {
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the other buffer have landed
+      __syncthreads();
+      buf ^= 1;
+    } else {
+      __syncthreads();  // every wave is done with the buffer before the next copy
+    }
   }
   acc += __shfl_xor(acc, 16);
   acc += __shfl_xor(acc, 32);
@@ -2614,10 +2628,11 @@ int sort_items(dg_handle* h) {
 
 template <int NT>
 void launch_density_nt(dg_handle* h) {
-  // two waves per SIMD while the amplitude rows (8 NT registers) leave room for
-  // both (NT <= 25: 400 functions); npts_pad is a multiple of 256
-  constexpr int W = NT <= 25 ? 8 : 4;
-  hipLaunchKernelGGL((k_density<NT, W>), dim3(nblocks(h->npts_pad, 16 * W)), dim3(64 * W), 0, h->stream, h->phi,
+  // one tile buffer and two work-groups per CU while the amplitude rows (8 NT
+  // registers) leave room for two waves per SIMD (NT <= 25: 400 functions);
+  // npts_pad is a multiple of 256
+  constexpr int NBUF = NT <= 25 ? 1 : 2;
+  hipLaunchKernelGGL((k_density<NT, 4, NBUF>), dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
                      h->ldphi, h->pfrag, h->rho, h->bmask);
 }
 
