@@ -257,20 +257,19 @@ __global__ void k_pfrag(const double* __restrict__ P, int nbf, int nt, double* f
 // 4(b+1) MFMAs in two accumulation chains (steps j even / odd), then dots it
 // with phi[p][16b+4v+g], which is its own B-operand register f[4b+v] (the tile
 // loop is unrolled, so every register index is a constant).  The P fragments
-// of one tile are staged in LDS and shared by the work-group's W = 4 waves
-// (64 points), copied from global memory straight into LDS (global_load_lds,
+// of one tile are staged in LDS and shared by the work-group's W waves
+// (16 W points), copied from global memory straight into LDS (global_load_lds,
 // 1 KB per wave instruction).  NT = ceil(nbf/16).
-// NBUF = 1 (NT <= 25, 400 functions): one tile buffer (<= 50 KB), so that two
-// work-groups share a CU, one wave of each per SIMD: while one work-group
-// copies a tile, waits at its barriers or loads its amplitude rows, the other
-// issues MFMAs.  NBUF = 2 (larger bases, whose amplitude registers leave room for
-// one wave per SIMD only): the next live tile is copied into the other buffer
-// while the current tile's MFMAs run.
+// NBUF = 1: one tile buffer (<= 50 KB at NT <= 25), so that two work-groups of
+// W = 4 share a CU, one wave of each per SIMD: while one work-group copies a
+// tile, waits at its barriers or loads its amplitude rows, the other issues
+// MFMAs.  NBUF = 2: the next live tile is copied into the other buffer while the
+// current tile's MFMAs run.
 // With screening, mask (one word per 64-point block, bit T for basis tile T)
 // drops the tiles whose amplitudes are negligible on the whole block: a wave
 // skips the MFMAs of the tiles off its block's word, and the work-group walks
-// (and fetches) the tiles on the block's word, so every branch on the walk is
-// uniform over the work-group.
+// (and fetches) the tiles on any of its blocks' words, so every branch on the
+// walk is uniform over the work-group.
 template <int NT, int W, int NBUF>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(NBUF == 1 ? 2 : 1))) void k_density(
     const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* rho,
@@ -2628,12 +2627,15 @@ int sort_items(dg_handle* h) {
 
 template <int NT>
 void launch_density_nt(dg_handle* h) {
-  // one tile buffer and two work-groups per CU while the amplitude rows (8 NT
-  // registers) leave room for two waves per SIMD (NT <= 25: 400 functions);
-  // npts_pad is a multiple of 256
-  constexpr int NBUF = NT <= 25 ? 1 : 2;
-  hipLaunchKernelGGL((k_density<NT, 4, NBUF>), dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
-                     h->ldphi, h->pfrag, h->rho, h->bmask);
+  // small bases (NT <= 12): eight waves share double-buffered tiles (the tiles
+  // are short, so the copy of the next one is what to hide); up to 400 functions:
+  // one tile buffer and two work-groups per CU (the amplitude rows, 8 NT
+  // registers, leave room for two waves per SIMD); beyond, double-buffered with
+  // one wave per SIMD.  npts_pad is a multiple of 256.
+  constexpr int W = NT <= 12 ? 8 : 4;
+  constexpr int NBUF = (NT > 12 && NT <= 25) ? 1 : 2;
+  hipLaunchKernelGGL((k_density<NT, W, NBUF>), dim3(nblocks(h->npts_pad, 16 * W)), dim3(64 * W), 0, h->stream,
+                     h->phi, h->ldphi, h->pfrag, h->rho, h->bmask);
 }
 
 template <int NT>
