@@ -1257,13 +1257,9 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // 4 NJ NOUT MFMAs: NJ = 2 with two outputs (J and XC), NJ = 4 with one.  The
 // block's 64-point blocks (its list: all, or with screening those where its
 // tiles are significant) are split evenly over nsplit work-groups (split-K);
-// the 4 waves interleave the 16 points of each block, the next step's operands
-// prefetched.  The 4 waves are added in LDS in wave order and written as 4 reduction groups
-// (I0 + r, J0, nJ) in the k_contract_reduce layout.  T = float (reduced
-// precision): FP32 amplitudes and u Phi on the FP32 16x16x4 MFMA, whose sums
-// are added, wave by wave, into the work-group's FP64 accumulators in LDS every
-// 16 steps (64 points per wave; no FP64 copy in registers, which keeps four
-// waves per SIMD); its register v holds row 4g + v and is stored in the FP64 order.
+// the 4 waves interleave the 16 points of each block.  The 4 waves are added in
+// LDS in wave order and written as 4 reduction groups (I0 + r, J0, nJ) in the
+// k_contract_reduce layout.
 // Work-group order: the hardware deals work-groups round-robin to the kXcds
 // XCDs, each with its own L2.  Work-group L (1-D grid, padded to a multiple of
 // kXcds) takes virtual id v = (L % kXcds) per + L / kXcds and order[v] = {tile
@@ -1271,163 +1267,20 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // whole splits, and within an XCD's range the blocks with the most MFMAs come
 // first, so that the short diagonal blocks fill the last round.
 constexpr int kXcds = 8;
-template <typename T, int NOUT, int NJ, bool DENSE>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_contract2d(const T* __restrict__ phi, int ldphi,
-                                                    const double* __restrict__ u0,
-                                                    const double* __restrict__ u1,
-                                                    const int* __restrict__ cblocks,
-                                                    const int* __restrict__ blist,
-                                                    const int* __restrict__ loff,
-                                                    const int2* __restrict__ order, int ngroups, int nsplit,
-                                                    double* part) {
-  constexpr bool f32 = sizeof(T) == 4;
-  __shared__ double red[(f32 ? NOUT : 1) * 4 * NJ * 256];
-  const int ncb = ngroups / 4, per = gridDim.x / kXcds;
-  const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-  if (v >= ncb * nsplit) return;
-  const int cbi = order[v].x, split = order[v].y;
-  const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, c = lane & 15;
-  const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
-  const int s0 = (int)(cnt * split / nsplit);  // the split's first list entry
-  const int* bl = blist + l0 + s0;
-  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
-  // DENSE: every block's list is all 64-point blocks, entry i = block i
-  auto point = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave + g; };
-  const int ia = 16 * I0 + c, jb = 16 * J0 + c;
-  typedef typename std::conditional<f32, flt4, dbl4>::type accT;
-  accT acc[NOUT][4][NJ];
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int t = 0; t < NJ; ++t) acc[o][r][t] = accT{0, 0, 0, 0};
-  if constexpr (f32) {
-    for (int e = threadIdx.x; e < NOUT * 4 * NJ * 256; e += 256) red[e] = 0.0;
-    __syncthreads();
-  }
-  // operands of the next PF steps in flight
-  constexpr int PF = 1;
-  T a[PF][4], bb[PF][NJ], w0[PF], w1[PF];
-  auto load = [&](int k, long p) {
-    const T* row = phi + p * ldphi;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) a[k][r] = row[ia + 16 * r];  // tiles r >= nI, t >= nJ: read, never used
-#pragma unroll
-    for (int t = 0; t < NJ; ++t) bb[k][t] = row[jb + 16 * t];
-    w0[k] = (T)u0[p];
-    w1[k] = (NOUT == 2) ? (T)u1[p] : (T)0;
-  };
-#pragma unroll
-  for (int k = 0; k < PF; ++k)
-    if (k < nst) load(k, point(k));
-  for (int st = 0; st < nst; ++st) {
-    T ca[4], cb[NJ];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ca[r] = a[0][r];
-#pragma unroll
-    for (int t = 0; t < NJ; ++t) cb[t] = bb[0][t];
-    const T cw0 = w0[0], cw1 = w1[0];
-#pragma unroll
-    for (int k = 0; k + 1 < PF; ++k) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[k][r] = a[k + 1][r];
-#pragma unroll
-      for (int t = 0; t < NJ; ++t) bb[k][t] = bb[k + 1][t];
-      w0[k] = w0[k + 1];
-      w1[k] = w1[k + 1];
-    }
-    if (st + PF < nst) load(PF - 1, point(st + PF));
-    auto mma = [&](int r, int t, T b0, T b1) {
-      if constexpr (f32) {
-        acc[0][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b0, acc[0][r][t], 0, 0, 0);
-        if (NOUT == 2)
-          acc[NOUT - 1][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b1, acc[NOUT - 1][r][t], 0, 0, 0);
-      } else {
-        acc[0][r][t] = mfma16x16x4(ca[r], b0, acc[0][r][t]);
-        if (NOUT == 2) acc[NOUT - 1][r][t] = mfma16x16x4(ca[r], b1, acc[NOUT - 1][r][t]);
-      }
-    };
-#pragma unroll
-    for (int t = 0; t < NJ; ++t) {
-      const T b0 = cw0 * cb[t], b1 = cw1 * cb[t];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (r < nI && t < nJ && I0 + r <= J0 + t) mma(r, t, b0, b1);
-    }
-    if constexpr (f32) {
-      if ((st & 15) == 15 || st + 1 == nst) {  // the same steps in every wave
-        for (int wv = 0; wv < 4; ++wv) {
-          __syncthreads();
-          if (wave == wv) {
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int t = 0; t < NJ; ++t) {
-#pragma unroll
-                  for (int v = 0; v < 4; ++v)
-                    red[((o * 4 + r) * NJ + t) * 256 + g * 64 + c + 16 * v] += (double)acc[o][r][t][v];
-                  acc[o][r][t] = accT{0, 0, 0, 0};
-                }
-          }
-        }
-      }
-    }
-  }
-  if constexpr (f32) {
-    __syncthreads();
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o)
-      for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
-        const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
-        part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[o * 4 * NJ * 256 + e];
-      }
-    return;
-  }
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o) {
-    for (int wv = 0; wv < 4; ++wv) {
-      __syncthreads();
-      if (wave == wv) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int t = 0; t < NJ; ++t)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              // register v of lane (c, g) = row 4v + g
-              double* q = red + (r * NJ + t) * 256 + v * 64 + lane;
-              const double x = (double)acc[o][r][t][v];
-              *q = (wv == 0) ? x : *q + x;
-            }
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
-      const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
-      part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[e];
-    }
-  }
-}
 
 // s_waitcnt immediate that waits until at most n of this wave's vector-memory
 // operations are outstanding (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
 // expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
 constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 
-// The FP64 k_contract2d with its operands staged through LDS: each wave copies
+// The FP64 2-D contraction, its operands staged through LDS: each wave copies
 // the amplitude rows and weights of step st + D - 1 into slot (st + D - 1) % D
 // of its own ring (global_load_lds, 16 bytes per lane, so the copies hold no
 // registers) while the MFMAs of step st read slot st % D; a step's copy is
 // NI vector-memory instructions, so waiting for vmcnt <= NI (D - 1) leaves the
 // later steps in flight.  Slot layout (doubles): the 4 points' rows of tiles
 // I0..I0+3 (point-major, 64 each), of tiles J0..J0+NJ-1 (16 NJ each), then
-// u0 and u1 of the 4 points.  Same blocks, order, splits and partial layout as
-// k_contract2d; the epilogue's LDS reduction reuses the ring.
+// u0 and u1 of the 4 points.  The epilogue's LDS reduction reuses the ring.
 template <int NOUT, int NJ, int D, bool DENSE>
 __global__ __launch_bounds__(256, 3) void k_contract2d_lds(const double* __restrict__ phi, int ldphi,
                                                           const double* __restrict__ u0,
@@ -1526,6 +1379,141 @@ __global__ __launch_bounds__(256, 3) void k_contract2d_lds(const double* __restr
               // register q of lane (c, g) = row 4q + g
               double* x = red + (r * NJ + t) * 256 + q * 64 + lane;
               *x = (wv == 0) ? acc[o][r][t][q] : *x + acc[o][r][t][q];
+            }
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4 * NJ * 256; e += 256) {
+      const int r = e / (NJ * 256), t = (e >> 8) % NJ, rem = e & 255;
+      part[(((long)split * ngroups + 4 * cbi + r) * NOUT + o) * 1024 + t * 256 + rem] = red[e];
+    }
+  }
+}
+
+// Reduced-precision 2-D contraction: FP32 amplitudes on the FP32 16x16x4 MFMA
+// (register v of lane (c, g) holds row 4g + v), staged through the same kind of
+// per-wave LDS ring (slot: the 4 points' FP32 rows of tiles I0..I0+3, of tiles
+// J0..J0+NJ-1, then u0 and u1 of the 4 points in FP64; three copy instructions per
+// step).  Every 16 steps (64 points) a wave adds its FP32 sums into FP64 sums it
+// keeps in registers, so the waves never wait for each other before the epilogue;
+// the FP64 sums (64 doubles per lane) leave room for two waves per SIMD, and the
+// ring, with two work-groups per CU, is D steps deep.  The weights are rounded to
+// FP32 and multiply the B operand in FP32.
+template <int NOUT, int NJ, int D, bool DENSE>
+__global__ __launch_bounds__(256, 2) void k_contract2d_lds_f32(const float* __restrict__ phi, int ldphi,
+                                                              const double* __restrict__ u0,
+                                                              const double* __restrict__ u1,
+                                                              const int* __restrict__ cblocks,
+                                                              const int* __restrict__ blist,
+                                                              const int* __restrict__ loff,
+                                                              const int2* __restrict__ order, int ngroups,
+                                                              int nsplit, double* part) {
+  constexpr int SA = 256, SB = 64 * NJ;            // floats per slot: A and B rows
+  constexpr int SLOT = (SA + SB) * 4 + 8 * 8;      // bytes per slot (+ 8 FP64 weights)
+  constexpr int NI = 3;                            // copy instructions per step
+  static_assert(4 * SLOT * D >= 4 * NJ * 256 * 8, "the epilogue reduction fits in the ring");
+  __shared__ __attribute__((aligned(16))) char ring[4 * SLOT * D];
+  double* red = reinterpret_cast<double*>(ring);
+  const int ncb = ngroups / 4, per = gridDim.x / kXcds;
+  const int v = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (v >= ncb * nsplit) return;
+  const int cbi = order[v].x, split = order[v].y;
+  const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
+  const int s0 = (int)(cnt * split / nsplit);
+  const int* bl = blist + l0 + s0;
+  const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
+  auto point4 = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave; };
+  char* my = ring + wave * SLOT * D;
+  // copy lane roles: A: point lane / 16, 4 columns from 16 I0 + 4 (lane % 16);
+  // B (the first 16 NJ lanes): point lane / (4 NJ), 4 columns from 16 J0 + 4 (lane % 4 NJ)
+  const int ag = lane >> 4, acol = 16 * I0 + 4 * (lane & 15);
+  const int bg = lane / (4 * NJ), bcol = 16 * J0 + 4 * (lane % (4 * NJ));
+  auto copy = [&](int t, int slot) {
+    const long p4 = point4(t);
+    char* dst = my + slot * SLOT;
+    typedef __attribute__((address_space(3))) void* lptr;
+    __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + ag) * ldphi + acol), (lptr)dst, 16, 0, 0);
+    if (lane < 16 * NJ)
+      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + bg) * ldphi + bcol), (lptr)(dst + SA * 4), 16, 0,
+                                       0);
+    // weights: lanes 0, 1 u0 of points 0-1, 2-3; lanes 2, 3 u1 (unused with one output)
+    if (lane < 4) {
+      const double* src = (lane < 2 ? u0 : u1) + p4 + 2 * (lane & 1);
+      __builtin_amdgcn_global_load_lds((const void*)src, (lptr)(dst + (SA + SB) * 4), 16, 0, 0);
+    }
+  };
+  flt4 acc[NOUT][4][NJ];
+  double sum[NOUT][4][NJ][4];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) {
+        acc[o][r][t] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum[o][r][t][q] = 0.0;
+      }
+#pragma unroll
+  for (int k = 0; k < D - 1; ++k)
+    if (k < nst) copy(k, k);
+  for (int st = 0; st < nst; ++st) {
+    if (st + D - 1 < nst) {
+      copy(st + D - 1, (st + D - 1) % D);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(NI * (D - 1)));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    const char* cur = my + (st % D) * SLOT;
+    const float* cf = reinterpret_cast<const float*>(cur);
+    const double* cwp = reinterpret_cast<const double*>(cur + (SA + SB) * 4);
+    float ca[4], cb[NJ];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ca[r] = cf[g * 64 + 16 * r + c];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) cb[t] = cf[SA + g * 16 * NJ + 16 * t + c];
+    const float cw0 = (float)cwp[g], cw1 = NOUT == 2 ? (float)cwp[4 + g] : 0.f;
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const float b0 = cw0 * cb[t], b1 = cw1 * cb[t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < nI && t < nJ && I0 + r <= J0 + t) {
+          acc[0][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b0, acc[0][r][t], 0, 0, 0);
+          if (NOUT == 2)
+            acc[NOUT - 1][r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[r], b1, acc[NOUT - 1][r][t], 0, 0, 0);
+        }
+    }
+    if ((st & 15) == 15 || st + 1 == nst) {
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int t = 0; t < NJ; ++t) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sum[o][r][t][q] += (double)acc[o][r][t][q];
+            acc[o][r][t] = flt4{0.f, 0.f, 0.f, 0.f};
+          }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    for (int wv = 0; wv < 4; ++wv) {
+      __syncthreads();
+      if (wave == wv) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int t = 0; t < NJ; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              // register q of lane (c, g) = row 4g + q
+              double* x = red + (r * NJ + t) * 256 + g * 64 + 16 * q + c;
+              *x = (wv == 0) ? sum[o][r][t][q] : *x + sum[o][r][t][q];
             }
       }
     }
@@ -2440,7 +2428,7 @@ struct ProfStat {
   int n = 0;
 };
 
-// Tile blocks of k_contract2d for one block width NJ: {I0, nI, J0, nJ} per
+// Tile blocks of the 2-D contraction for one block width NJ: {I0, nI, J0, nJ} per
 // block, its four reduction groups (I0 + r, J0, nJ) in the k_contract_reduce
 // layout, the 64-point block list of each block (all blocks, or with screening
 // those where its tiles are significant), and per virtual work-group id the
@@ -2459,7 +2447,7 @@ struct dg_handle {
   long npts = 0, npts_pad = 0, cell = 0;
   int ldphi = 0, ntb = 0, ngroups = 0, nsplit_small = 0;
   int cus = 256;  // compute units of the device
-  // tile blocks of k_contract2d: [0] 4 x 2 (two outputs), [1] 4 x 4 (one output)
+  // tile blocks of the 2-D contraction: [0] 4 x 2 (two outputs), [1] 4 x 4 (one output)
   TileBlocks tb[2];
   long nlda_blocks = 0;
   double zsum = 0.0, xfac = 0.0;
@@ -2977,14 +2965,15 @@ void launch_contract2d(dg_handle* h, const T* phi, const double* u0, const doubl
     else
       hipLaunchKernelGGL((k_contract2d_lds<NOUT, NJ, D, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0,
                          u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
-    return;
+  } else {
+    constexpr int D = NJ == 2 ? 12 : 9;  // ring slots: 2 work-groups per CU within 160 KB of LDS
+    if (h->screen == 0.0)
+      hipLaunchKernelGGL((k_contract2d_lds_f32<NOUT, NJ, D, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi,
+                         u0, u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+    else
+      hipLaunchKernelGGL((k_contract2d_lds_f32<NOUT, NJ, D, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi,
+                         u0, u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
   }
-  if (h->screen == 0.0)
-    hipLaunchKernelGGL((k_contract2d<T, NOUT, NJ, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1, t.cb,
-                       t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
-  else
-    hipLaunchKernelGGL((k_contract2d<T, NOUT, NJ, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1,
-                       t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
 }
 
 // sa (optional): the build's scalars, evaluated by one more work-group of the reduction
@@ -3449,15 +3438,15 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds, 1)));
     h->cross_wgs = std::max(1, std::min(h->nitems, cus * std::min(per_cu, 2)));
     // 2-D contraction: splits per tile block such that the work-groups make about
-    // three rounds of the resident slots (3 FP64 / 4 FP32 work-groups per CU); with
+    // three rounds of the resident slots (3 FP64 / 2 FP32 work-groups per CU); with
     // the longest blocks first, the short ones even out the last round (C30H62: one
     // round 14.3 ms, two 12.8, three 12.35, four 12.4)
-    const long slots = (long)cus * (h->precision == DG_PREC_MIXED ? 4 : 3);
+    const long slots = (long)cus * (h->precision == DG_PREC_MIXED ? 2 : 3);
     for (int q = 0; q < 2; ++q) {
       TileBlocks& t = h->tb[q];
       const std::vector<int>& c = cbs[q];
       t.nsplit = (int)std::max(1L, std::min(std::min(512L, 3 * slots / std::max(t.ncb, 1)), h->npts_pad / 64));
-      // virtual work-group order of k_contract2d: split-major, each XCD's range
+      // virtual work-group order of the 2-D contraction: split-major, each XCD's range
       // re-ordered by decreasing tile pairs of the block (stable)
       const long nv = (long)t.ncb * t.nsplit, per = round_up(nv, kXcds) / kXcds;
       std::vector<int> work(std::max(t.ncb, 1), 0);
@@ -3484,7 +3473,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
   DG_TRY(dalloc(&h->leb, 4 * (size_t)h->nang));
   for (double** b : {&h->px, &h->py, &h->pz, &h->w, &h->wb, &h->rho, &h->uxc, &h->exd, &h->uJ, &h->V, &h->vcell})
     DG_TRY(dalloc(b, np));
-  // one row of slack: k_contract2d reads whole 4 x NJ tile blocks, whose tiles
+  // one row of slack: the 2-D contraction reads whole 4 x NJ tile blocks, whose tiles
   // past the last function tile reach up to 48 values beyond a row
   DG_TRY(dalloc(&h->phi, (size_t)(np + 1) * h->ldphi));
   DG_TRY(dalloc(&h->sh_atom, h->nshell));
