@@ -48,6 +48,7 @@ class DeviceStep {
   double* host_ = nullptr;  // pinned: energy terms [0..4], SP2 control block from [8]
   double *H_ = nullptr, *X_ = nullptr, *F_ = nullptr, *Fp_ = nullptr, *Pn_ = nullptr, *P2_ = nullptr;
   double *P_ = nullptr, *J_ = nullptr, *XC_ = nullptr, *scal_ = nullptr, *eout_ = nullptr;
+  double *epart_ = nullptr, *glo_ = nullptr, *ghi_ = nullptr;  // energy partial sums, Gershgorin bounds
 };
 
 // The same step for every molecule of a batch (each at most 32 functions), one

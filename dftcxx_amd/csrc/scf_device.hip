@@ -38,42 +38,62 @@ struct Sp2Ctl {
 };
 
 // C = alpha op(A) op(B) + beta D (D may be NULL), column-major, M x N with inner
-// dimension K.  A work-group computes a 32 x 32 block of C, each of its four
-// waves a 16 x 16 tile on the FP64 16x16x4 MFMA: lane (g, c) supplies
+// dimension K.  A work-group computes one 16 x 16 tile of C on the FP64 16x16x4
+// MFMA, its four waves a quarter of the K range each: lane (g, c) supplies
 // op(A)(i0 + c, k + g) and op(B)(k + g, j0 + c), and result register v holds
-// C(i0 + 4v + g, j0 + c).  TA reads A transposed (A(k, i)), TB reads B
+// C(i0 + 4v + g, j0 + c).  A wave issues the loads of up to kGemmBatch 16-wide
+// K blocks before its first MFMA (at N <= 512 all of its range), so that it
+// waits for memory once, not once per block; the four partial tiles are added
+// in wave order through LDS.  TA reads A transposed (A(k, i)), TB reads B
 // transposed (B(j, k)); for a symmetric B, TB = true gives the same product
 // with lane-contiguous loads.  The launch returns at once when ctl->done.
+constexpr int kGemmBatch = 8;
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
                                               int lda, const double* __restrict__ B, int ldb, double beta,
                                               const double* __restrict__ D, double* __restrict__ C, int ldc,
                                               const Sp2Ctl* __restrict__ ctl) {
   if (ctl && ctl->done) return;
+  __shared__ double part[3][256];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const int i0 = blockIdx.x * 32 + (wave & 1) * 16, j0 = blockIdx.y * 32 + (wave >> 1) * 16;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
   const int ia = i0 + c, jb = j0 + c;
   const bool arow = ia < M, bcol = jb < N;
+  const int nkb = (K + 15) / 16;
+  const int kb0 = wave * nkb / 4, kb1 = (wave + 1) * nkb / 4;
   dbl4 d = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    double a[4], b[4];
+  for (int kb = kb0; kb < kb1; kb += kGemmBatch) {
+    double a[kGemmBatch][4], b[kGemmBatch][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 4 * u + g;
-      a[u] = (arow && k < K) ? (TA ? A[k + (long)ia * lda] : A[ia + (long)k * lda]) : 0.0;
-      b[u] = (bcol && k < K) ? (TB ? B[jb + (long)k * ldb] : B[k + (long)jb * ldb]) : 0.0;
-    }
+    for (int q = 0; q < kGemmBatch; ++q)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) d = mfma_f64(a[u], b[u], d);
+      for (int u = 0; u < 4; ++u) {
+        const int k = 16 * (kb + q) + 4 * u + g;
+        const bool in = kb + q < kb1 && k < K;
+        a[q][u] = (in && arow) ? (TA ? A[k + (long)ia * lda] : A[ia + (long)k * lda]) : 0.0;
+        b[q][u] = (in && bcol) ? (TB ? B[jb + (long)k * ldb] : B[k + (long)jb * ldb]) : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < kGemmBatch; ++q)
+      if (kb + q < kb1) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d = mfma_f64(a[q][u], b[q][u], d);
+      }
   }
-  if (!bcol) return;
+  if (wave > 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[wave - 1][v * 64 + lane] = d[v];
+  }
+  __syncthreads();
+  if (wave != 0 || !bcol) return;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int row = i0 + 4 * v + g;
     if (row >= M) continue;
+    const double x = ((d[v] + part[0][v * 64 + lane]) + part[1][v * 64 + lane]) + part[2][v * 64 + lane];
     const long e = row + (long)jb * ldc;
-    C[e] = alpha * d[v] + (D ? beta * D[e] : 0.0);
+    C[e] = alpha * x + (D ? beta * D[e] : 0.0);
   }
 }
 
@@ -85,21 +105,34 @@ __global__ void k_fock(const double* __restrict__ H, const double* __restrict__ 
   if (e < nn) F[e] = H[e] + (J ? 2.0 * J[e] : 0.0) + (XC ? XC[e] : 0.0);
 }
 
-// Gershgorin bounds of the symmetric F' and P0 = (h_max I - F') / (h_max - h_min);
-// resets the control block.  One work-group.
-__global__ __launch_bounds__(1024) void k_sp2_init(const double* __restrict__ F, int n, double* P, Sp2Ctl* ctl) {
-  __shared__ double lo_s[1024], hi_s[1024];
-  double lo = 1e300, hi = -1e300;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double* col = F + (long)i * n;  // row i of the symmetric F'
-    double r = 0.0;
-    for (int j = 0; j < n; ++j)
-      if (j != i) r += fabs(col[j]);
-    lo = fmin(lo, col[i] - r);
-    hi = fmax(hi, col[i] + r);
+// Gershgorin discs of the symmetric F': row i (its column i, contiguous) gives
+// lo[i] = F'_ii - sum_{j != i} |F'_ij| and hi[i] = F'_ii + the same; one wave per row.
+__global__ __launch_bounds__(256) void k_gershgorin(const double* __restrict__ F, int n, double* lo, double* hi) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* col = F + (long)i * n;
+  double r = 0.0;
+  for (int j = lane; j < n; j += 64)
+    if (j != i) r += fabs(col[j]);
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+  if (lane == 0) {
+    lo[i] = col[i] - r;
+    hi[i] = col[i] + r;
   }
-  lo_s[threadIdx.x] = lo;
-  hi_s[threadIdx.x] = hi;
+}
+
+// h_min = min lo, h_max = max hi (exact, so the order does not matter); resets
+// the control block.  One work-group.
+__global__ __launch_bounds__(1024) void k_sp2_init(const double* __restrict__ lo, const double* __restrict__ hi,
+                                                   int n, Sp2Ctl* ctl) {
+  __shared__ double lo_s[1024], hi_s[1024];
+  double l = 1e300, h = -1e300;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    l = fmin(l, lo[i]);
+    h = fmax(h, hi[i]);
+  }
+  lo_s[threadIdx.x] = l;
+  hi_s[threadIdx.x] = h;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
@@ -108,29 +141,37 @@ __global__ __launch_bounds__(1024) void k_sp2_init(const double* __restrict__ F,
     }
     __syncthreads();
   }
-  const double hmin = lo_s[0], hmax = hi_s[0];
-  const double inv = 1.0 / fmax(hmax - hmin, 1e-300);
-  for (long e = threadIdx.x; e < (long)n * n; e += blockDim.x) {
-    const long i = e % n, j = e / n;
-    P[e] = ((i == j ? hmax : 0.0) - F[e]) * inv;
-  }
   if (threadIdx.x == 0) {
     ctl->done = ctl->iter = ctl->fail = 0;
     ctl->err = ctl->err_prev = 1e300;
-    ctl->hmin = hmin;
-    ctl->hmax = hmax;
+    ctl->hmin = lo_s[0];
+    ctl->hmax = hi_s[0];
   }
 }
 
-// One SP2 decision from tr(P) and tr(P^2) (fixed summation order): the
+// P0 = (h_max I - F') / (h_max - h_min), the linear map of the spectrum of F' into [0, 1].
+__global__ void k_sp2_p0(const double* __restrict__ F, int n, double* P, const Sp2Ctl* __restrict__ ctl) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const double hmin = ctl->hmin, hmax = ctl->hmax;
+  const double inv = 1.0 / fmax(hmax - hmin, 1e-300);
+  const long i = e % n, j = e / n;
+  P[e] = ((i == j ? hmax : 0.0) - F[e]) * inv;
+}
+
+// One SP2 step after P2 = P^2, over the elements of P: every work-group forms
+// tr(P) and tr(P^2) in the same fixed order, so all take the same decision: the
 // projection stops once the idempotency error tr(P) - tr(P^2) falls below tol
 // (the eigenvalues of P are then within about tol of 0 or 1; two McWeeny steps
-// follow, k_mcweeny); it fails after maxit steps.  Else records which update
-// the next kernel applies (ctl->pad: 1 = P^2, 0 = 2P - P^2).  (A test on the
-// error's decrease from one step to the next would not do: a step squares the
+// follow); it fails at step maxit - 1.  Else Pout = P^2 or 2P - P^2, whichever
+// trace lies closer to nocc (Pout is not P, whose diagonal other work-groups
+// may still be reading).  Work-group 0 records the outcome in ctl (iter = the
+// index of the step whose input holds the current P).  (A test on the error's
+// decrease from one step to the next would not do: a step squares the
 // eigenvalue errors on one side of the gap but doubles them on the other.)
-__global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P, const double* __restrict__ P2,
-                                                    int n, int nocc, double tol, int maxit, Sp2Ctl* ctl) {
+__global__ __launch_bounds__(256) void k_sp2_step(const double* __restrict__ P, const double* __restrict__ P2,
+                                                  double* __restrict__ Pout, int n, int nocc, double tol, int step,
+                                                  int maxit, Sp2Ctl* ctl) {
   if (ctl->done) return;
   __shared__ double t1s[256], t2s[256];
   double t1 = 0.0, t2 = 0.0;
@@ -148,23 +189,26 @@ __global__ __launch_bounds__(256) void k_sp2_decide(const double* __restrict__ P
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
   const double tr1 = t1s[0], tr2 = t2s[0];
   const double err = fabs(tr1 - tr2);
-  ctl->err_prev = ctl->err;
-  ctl->err = err;
-  if (err < tol) {
-    ctl->done = 1;
-    return;
-  }
-  if (ctl->iter + 1 >= maxit) {
-    ctl->done = 1;
-    ctl->fail = 1;
-    return;
-  }
-  ctl->iter += 1;
+  const bool stop = err < tol, fail = !stop && step + 1 >= maxit;
   // trace after P -> P^2 is tr2, after P -> 2P - P^2 it is 2 tr1 - tr2
-  ctl->pad = fabs(tr2 - nocc) < fabs(2.0 * tr1 - tr2 - nocc) ? 1 : 0;
+  const bool sq = fabs(tr2 - nocc) < fabs(2.0 * tr1 - tr2 - nocc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->err_prev = ctl->err;
+    ctl->err = err;
+    if (stop || fail) {
+      ctl->done = 1;
+      ctl->fail = fail ? 1 : 0;
+    } else {
+      ctl->iter = step + 1;
+      ctl->pad = sq ? 1 : 0;
+    }
+  }
+  if (stop || fail) return;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  Pout[e] = sq ? P2[e] : 2.0 * P[e] - P2[e];
 }
 
 // dst = src elementwise: the result Q = 3 P^2 - 2 P^2 P of a McWeeny step
@@ -175,13 +219,6 @@ __global__ void k_copy(double* dst, const double* __restrict__ src, long nn) {
   if (e < nn) dst[e] = src[e];
 }
 
-__global__ void k_sp2_apply(double* P, const double* __restrict__ P2, long nn, const Sp2Ctl* __restrict__ ctl) {
-  if (ctl->done) return;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nn) return;
-  P[e] = ctl->pad ? P2[e] : 2.0 * P[e] - P2[e];
-}
-
 // P = Pn on the first step, else (1 - m) Pn + m P (src/dft.cpp:331, 358).
 __global__ void k_mix(double* P, const double* __restrict__ Pn, long nn, double m, int first) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,17 +226,16 @@ __global__ void k_mix(double* P, const double* __restrict__ Pn, long nn, double 
   P[e] = first ? Pn[e] : (1.0 - m) * Pn[e] + m * P[e];
 }
 
-// E = 2 tr(PH) + E_J + E_nuc + E_xc (src/dft.cpp:441-447) with E_J = 2 tr(PJ), or
-// the pointwise value scal[3] (use_point_ej); E_xc = scal[0] when use_exc.
-// out = {E_one, E_J, E_xc, total, N after correction}.  One work-group, fixed order.
-__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ P, const double* __restrict__ H,
-                                                const double* __restrict__ J, int n,
-                                                const double* __restrict__ scal, double enuc, int use_exc,
-                                                int use_point_ej, double* out) {
+// Partial sums of tr(PH) and tr(PJ) = sum_e P_e H_e, sum_e P_e J_e (P, H, J
+// symmetric): work-group b sums its contiguous range of the n^2 elements in a
+// fixed order into part[2b], part[2b + 1].
+constexpr int kEnergyParts = 64;
+__global__ __launch_bounds__(256) void k_energy_part(const double* __restrict__ P, const double* __restrict__ H,
+                                                     const double* __restrict__ J, long nn, double* part) {
   __shared__ double s1[256], s2[256];
+  const long e0 = nn * blockIdx.x / gridDim.x, e1 = nn * (blockIdx.x + 1) / gridDim.x;
   double t1 = 0.0, t2 = 0.0;
-  const long nn = (long)n * n;
-  for (long e = threadIdx.x; e < nn; e += blockDim.x) {
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     t1 += P[e] * H[e];
     t2 += P[e] * J[e];
   }
@@ -214,15 +250,31 @@ __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ P, co
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double e1 = 2.0 * s1[0];
-    const double ej = use_point_ej ? scal[3] : 2.0 * s2[0];
-    const double exc = use_exc ? scal[0] : 0.0;
-    out[0] = e1;
-    out[1] = ej;
-    out[2] = exc;
-    out[3] = e1 + ej + enuc + exc;
-    out[4] = scal[2];
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
   }
+}
+
+// E = 2 tr(PH) + E_J + E_nuc + E_xc (src/dft.cpp:441-447) with E_J = 2 tr(PJ), or
+// the pointwise value scal[3] (use_point_ej); E_xc = scal[0] when use_exc.
+// out = {E_one, E_J, E_xc, total, N after correction}.  The partial sums of
+// k_energy_part are added in order by one thread.
+__global__ void k_energy(const double* __restrict__ part, int nparts, const double* __restrict__ scal, double enuc,
+                         int use_exc, int use_point_ej, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t1 = 0.0, t2 = 0.0;
+  for (int b = 0; b < nparts; ++b) {
+    t1 += part[2 * b];
+    t2 += part[2 * b + 1];
+  }
+  const double e1 = 2.0 * t1;
+  const double ej = use_point_ej ? scal[3] : 2.0 * t2;
+  const double exc = use_exc ? scal[0] : 0.0;
+  out[0] = e1;
+  out[1] = ej;
+  out[2] = exc;
+  out[3] = e1 + ej + enuc + exc;
+  out[4] = scal[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -383,8 +435,10 @@ namespace dftcxx {
 
 DeviceStep::DeviceStep(int n, int nocc, hipStream_t stream) : n_(n), nocc_(nocc), st_(stream) {
   const size_t nn = (size_t)n * n;
-  // H, X, F (also P'), F', Pn, P2 (also X P'), P, J, XC, scalars(4), energy(5)
-  check(hipMalloc((void**)&buf_, sizeof(double) * (9 * nn + 16)), "hipMalloc of the SCF buffers");
+  // H, X, F (also P'), F', Pn, P2 (also X P'), P, J, XC, scalars(4), energy(5 of 8),
+  // energy partial sums (2 kEnergyParts), Gershgorin bounds (2 n)
+  const size_t total = 9 * nn + 12 + 2 * kEnergyParts + 2 * (size_t)n;
+  check(hipMalloc((void**)&buf_, sizeof(double) * total), "hipMalloc of the SCF buffers");
   check(hipMalloc((void**)&ctl_, sizeof(Sp2Ctl)), "hipMalloc of the SCF control block");
   check(hipHostMalloc((void**)&host_, sizeof(double) * 8 + sizeof(Sp2Ctl), hipHostMallocDefault),
         "hipHostMalloc of the SCF staging");
@@ -399,7 +453,10 @@ DeviceStep::DeviceStep(int n, int nocc, hipStream_t stream) : n_(n), nocc_(nocc)
   XC_ = J_ + nn;
   scal_ = XC_ + nn;
   eout_ = scal_ + 4;
-  check(hipMemsetAsync(buf_, 0, sizeof(double) * (9 * nn + 16), st_), "hipMemsetAsync");
+  epart_ = eout_ + 8;
+  glo_ = epart_ + 2 * kEnergyParts;
+  ghi_ = glo_ + n;
+  check(hipMemsetAsync(buf_, 0, sizeof(double) * total, st_), "hipMemsetAsync");
 }
 
 DeviceStep::~DeviceStep() {
@@ -418,7 +475,7 @@ void DeviceStep::upload_core(const double* H, const double* X) {
 
 template <bool TA, bool TB>
 static void gemm(hipStream_t st, int n, const double* A, const double* B, double* C, const Sp2Ctl* ctl) {
-  const dim3 grid(nb(n, 32), nb(n, 32));
+  const dim3 grid(nb(n, 16), nb(n, 16));
   hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, st, n, n, n, 1.0, A, n, B, n, 0.0, nullptr, C, n, ctl);
 }
 
@@ -433,17 +490,22 @@ bool DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
   gemm<true, false>(st_, n_, X_, F_, Pn_, nullptr);
   gemm<false, false>(st_, n_, Pn_, X_, Fp_, nullptr);
   // SP2 on F' into F (as P'), P2 as the square
-  hipLaunchKernelGGL(k_sp2_init, dim3(1), dim3(1024), 0, st_, Fp_, n_, F_, ctl);
+  hipLaunchKernelGGL(k_gershgorin, dim3(nb(n_, 4)), dim3(256), 0, st_, Fp_, n_, glo_, ghi_);
+  hipLaunchKernelGGL(k_sp2_init, dim3(1), dim3(1024), 0, st_, glo_, ghi_, n_, ctl);
+  // the iterates alternate between F and F' (free once P0 is formed): step k
+  // reads pb[k % 2] and writes pb[(k + 1) % 2]
+  hipLaunchKernelGGL(k_sp2_p0, dim3(nb(nn, 256)), dim3(256), 0, st_, Fp_, n_, F_, ctl);
+  double* pb[2] = {F_, Fp_};
   const double tol = 1e-6;
   // steps in groups, the control block read back between groups (a converged
   // projection makes the rest of a group return at once)
   Sp2Ctl* hc = reinterpret_cast<Sp2Ctl*>(host_ + 8);
   for (int it = 0; it < kMaxSteps;) {
     const int group = it == 0 ? 24 : 8;
-    for (int k = 0; k < group; ++k) {
-      gemm<false, true>(st_, n_, F_, F_, P2_, ctl);  // P' symmetric: B read transposed
-      hipLaunchKernelGGL(k_sp2_decide, dim3(1), dim3(256), 0, st_, F_, P2_, n_, nocc_, tol, kMaxSteps, ctl);
-      hipLaunchKernelGGL(k_sp2_apply, dim3(nb(nn, 256)), dim3(256), 0, st_, F_, P2_, nn, ctl);
+    for (int k = it; k < it + group; ++k) {
+      gemm<false, true>(st_, n_, pb[k & 1], pb[k & 1], P2_, ctl);  // P' symmetric: B read transposed
+      hipLaunchKernelGGL(k_sp2_step, dim3(nb(nn, 256)), dim3(256), 0, st_, pb[k & 1], P2_, pb[(k + 1) & 1], n_,
+                         nocc_, tol, k, kMaxSteps, ctl);
     }
     it += group;
     check(hipMemcpyAsync(hc, ctl_, sizeof(Sp2Ctl), hipMemcpyDeviceToHost, st_), "copy of the SP2 state");
@@ -452,10 +514,12 @@ bool DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
   }
   // no gap: P is left as it was, the caller forms this step on the host
   if (!hc->done || hc->fail) return false;
+  // the converged P' is the input of step hc->iter
+  if (hc->iter & 1) hipLaunchKernelGGL(k_copy, dim3(nb(nn, 256)), dim3(256), 0, st_, F_, Fp_, nn);
   // two McWeeny steps: eigenvalue errors 1e-6 -> 1e-12 -> rounding
   for (int k = 0; k < 2; ++k) {
     gemm<false, true>(st_, n_, F_, F_, P2_, nullptr);
-    const dim3 grid(nb(n_, 32), nb(n_, 32));
+    const dim3 grid(nb(n_, 16), nb(n_, 16));
     hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, st_, n_, n_, n_, -2.0, P2_, n_, F_, n_, 3.0, P2_,
                        Fp_, n_, (const Sp2Ctl*)nullptr);
     hipLaunchKernelGGL(k_copy, dim3(nb(nn, 256)), dim3(256), 0, st_, F_, Fp_, nn);
@@ -469,7 +533,8 @@ bool DeviceStep::density(bool with_xc, bool first, double mixing, bool fused) {
 }
 
 void DeviceStep::energy(double enuc, bool with_xc, bool point_ej) {
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, st_, P_, H_, J_, n_, scal_, enuc, with_xc ? 1 : 0,
+  hipLaunchKernelGGL(k_energy_part, dim3(kEnergyParts), dim3(256), 0, st_, P_, H_, J_, (long)n_ * n_, epart_);
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st_, epart_, kEnergyParts, scal_, enuc, with_xc ? 1 : 0,
                      point_ej ? 1 : 0, eout_);
   check(hipGetLastError(), "launch of the energy");
   check(hipMemcpyAsync(host_, eout_, 5 * sizeof(double), hipMemcpyDeviceToHost, st_), "copy of the energy");
