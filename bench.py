@@ -40,10 +40,9 @@ def traffic_key(label, names):
     for k in sorted(names):
         base = k.split("<")[0]
         args = [a.strip() for a in k[k.find("<") + 1:k.rfind(">")].split(",")] if "<" in k else []
-        # two-output (J and XC) variants: contract2d_lds<NOUT, ...> (FP64),
-        # contract2d<T, NOUT, ...>, contract_small<T, NOUT, ...>
+        # two-output (J and XC) FP64 variants: contract2d_lds<NOUT, ...>,
+        # contract_small<NT, NOUT, ...>
         if label == "contract" and ((base == "contract2d_lds" and args[:1] == ["2"]) or
-                                    (base == "contract2d" and args[:2] == ["double", "2"]) or
                                     (base == "contract_small" and len(args) > 1 and args[1] == "2")):
             return k
         if label == "density" and base in ("density", "density_pl"):

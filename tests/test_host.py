@@ -190,6 +190,22 @@ def test_basis_json_matches_basis_files():
                 [(l, list(a), list(c)) for l, a, c in mine[z]]
 
 
+def test_bench_traffic_key_picks_the_fp64_two_output_kernels():
+    # roofline.traffic must come from the kernel the build's timing is for: the
+    # two-output FP64 contraction, not the single-output one the host path runs
+    sys.path.insert(0, ROOT)
+    from bench import traffic_key
+    names = ["contract2d_lds<1, 4, 3, true>", "contract2d_lds<2, 2, 4, true>", "contract2d_lds_f32<2, 2, 12, true>",
+             "density<25, 4, 1>", "hartree_cross<8, 2>", "vreduce"]
+    assert traffic_key("contract", names) == "contract2d_lds<2, 2, 4, true>"
+    assert traffic_key("density", names) == "density<25, 4, 1>"
+    assert traffic_key("hartree_cross", names) == "hartree_cross<8, 2>"
+    small = ["contract_small<5, 1, 2>", "contract_small<5, 2, 2>", "density_pl<5>"]
+    assert traffic_key("contract", small) == "contract_small<5, 2, 2>"
+    assert traffic_key("density", small) == "density_pl<5>"
+    assert traffic_key("contract", ["vreduce"]) is None
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
