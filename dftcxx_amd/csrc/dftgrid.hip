@@ -56,6 +56,11 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// s_waitcnt immediate that waits until at most n of this wave's vector-memory
+// operations are outstanding (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
 // Wave-level sum in a fixed butterfly order (all 64 lanes end with the total).
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -348,6 +353,83 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(NBUF == 
     } else {
       __syncthreads();  // every wave is done with the buffer before the next copy
     }
+  }
+  acc += __shfl_xor(acc, 16);
+  acc += __shfl_xor(acc, 32);
+  if (g == 0) rho[p0 + c] = 2.0 * acc;
+}
+
+// k_density without screening, its P fragments streamed through a ring.  The
+// fragments are stored tile after tile, reduction tile after reduction tile
+// (unit (b, k), k <= b: 4 steps x 64 lanes = 2 KB), which is the order the
+// tiles consume them; the stream is cut into chunks of kDensityChunk units
+// (16 KB, four 1 KB copy instructions per wave, the last chunk reading into the
+// buffer's slack) and each chunk is copied two chunks ahead into one of three
+// LDS slots (48 KB: two work-groups per CU).  One wait and barrier per chunk,
+// which then also tells that every wave is done with the slot the next copy
+// overwrites.  W = 4 waves share the chunks.
+constexpr int kDensityChunk = 8;
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_density_ring(
+    const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* rho) {
+  constexpr int C = kDensityChunk, U = NT * (NT + 1) / 2, NCH = (U + C - 1) / C;
+  __shared__ double ring[3][C * 256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const double* row = phi + (p0 + c) * ldphi;
+  double f[4 * NT];
+#pragma unroll
+  for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  // chunk j: units [C j, C j + C), 16 x 1 KB; wave w copies pieces w, w + 4, w + 8, w + 12
+  auto copy = [&](int j) {
+    const double* src = frag + (long)j * C * 256;
+    double* dst = ring[j % 3];
+#pragma unroll
+    for (int q = 0; q < C * 2 / 4; ++q) {
+      const int piece = wave + 4 * q;
+      __builtin_amdgcn_global_load_lds((const void*)(src + piece * 128 + lane * 2),
+                                       (__attribute__((address_space(3))) void*)(dst + piece * 128), 16, 0, 0);
+    }
+  };
+  copy(0);
+  if (NCH > 1) copy(1);
+  double acc = 0.0;
+  dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0;
+  int u = 0;
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+#pragma unroll
+    for (int k = 0; k <= b; ++k, ++u) {
+      if (u % C == 0) {
+        const int j = u / C;
+        // this wave's copies of chunk j have landed (those of chunk j + 1 may not)
+        if (j + 1 < NCH)
+          __builtin_amdgcn_s_waitcnt(waitcnt_vm(C * 2 / 4));
+        else
+          __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+        // a bare barrier: __syncthreads' release fence would also wait for the
+        // copies of chunk j + 1; the empty asm statements keep the compiler from
+        // moving memory accesses across it (the reads of chunk j - 1 have been
+        // consumed by MFMAs before it, so none is in flight)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (j + 2 < NCH) copy(j + 2);  // into the slot of chunk j - 1, which every wave is done with
+      }
+      if (k == 0) d0 = d1 = dbl4{0.0, 0.0, 0.0, 0.0};
+      const double* cur = ring[(u / C) % 3] + (u % C) * 256;
+      double av[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) av[jj] = cur[jj * 64 + lane];
+      d0 = mfma16x16x4(av[0], f[4 * k], d0);
+      d1 = mfma16x16x4(av[1], f[4 * k + 1], d1);
+      d0 = mfma16x16x4(av[2], f[4 * k + 2], d0);
+      d1 = mfma16x16x4(av[3], f[4 * k + 3], d1);
+    }
+    // register v of lane (c, g) holds T_b[p0+c][16b + 4g + v]; f[4b + v] = phi[p0+c][16b + 4g + v]
+    acc += (d0[0] + d1[0]) * f[4 * b] + (d0[1] + d1[1]) * f[4 * b + 1] + (d0[2] + d1[2]) * f[4 * b + 2] +
+           (d0[3] + d1[3]) * f[4 * b + 3];
   }
   acc += __shfl_xor(acc, 16);
   acc += __shfl_xor(acc, 32);
@@ -1267,11 +1349,6 @@ __global__ __launch_bounds__(256) void k_vreduce(const double* __restrict__ vpar
 // whole splits, and within an XCD's range the blocks with the most MFMAs come
 // first, so that the short diagonal blocks fill the last round.
 constexpr int kXcds = 8;
-
-// s_waitcnt immediate that waits until at most n of this wave's vector-memory
-// operations are outstanding (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
-// expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
-constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 
 // The FP64 2-D contraction, its operands staged through LDS: each wave copies
 // the amplitude rows and weights of step st + D - 1 into slot (st + D - 1) % D
@@ -2622,6 +2699,13 @@ void launch_density_nt(dg_handle* h) {
   // one wave per SIMD.  npts_pad is a multiple of 256.
   constexpr int W = NT <= 12 ? 8 : 4;
   constexpr int NBUF = (NT > 12 && NT <= 25) ? 1 : 2;
+  if constexpr (NT > 12 && NT <= 25) {
+    if (!h->bmask) {  // without screening: the fragments streamed through a ring
+      hipLaunchKernelGGL(k_density_ring<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
+                         h->ldphi, h->pfrag, h->rho);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_density<NT, W, NBUF>), dim3(nblocks(h->npts_pad, 16 * W)), dim3(64 * W), 0, h->stream,
                      h->phi, h->ldphi, h->pfrag, h->rho, h->bmask);
 }
@@ -3492,7 +3576,8 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     h->packed = (long)h->nbf * h->nbf;
   }
   DG_TRY(dalloc(&h->P, (size_t)h->packed));
-  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1) * std::max(1, h->nmol)));
+  // (+ one chunk of slack: k_density_ring copies whole 16 KB chunks)
+  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1) * std::max(1, h->nmol) + 256 * kDensityChunk));
   DG_TRY(dalloc(&h->vpart, (size_t)vtotal));
   DG_TRY(dalloc(&h->vbase, (size_t)h->natoms));
   DG_TRY(dalloc(&h->q, h->natoms));
