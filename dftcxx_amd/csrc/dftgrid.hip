@@ -363,12 +363,55 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(NBUF == 
 // fragments are stored tile after tile, reduction tile after reduction tile
 // (unit (b, k), k <= b: 4 steps x 64 lanes = 2 KB), which is the order the
 // tiles consume them; the stream is cut into chunks of kDensityChunk units
-// (16 KB, four 1 KB copy instructions per wave, the last chunk reading into the
-// buffer's slack) and each chunk is copied two chunks ahead into one of three
-// LDS slots (48 KB: two work-groups per CU).  One wait and barrier per chunk,
-// which then also tells that every wave is done with the slot the next copy
-// overwrites.  W = 4 waves share the chunks.
+// (16 KB, four 1 KB copy instructions per wave; the last chunk ends at the end
+// of the stream and overlaps the one before) and each chunk is copied two
+// chunks ahead into one of three LDS slots (48 KB: two work-groups per CU).  One wait and barrier per chunk, which then also
+// tells that every wave is done with the slot the next copy overwrites.  W = 4
+// waves share the chunks.
 constexpr int kDensityChunk = 8;
+
+// Chunks of C units of a fragment stream of U >= C units (256 doubles each):
+// chunk j starts at unit C j, except the last, which ends at unit U (so every
+// chunk is whole and no copy reads past the stream).
+template <int C, int U>
+__device__ __forceinline__ constexpr int stream_start(int j) {
+  return j < (U + C - 1) / C - 1 ? j * C : U - C;
+}
+
+// Copy of chunk j into LDS slot dst: 1 KB pieces, wave w takes pieces w, w + 4,
+// ..., C / 2 instructions per wave.  (Every wave issues the same count, which the
+// wait below relies on; a conditional copy here would also cost registers.)
+template <int C, int U>
+__device__ __forceinline__ void stream_copy(const double* src, double* dst, int j, int wave, int lane) {
+  static_assert(U >= C && C % 2 == 0, "whole chunks");
+  const double* s0 = src + (long)stream_start<C, U>(j) * 256;
+#pragma unroll
+  for (int q = 0; q < C * 2 / 4; ++q) {
+    const int piece = wave + 4 * q;
+    __builtin_amdgcn_global_load_lds((const void*)(s0 + piece * 128 + lane * 2),
+                                     (__attribute__((address_space(3))) void*)(dst + piece * 128), 16, 0, 0);
+  }
+}
+
+// The wait at the start of chunk j, chunk j + 1 in flight (its C / 2 copies).
+template <int C, int U>
+__device__ __forceinline__ void stream_wait(int j) {
+  constexpr int NCH = (U + C - 1) / C;
+  if (j + 1 < NCH)
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(C * 2 / 4));
+  else
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+}
+
+// A work-group barrier without the release fence of __syncthreads (which would
+// wait for every copy in flight); the empty asm statements keep the compiler
+// from moving memory accesses across it.  Used where the waves' LDS reads before
+// it have all been consumed by MFMAs, so that none is in flight.
+__device__ __forceinline__ void bare_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 template <int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_density_ring(
     const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* rho) {
@@ -381,17 +424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   double f[4 * NT];
 #pragma unroll
   for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
-  // chunk j: units [C j, C j + C), 16 x 1 KB; wave w copies pieces w, w + 4, w + 8, w + 12
-  auto copy = [&](int j) {
-    const double* src = frag + (long)j * C * 256;
-    double* dst = ring[j % 3];
-#pragma unroll
-    for (int q = 0; q < C * 2 / 4; ++q) {
-      const int piece = wave + 4 * q;
-      __builtin_amdgcn_global_load_lds((const void*)(src + piece * 128 + lane * 2),
-                                       (__attribute__((address_space(3))) void*)(dst + piece * 128), 16, 0, 0);
-    }
-  };
+  auto copy = [&](int j) { stream_copy<C, U>(frag, ring[j % 3], j, wave, lane); };
   copy(0);
   if (NCH > 1) copy(1);
   double acc = 0.0;
@@ -403,22 +436,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int k = 0; k <= b; ++k, ++u) {
       if (u % C == 0) {
         const int j = u / C;
-        // this wave's copies of chunk j have landed (those of chunk j + 1 may not)
-        if (j + 1 < NCH)
-          __builtin_amdgcn_s_waitcnt(waitcnt_vm(C * 2 / 4));
-        else
-          __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-        // a bare barrier: __syncthreads' release fence would also wait for the
-        // copies of chunk j + 1; the empty asm statements keep the compiler from
-        // moving memory accesses across it (the reads of chunk j - 1 have been
-        // consumed by MFMAs before it, so none is in flight)
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-        if (j + 2 < NCH) copy(j + 2);  // into the slot of chunk j - 1, which every wave is done with
+        stream_wait<C, U>(j);  // this wave's copies of chunk j have landed
+        bare_barrier();        // and every wave's; all are done with chunk j - 1
+        if (j + 2 < NCH) copy(j + 2);  // into the slot of chunk j - 1
       }
       if (k == 0) d0 = d1 = dbl4{0.0, 0.0, 0.0, 0.0};
-      const double* cur = ring[(u / C) % 3] + (u % C) * 256;
+      const double* cur = ring[(u / C) % 3] + (u - stream_start<C, U>(u / C)) * 256;
       double av[4];
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) av[jj] = cur[jj * 64 + lane];
@@ -1915,6 +1938,53 @@ __global__ __launch_bounds__(64 * W) void k_tmat(const double* __restrict__ phi,
   }
 }
 
+// k_tmat with the full-P fragments streamed through a ring: unit (b, k) = the
+// 4 steps of reduction tile k for column tile b (2 KB), stored b-major, which
+// is the order consumed; chunks of kDensityChunk units copied two ahead into
+// three LDS slots (48 KB: two work-groups of W = 4 waves per CU), one wait and
+// barrier per chunk.  Same chains and sums as k_tmat, so the same T.
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tmat_ring(
+    const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* tmat) {
+  constexpr int C = kDensityChunk, U = NT * NT, NCH = (U + C - 1) / C;
+  __shared__ double ring[3][C * 256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const long p0 = (long)blockIdx.x * 64 + wave * 16;
+  const double* row = phi + (p0 + c) * ldphi;
+  double f[4 * NT];
+#pragma unroll
+  for (int s = 0; s < 4 * NT; ++s) f[s] = row[16 * (s >> 2) + 4 * g + (s & 3)];
+  auto copy = [&](int j) { stream_copy<C, U>(frag, ring[j % 3], j, wave, lane); };
+  copy(0);
+  if (NCH > 1) copy(1);
+  dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0, d3 = d0;
+  int u = 0;
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+#pragma unroll
+    for (int k = 0; k < NT; ++k, ++u) {
+      if (u % C == 0) {
+        const int j = u / C;
+        stream_wait<C, U>(j);
+        bare_barrier();
+        if (j + 2 < NCH) copy(j + 2);
+      }
+      if (k == 0) d0 = d1 = d2 = d3 = dbl4{0.0, 0.0, 0.0, 0.0};
+      const double* cur = ring[(u / C) % 3] + (u - stream_start<C, U>(u / C)) * 256;
+      d0 = mfma16x16x4(cur[lane], f[4 * k], d0);
+      d1 = mfma16x16x4(cur[64 + lane], f[4 * k + 1], d1);
+      d2 = mfma16x16x4(cur[128 + lane], f[4 * k + 2], d2);
+      d3 = mfma16x16x4(cur[192 + lane], f[4 * k + 3], d3);
+    }
+    double* out = tmat + (p0 + c) * ldphi + 16 * b + g;
+    out[0] = (d0[0] + d1[0]) + (d2[0] + d3[0]);
+    out[4] = (d0[1] + d1[1]) + (d2[1] + d3[1]);
+    out[8] = (d0[2] + d1[2]) + (d2[2] + d3[2]);
+    out[12] = (d0[3] + d1[3]) + (d2[3] + d3[3]);
+  }
+}
+
 // Value and gradient of component `comp` of a shell with radial sums
 // r0 = sum cN0 e, d0 = sum cN0 (-2 alpha) e (and r1, d1 for the mixed d
 // components): phi = R x^l y^m z^n, d/dx phi = l x^(l-1) y^m z^n R + x phi' with R' = d.
@@ -2751,6 +2821,11 @@ int launch_density(dg_handle* h, const double* dP) {
 
 template <int NT>
 void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat) {
+  // 13 to 25 tiles: the fragments streamed through a ring, two work-groups per CU
+  if constexpr (NT > 12 && NT <= 25) {
+    hipLaunchKernelGGL(k_tmat_ring<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
+    return;
+  }
   // eight waves (two per SIMD) when the rows come in 128s and the amplitude
   // registers (8 NT) leave room for both
   if constexpr (NT <= 25) {
@@ -3576,8 +3651,7 @@ static int create_impl(const dg_molecule* mol, const dg_basis* basis, const dg_g
     h->packed = (long)h->nbf * h->nbf;
   }
   DG_TRY(dalloc(&h->P, (size_t)h->packed));
-  // (+ one chunk of slack: k_density_ring copies whole 16 KB chunks)
-  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1) * std::max(1, h->nmol) + 256 * kDensityChunk));
+  DG_TRY(dalloc(&h->pfrag, (size_t)128 * h->ntb * (h->ntb + 1) * std::max(1, h->nmol)));
   DG_TRY(dalloc(&h->vpart, (size_t)vtotal));
   DG_TRY(dalloc(&h->vbase, (size_t)h->natoms));
   DG_TRY(dalloc(&h->q, h->natoms));
