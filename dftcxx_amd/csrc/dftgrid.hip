@@ -404,11 +404,14 @@ __device__ __forceinline__ void stream_wait(int j) {
 }
 
 // A work-group barrier without the release fence of __syncthreads (which would
-// wait for every copy in flight); the empty asm statements keep the compiler
-// from moving memory accesses across it.  Used where the waves' LDS reads before
-// it have all been consumed by MFMAs, so that none is in flight.
+// also wait for every copy in flight): this wave's LDS reads are waited for
+// (lgkmcnt 0, vmcnt and expcnt left at their maxima), since the compiler may
+// issue a read before the barrier and consume it after, while the slot it reads
+// is overwritten by another wave's copy issued after the barrier; the empty asm
+// statements keep the compiler from moving memory accesses across it.
 __device__ __forceinline__ void bare_barrier() {
   asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4));  // lgkmcnt(0)
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
 }
@@ -2762,15 +2765,16 @@ int sort_items(dg_handle* h) {
 
 template <int NT>
 void launch_density_nt(dg_handle* h) {
-  // small bases (NT <= 12): eight waves share double-buffered tiles (the tiles
-  // are short, so the copy of the next one is what to hide); up to 400 functions:
-  // one tile buffer and two work-groups per CU (the amplitude rows, 8 NT
-  // registers, leave room for two waves per SIMD); beyond, double-buffered with
-  // one wave per SIMD.  npts_pad is a multiple of 256.
+  // Without screening, up to 400 functions: the fragments streamed through a
+  // ring, two work-groups per CU (the amplitude rows, 8 NT registers, leave room
+  // for two waves per SIMD).  With screening: small bases (NT <= 12) eight waves
+  // share double-buffered tiles; up to 400 functions one tile buffer and two
+  // work-groups per CU; beyond, double-buffered with one wave per SIMD.
+  // npts_pad is a multiple of 256.
   constexpr int W = NT <= 12 ? 8 : 4;
   constexpr int NBUF = (NT > 12 && NT <= 25) ? 1 : 2;
-  if constexpr (NT > 12 && NT <= 25) {
-    if (!h->bmask) {  // without screening: the fragments streamed through a ring
+  if constexpr (NT >= 8 && NT <= 25) {
+    if (!h->bmask) {
       hipLaunchKernelGGL(k_density_ring<NT>, dim3(nblocks(h->npts_pad, 64)), dim3(256), 0, h->stream, h->phi,
                          h->ldphi, h->pfrag, h->rho);
       return;
@@ -2821,8 +2825,8 @@ int launch_density(dg_handle* h, const double* dP) {
 
 template <int NT>
 void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat) {
-  // 13 to 25 tiles: the fragments streamed through a ring, two work-groups per CU
-  if constexpr (NT > 12 && NT <= 25) {
+  // 8 to 25 tiles: the fragments streamed through a ring, two work-groups per CU
+  if constexpr (NT >= 8 && NT <= 25) {
     hipLaunchKernelGGL(k_tmat_ring<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
     return;
   }
