@@ -1403,35 +1403,38 @@ __global__ __launch_bounds__(256, 3) void k_contract2d_lds(const double* __restr
   if (v >= ncb * nsplit) return;
   const int cbi = order[v].x, split = order[v].y;
   const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index as a scalar, so that the ring addresses stay in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
   const int s0 = (int)(cnt * split / nsplit);
   const int* bl = blist + l0 + s0;
   const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
-  // first of the 4 points of step t (this wave's; points p4 .. p4 + 3)
-  auto point4 = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave; };
   double* my = ring + wave * SLOT * D;
-  // copy lane roles: A pair k = 64 i + lane -> point k / 32, columns 2 (k % 32), +1
+  // step t covers this wave's points p .. p + 3, p = 4 wave + rows(t) with
+  // rows(t) = 64 s0 + 16 t dense, 64 blist[t / 4] + 16 (t % 4) screened
+  auto rows = [&](int t) { return DENSE ? 16L * t : 64L * bl[t >> 2] + 16 * (t & 3); };
+  // copy lane roles: A pair k = 64 i + lane -> point k / 32, columns 2 (k % 32), +1;
+  // weights: lanes 0, 1 u0 of points 0-1, 2-3, lanes 2, 3 u1 (unused with one output).
+  // Each lane's sources at rows(t) = 0, so that a copy adds one row offset.
   const int ag = lane >> 5, acol = 16 * I0 + 2 * (lane & 31);
   const int bg = NJ == 2 ? lane >> 4 : lane >> 5, bcol = 16 * J0 + 2 * (NJ == 2 ? (lane & 15) : (lane & 31));
+  const long p0 = (DENSE ? 64L * s0 : 0L) + 4 * wave;
+  const double* pa = phi + (p0 + ag) * ldphi + acol;
+  const double* pb = phi + (p0 + bg) * ldphi + bcol;
+  const double* pw = (lane < 2 ? u0 : u1) + p0 + 2 * (lane & 1);
   auto copy = [&](int t, int slot) {
-    const long p4 = point4(t);
     double* dst = my + slot * SLOT;
     typedef __attribute__((address_space(3))) void* lptr;
+    const long r = rows(t), o = r * ldphi;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + 2 * i + ag) * ldphi + acol),
-                                       (lptr)(dst + i * 128), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(pa + o + 2L * i * ldphi), (lptr)(dst + i * 128), 16, 0, 0);
 #pragma unroll
     for (int i = 0; i < NJ / 2; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + (NJ == 2 ? 0 : 2 * i) + bg) * ldphi + bcol),
+      __builtin_amdgcn_global_load_lds((const void*)(pb + o + (NJ == 2 ? 0L : 2L * i * ldphi)),
                                        (lptr)(dst + SA + i * 128), 16, 0, 0);
-    // weights: lanes 0, 1 u0 of points 0-1, 2-3; lanes 2, 3 u1 (unused with one output)
-    if (lane < 4) {
-      const double* src = (lane < 2 ? u0 : u1) + p4 + 2 * (lane & 1);
-      __builtin_amdgcn_global_load_lds((const void*)src, (lptr)(dst + SA + SB), 16, 0, 0);
-    }
+    if (lane < 4) __builtin_amdgcn_global_load_lds((const void*)(pw + r), (lptr)(dst + SA + SB), 16, 0, 0);
   };
   dbl4 acc[NOUT][4][NJ];
 #pragma unroll
@@ -1522,31 +1525,34 @@ __global__ __launch_bounds__(256, 2) void k_contract2d_lds_f32(const float* __re
   if (v >= ncb * nsplit) return;
   const int cbi = order[v].x, split = order[v].y;
   const int I0 = cblocks[4 * cbi], nI = cblocks[4 * cbi + 1], J0 = cblocks[4 * cbi + 2], nJ = cblocks[4 * cbi + 3];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index as a scalar, so that the ring addresses stay in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   const long l0 = loff[cbi], cnt = loff[cbi + 1] - l0;
   const int s0 = (int)(cnt * split / nsplit);
   const int* bl = blist + l0 + s0;
   const int nst = 4 * (int)(cnt * (split + 1) / nsplit - s0);
-  auto point4 = [&](int t) { return 64L * (DENSE ? s0 + (t >> 2) : bl[t >> 2]) + 16 * (t & 3) + 4 * wave; };
   char* my = ring + wave * SLOT * D;
+  // step t covers this wave's points p .. p + 3, p = 4 wave + rows(t) with
+  // rows(t) = 64 s0 + 16 t dense, 64 blist[t / 4] + 16 (t % 4) screened
+  auto rows = [&](int t) { return DENSE ? 16L * t : 64L * bl[t >> 2] + 16 * (t & 3); };
   // copy lane roles: A: point lane / 16, 4 columns from 16 I0 + 4 (lane % 16);
-  // B (the first 16 NJ lanes): point lane / (4 NJ), 4 columns from 16 J0 + 4 (lane % 4 NJ)
+  // B (the first 16 NJ lanes): point lane / (4 NJ), 4 columns from 16 J0 + 4 (lane % 4 NJ);
+  // weights: lanes 0, 1 u0 of points 0-1, 2-3, lanes 2, 3 u1 (unused with one output).
+  // Each lane's sources at rows(t) = 0, so that a copy adds one row offset.
   const int ag = lane >> 4, acol = 16 * I0 + 4 * (lane & 15);
   const int bg = lane / (4 * NJ), bcol = 16 * J0 + 4 * (lane % (4 * NJ));
+  const long p0 = (DENSE ? 64L * s0 : 0L) + 4 * wave;
+  const float* pa = phi + (p0 + ag) * ldphi + acol;
+  const float* pb = phi + (p0 + bg) * ldphi + bcol;
+  const double* pw = (lane < 2 ? u0 : u1) + p0 + 2 * (lane & 1);
   auto copy = [&](int t, int slot) {
-    const long p4 = point4(t);
     char* dst = my + slot * SLOT;
     typedef __attribute__((address_space(3))) void* lptr;
-    __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + ag) * ldphi + acol), (lptr)dst, 16, 0, 0);
-    if (lane < 16 * NJ)
-      __builtin_amdgcn_global_load_lds((const void*)(phi + (p4 + bg) * ldphi + bcol), (lptr)(dst + SA * 4), 16, 0,
-                                       0);
-    // weights: lanes 0, 1 u0 of points 0-1, 2-3; lanes 2, 3 u1 (unused with one output)
-    if (lane < 4) {
-      const double* src = (lane < 2 ? u0 : u1) + p4 + 2 * (lane & 1);
-      __builtin_amdgcn_global_load_lds((const void*)src, (lptr)(dst + (SA + SB) * 4), 16, 0, 0);
-    }
+    const long r = rows(t), o = r * ldphi;
+    __builtin_amdgcn_global_load_lds((const void*)(pa + o), (lptr)dst, 16, 0, 0);
+    if (lane < 16 * NJ) __builtin_amdgcn_global_load_lds((const void*)(pb + o), (lptr)(dst + SA * 4), 16, 0, 0);
+    if (lane < 4) __builtin_amdgcn_global_load_lds((const void*)(pw + r), (lptr)(dst + (SA + SB) * 4), 16, 0, 0);
   };
   flt4 acc[NOUT][4][NJ];
   double sum[NOUT][4][NJ][4];
