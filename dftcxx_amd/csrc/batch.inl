@@ -296,13 +296,19 @@ int batch_mol_of_atom(const dg_handle* h, int k) {
 // molecule outside its own cell, in ranges of at most len (<= kMaxItem) points,
 // k ascending: {k, p_begin, p_end, offset of the item's point order in perm}.
 // An item's order holds its points and at most one pad slot per spline
-// interval group (nrad + 1 groups); *perm_len receives the total.
+// interval group (nrad + 1 groups); *perm_len receives the total.  A range is
+// cut into equal items of at most cap points, cap chosen so that an item's
+// padded order never exceeds len slots: the cross kernel walks an item in
+// passes of 2048 slots, and an order just over a multiple of that would take
+// a nearly empty extra pass.
 std::vector<int> cross_items(const dg_handle* h, int len, long* perm_len) {
   std::vector<int> items;
   long off = 0;
+  const long cap = std::max<long>(len / 2, len - (h->nrad + 1));
   auto add = [&](int k, long b, long e) {
-    for (long p = b; p < e; p += len) {
-      const long q = std::min(e, p + len);
+    const long m = (e - b + cap - 1) / cap;
+    for (long r = 0; r < m; ++r) {
+      const long p = b + (e - b) * r / m, q = b + (e - b) * (r + 1) / m;
       items.push_back(k);
       items.push_back((int)p);
       items.push_back((int)q);
