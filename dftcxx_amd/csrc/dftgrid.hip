@@ -2241,6 +2241,9 @@ constexpr double kBeckeSkip = 1e-20;
 // caller reports the gradient as unsupported.
 constexpr int kBeckeActive = 24;
 
+// Active entries per pass of the Becke-term loops (held in registers).
+constexpr int kBeckeBlock = 8;
+
 __device__ __forceinline__ double atom_dist(const double* __restrict__ atoms, int m, double x, double y, double z) {
   const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
   return fmax(sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
@@ -2359,59 +2362,92 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
   }
   const double pref = (valid && Z > 0.0) ? w0a / Z : 0.0;
   const double wbv = (valid && Z > 0.0) ? Pown / Z : 0.0;
+  const bool on = valid && pref != 0.0;
   double own_acc[3] = {0.0, 0.0, 0.0};
-  int tnext = 0;  // the active list is in ascending atom order: its next entry
-  for (int B = 0; B < natoms; ++B) {
-    double gB[3] = {0.0, 0.0, 0.0};
-    double PB = 0.0;
-    if (tnext < nact && act_n[tnext * 64 + lane] == B) {
-      PB = act_P[tnext * 64 + lane];
-      ++tnext;
+  // Every pair (active n, atom m != n) enters twice with the same
+  // rho = ratio(mu_nm), f = P_n rho / R_nm and e_nm: into dP_n/dR_m as
+  // f (u_m + mu e_nm), and into dP_n/dR_n as -f (u_n + mu e_nm).  Passes over
+  // kBeckeBlock active entries at a time (registers), atoms m in order: output m
+  // is complete at the end of its step; output n sums F = sum_m f and
+  // G = sum_m f mu e_nm over the pass and is added after it, as
+  // dP_n/dR_n = -(u_n F + G).  Per atom B the sums go to part[B] in a fixed order:
+  // the m terms of pass 0, the n terms of pass 0, then those of pass 1, ...
+  int nmax = on ? nact : 0;
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+  for (int t0 = 0; t0 < nmax; t0 += kBeckeBlock) {
+    int nk[kBeckeBlock];
+    double Pk[kBeckeBlock], dk[kBeckeBlock], wk[kBeckeBlock][3], F[kBeckeBlock], G[kBeckeBlock][3];
+#pragma unroll
+    for (int k = 0; k < kBeckeBlock; ++k) {
+      const bool in = on && t0 + k < nact;
+      nk[k] = in ? act_n[(t0 + k) * 64 + lane] : -1;
+      Pk[k] = in ? act_P[(t0 + k) * 64 + lane] : 0.0;
+      const int n = in ? nk[k] : 0;
+      // w = R_n - r
+      wk[k][0] = atoms[4 * n] - x;
+      wk[k][1] = atoms[4 * n + 1] - y;
+      wk[k][2] = atoms[4 * n + 2] - z;
+      dk[k] = atom_dist(atoms, n, x, y, z);
+      F[k] = G[k][0] = G[k][1] = G[k][2] = 0.0;
     }
-    if (valid && B != own && pref != 0.0) {
-      const double dB = atom_dist(atoms, B, x, y, z);
-      const double uB[3] = {(x - atoms[4 * B]) / dB, (y - atoms[4 * B + 1]) / dB, (z - atoms[4 * B + 2]) / dB};
-      double sumdP[3] = {0.0, 0.0, 0.0}, dPa[3] = {0.0, 0.0, 0.0};
-      if (PB > skip) {
-        // dP_B/dR_B = P_B sum_{m != B} ratio(mu_Bm) (-(u_B + mu_Bm e_Bm)/R_Bm)
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int m = 0; m < natoms; ++m) {
-          if (m == B) continue;
-          const double iR = invR[B * natoms + m];
-          const double mu = (dB - atom_dist(atoms, m, x, y, z)) * iR;
-          double sv, rt;
-          cutoff_and_ratio(mu, sv, rt);
-          const double e[3] = {(atoms[4 * B] - atoms[4 * m]) * iR, (atoms[4 * B + 1] - atoms[4 * m + 1]) * iR,
-                               (atoms[4 * B + 2] - atoms[4 * m + 2]) * iR};
-          for (int d = 0; d < 3; ++d) acc[d] += rt * (-(uB[d] + mu * e[d]) * iR);
-        }
-        for (int d = 0; d < 3; ++d) sumdP[d] = PB * acc[d];
-      }
-      // n != B: dP_n/dR_B = P_n ratio(mu_nB) (u_B + mu_nB e_nB)/R_nB
-      for (int t = 0; t < nact; ++t) {
-        const int n = act_n[t * 64 + lane];
-        if (n == B) continue;
-        const double iR = invR[n * natoms + B];
-        const double mu = (atom_dist(atoms, n, x, y, z) - dB) * iR;
+    for (int m = 0; m < natoms; ++m) {
+      const double dm = atom_dist(atoms, m, x, y, z);
+      const double vm[3] = {x - atoms[4 * m], y - atoms[4 * m + 1], z - atoms[4 * m + 2]};  // r - R_m
+      const double um[3] = {vm[0] / dm, vm[1] / dm, vm[2] / dm};
+      double S[3] = {0.0, 0.0, 0.0}, Dp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < kBeckeBlock; ++k) {
+        const int n = nk[k];
+        if (n < 0 || n == m) continue;
+        const double iR = invR[n * natoms + m];
+        const double mu = (dk[k] - dm) * iR;
         double sv, rt;
         cutoff_and_ratio(mu, sv, rt);
-        const double e[3] = {(atoms[4 * n] - atoms[4 * B]) * iR, (atoms[4 * n + 1] - atoms[4 * B + 1]) * iR,
-                             (atoms[4 * n + 2] - atoms[4 * B + 2]) * iR};
-        const double f = act_P[t * 64 + lane] * rt * iR;
+        const double f = Pk[k] * rt * iR, fm = f * mu;
+        F[k] += f;
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const double tv = f * (uB[d] + mu * e[d]);
-          sumdP[d] += tv;
-          if (n == own) dPa[d] = tv;
+          const double me = fm * ((wk[k][d] + vm[d]) * iR);  // f mu e_nm
+          const double tv = f * um[d] + me;
+          S[d] += tv;
+          if (n == own) Dp[d] = tv;
+          G[k][d] += me;
         }
       }
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
-        gB[d] = pref * (dPa[d] - wbv * sumdP[d]);
-        own_acc[d] -= gB[d];
+        const double gm = (on && m != own) ? pref * (Dp[d] - wbv * S[d]) : 0.0;
+        own_acc[d] -= gm;
+        const double t = wave_sum(gm);
+        if (lane == 0) part[((long)m * 3 + d) * nblk + blockIdx.x] += t;
       }
     }
-    for (int d = 0; d < 3; ++d) {
-      const double t = wave_sum(gB[d]);
-      if (lane == 0) part[((long)B * 3 + d) * nblk + blockIdx.x] += t;
+    // the pass's entries n: d wb/dR_n gains pref wbv (u_n F + G), u_n = -w / d_n
+#pragma unroll
+    for (int k = 0; k < kBeckeBlock; ++k) {
+      const bool use = nk[k] >= 0 && nk[k] != own;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        G[k][d] = use ? pref * wbv * (G[k][d] - wk[k][d] / dk[k] * F[k]) : 0.0;
+        own_acc[d] -= G[k][d];
+      }
+    }
+    for (int B = 0; B < natoms; ++B) {
+      double v[3] = {0.0, 0.0, 0.0};
+      bool has = false;
+#pragma unroll
+      for (int k = 0; k < kBeckeBlock; ++k)
+        if (nk[k] == B) {
+          has = true;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) v[d] = G[k][d];
+        }
+      if (__ballot(has) == 0) continue;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double t = wave_sum(v[d]);
+        if (lane == 0) part[((long)B * 3 + d) * nblk + blockIdx.x] += t;
+      }
     }
   }
   // the owner atom's share (translation invariance), per point
