@@ -2315,20 +2315,49 @@ __global__ __launch_bounds__(64) void k_force_becke(const double* __restrict__ a
     pnear = cell_function(atoms, invR, natoms, near, near, dnear, dnear, x, y, z, 0.0);
     const double cut0 = 1e-4 * kBeckeSkip * pnear, pre = kBeckeSkip * pnear;
     Z = pnear;
-    for (int n = 0; n < natoms; ++n) {
-      double P = pnear;
-      if (n != near) {
-        P = cell_function(atoms, invR, natoms, n, near, atom_dist(atoms, n, x, y, z), dnear, x, y, z, cut0);
-        Z += P;
+    // the cell functions of kBeckeBlock atoms at a time, each factor sequence as in
+    // cell_function (the nearest atom first, then m ascending, stopped once the
+    // product is at most cut0), the distance to each atom m shared by the block
+    for (int n0 = 0; n0 < natoms; n0 += kBeckeBlock) {
+      double dn[kBeckeBlock], prod[kBeckeBlock];
+      bool alive = false;
+#pragma unroll
+      for (int k = 0; k < kBeckeBlock; ++k) {
+        const int n = n0 + k;
+        const bool in = n < natoms && n != near;
+        const int ns = in ? n : near;
+        dn[k] = atom_dist(atoms, ns, x, y, z);
+        prod[k] = in ? becke_s((dn[k] - dnear) * invR[ns * natoms + near]) : 0.0;
+        alive |= prod[k] > cut0;
       }
-      if (n == own) Pown = P;
-      if (P > pre) {
-        if (ncand < kBeckeActive) {
-          act_n[ncand * 64 + lane] = (unsigned short)n;
-          act_P[ncand * 64 + lane] = P;
-          ++ncand;
-        } else {
-          cand_full = true;
+      for (int m = 0; m < natoms && __ballot(alive) != 0; ++m) {
+        const double dm = atom_dist(atoms, m, x, y, z);
+        alive = false;
+#pragma unroll
+        for (int k = 0; k < kBeckeBlock; ++k) {
+          const int n = min(n0 + k, natoms - 1);  // past the last atom: prod[k] = 0, no factor taken
+          if (m != n && m != near && prod[k] > cut0) prod[k] *= becke_s((dn[k] - dm) * invR[n * natoms + m]);
+          alive |= prod[k] > cut0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBeckeBlock; ++k) {
+        const int n = n0 + k;
+        if (n >= natoms) break;
+        double P = pnear;
+        if (n != near) {
+          P = prod[k];
+          Z += P;
+        }
+        if (n == own) Pown = P;
+        if (P > pre) {
+          if (ncand < kBeckeActive) {
+            act_n[ncand * 64 + lane] = (unsigned short)n;
+            act_P[ncand * 64 + lane] = P;
+            ++ncand;
+          } else {
+            cand_full = true;
+          }
         }
       }
     }
