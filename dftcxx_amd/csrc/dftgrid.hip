@@ -2052,6 +2052,11 @@ __global__ __launch_bounds__(64) void k_force_basis(
       const double r2 = dx * dx + dy * dy + dz * dz;
       for (int t = ash_start[A]; t < ash_start[A + 1]; ++t) {
         const int s = ash_list[t];
+        // every primitive's exp(-alpha r2) below e^-100 (1e-43): the shell's terms are
+        // below the rounding of the sums they would enter, and it is skipped
+        double amin = alpha[sh_off[s]];
+        for (int k = 1; k < sh_nprim[s]; ++k) amin = fmin(amin, alpha[sh_off[s] + k]);
+        if (amin * r2 > 100.0) continue;
         double r0 = 0.0, d0 = 0.0, r1 = 0.0, d1 = 0.0;
         for (int k = 0; k < sh_nprim[s]; ++k) {
           const int q = sh_off[s] + k;
