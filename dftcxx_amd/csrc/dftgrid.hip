@@ -1952,10 +1952,16 @@ __global__ __launch_bounds__(64 * W) void k_tmat(const double* __restrict__ phi,
 // is the order consumed; chunks of kDensityChunk units copied two ahead into
 // three LDS slots (48 KB: two work-groups of W = 4 waves per CU), one wait and
 // barrier per chunk.  Same chains and sums as k_tmat, so the same T.
+// With tmask (bit b of tmask[block]: tile b matters at the block's 64 points,
+// k_tile_mask), the block forms and writes only the column tiles in its mask
+// and sums only over the reduction tiles in it (the others' amplitudes are
+// below e^-100 there); the ring's copies, waits and barriers run unchanged.
 template <int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tmat_ring(
-    const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* tmat) {
+    const double* __restrict__ phi, int ldphi, const double* __restrict__ frag, double* tmat,
+    const unsigned* __restrict__ tmask) {
   constexpr int C = kDensityChunk, U = NT * NT, NCH = (U + C - 1) / C;
+  const unsigned tm = tmask ? tmask[blockIdx.x] : ~0u;
   __shared__ double ring[3][C * 256];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
@@ -1967,30 +1973,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   auto copy = [&](int j) { stream_copy<C, U>(frag, ring[j % 3], j, wave, lane); };
   copy(0);
   if (NCH > 1) copy(1);
-  dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0, d3 = d0;
-  int u = 0;
+  // the ring's wait, barrier and next copy at the first unit of every chunk
+  auto ring_step = [&](int u) {
+    if (u % C == 0) {
+      const int j = u / C;
+      stream_wait<C, U>(j);
+      bare_barrier();
+      if (j + 2 < NCH) copy(j + 2);
+    }
+  };
 #pragma unroll
   for (int b = 0; b < NT; ++b) {
+    if ((tm >> b) & 1u) {
+      dbl4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0, d3 = d0;
 #pragma unroll
-    for (int k = 0; k < NT; ++k, ++u) {
-      if (u % C == 0) {
-        const int j = u / C;
-        stream_wait<C, U>(j);
-        bare_barrier();
-        if (j + 2 < NCH) copy(j + 2);
+      for (int k = 0; k < NT; ++k) {
+        const int u = b * NT + k;
+        ring_step(u);
+        if ((tm >> k) & 1u) {
+          const double* cur = ring[(u / C) % 3] + (u - stream_start<C, U>(u / C)) * 256;
+          d0 = mfma16x16x4(cur[lane], f[4 * k], d0);
+          d1 = mfma16x16x4(cur[64 + lane], f[4 * k + 1], d1);
+          d2 = mfma16x16x4(cur[128 + lane], f[4 * k + 2], d2);
+          d3 = mfma16x16x4(cur[192 + lane], f[4 * k + 3], d3);
+        }
       }
-      if (k == 0) d0 = d1 = d2 = d3 = dbl4{0.0, 0.0, 0.0, 0.0};
-      const double* cur = ring[(u / C) % 3] + (u - stream_start<C, U>(u / C)) * 256;
-      d0 = mfma16x16x4(cur[lane], f[4 * k], d0);
-      d1 = mfma16x16x4(cur[64 + lane], f[4 * k + 1], d1);
-      d2 = mfma16x16x4(cur[128 + lane], f[4 * k + 2], d2);
-      d3 = mfma16x16x4(cur[192 + lane], f[4 * k + 3], d3);
+      double* out = tmat + (p0 + c) * ldphi + 16 * b + g;
+      out[0] = (d0[0] + d1[0]) + (d2[0] + d3[0]);
+      out[4] = (d0[1] + d1[1]) + (d2[1] + d3[1]);
+      out[8] = (d0[2] + d1[2]) + (d2[2] + d3[2]);
+      out[12] = (d0[3] + d1[3]) + (d2[3] + d3[3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) ring_step(b * NT + k);
     }
-    double* out = tmat + (p0 + c) * ldphi + 16 * b + g;
-    out[0] = (d0[0] + d1[0]) + (d2[0] + d3[0]);
-    out[4] = (d0[1] + d1[1]) + (d2[1] + d3[1]);
-    out[8] = (d0[2] + d1[2]) + (d2[2] + d3[2]);
-    out[12] = (d0[3] + d1[3]) + (d2[3] + d3[3]);
   }
 }
 
@@ -2025,6 +2041,46 @@ __device__ __forceinline__ void shell_grad(int L, int comp, double r0, double d0
 }
 
 
+// A shell is taken at a point while alpha_min r^2 <= kShellReach (some
+// primitive's exp(-alpha r^2) at least e^-100).
+constexpr double kShellReach = 100.0;
+
+// Function tiles the force terms read, per 64-point block: bit t of mask[blk]
+// is set when a shell with a function in tile t (16 functions) passes the
+// shell test at one of the block's points, with a 1 % margin on kShellReach so
+// that every shell k_force_basis takes is covered.  Four threads per point,
+// each taking every fourth shell; the bits are OR-ed over the block.
+__global__ __launch_bounds__(256) void k_tile_mask(const double* __restrict__ atoms, int nshell,
+                                                   const int* __restrict__ sh_atom, const int* __restrict__ sh_L,
+                                                   const int* __restrict__ sh_nprim, const int* __restrict__ sh_off,
+                                                   const int* __restrict__ sh_cgf, const double* __restrict__ alpha,
+                                                   long npts, const double* __restrict__ px,
+                                                   const double* __restrict__ py, const double* __restrict__ pz,
+                                                   unsigned* mask) {
+  __shared__ unsigned red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long p = (long)blockIdx.x * 64 + lane;
+  unsigned m = 0;
+  if (p < npts) {
+    const double x = px[p], y = py[p], z = pz[p];
+    for (int s = wave; s < nshell; s += 4) {
+      const int a = sh_atom[s];
+      const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      double amin = alpha[sh_off[s]];
+      for (int k = 1; k < sh_nprim[s]; ++k) amin = fmin(amin, alpha[sh_off[s] + k]);
+      if (amin * r2 <= 1.01 * kShellReach) {
+        const int L = sh_L[s], i0 = sh_cgf[s], i1 = i0 + (L + 1) * (L + 2) / 2 - 1;
+        for (int t = i0 >> 4; t <= (i1 >> 4); ++t) m |= 1u << t;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) m |= (unsigned)__shfl_xor((int)m, o);
+  if (lane == 0) red[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) mask[blockIdx.x] = red[0] | red[1] | red[2] | red[3];
+}
+
 // Basis-centre and grid-motion terms of dE_xc/dR, one wave per 64 points.
 // For point p and atom A: g_A(p) = ub_p sum_{i on A} T_pi grad phi_i(r_p).
 // part[(A*3+d)*nblk + blk] = -4 sum_{p in blk} g_A(p)  (functions move with A)
@@ -2052,11 +2108,11 @@ __global__ __launch_bounds__(64) void k_force_basis(
       const double r2 = dx * dx + dy * dy + dz * dz;
       for (int t = ash_start[A]; t < ash_start[A + 1]; ++t) {
         const int s = ash_list[t];
-        // every primitive's exp(-alpha r2) below e^-100 (1e-43): the shell's terms are
-        // below the rounding of the sums they would enter, and it is skipped
+        // every primitive's exp(-alpha r2) below e^-kShellReach (1e-43): the shell's
+        // terms are below the rounding of the sums they would enter, and it is skipped
         double amin = alpha[sh_off[s]];
         for (int k = 1; k < sh_nprim[s]; ++k) amin = fmin(amin, alpha[sh_off[s] + k]);
-        if (amin * r2 > 100.0) continue;
+        if (amin * r2 > kShellReach) continue;
         double r0 = 0.0, d0 = 0.0, r1 = 0.0, d1 = 0.0;
         for (int k = 0; k < sh_nprim[s]; ++k) {
           const int q = sh_off[s] + k;
@@ -2729,6 +2785,7 @@ struct dg_handle {
   double *adj_ub = nullptr, *adj_ab = nullptr, *adj_udp = nullptr, *adj_T = nullptr, *adj_frag = nullptr;
   double *adj_part = nullptr, *adj_mot = nullptr, *adj_lam = nullptr, *adj_out = nullptr, *adj_dP = nullptr;
   int *ash_start = nullptr, *ash_list = nullptr;
+  unsigned* adj_tmask = nullptr;  // per 64-point block, the function tiles the force terms read
   // state
   bool has_density = false, has_hartree = false, has_xc = false, corrected = false;
   const double* last_P = nullptr;  // device density matrix of the last density evaluation
@@ -2752,6 +2809,7 @@ struct dg_handle {
                     tb[0].cb, tb[0].groups, tb[0].list, tb[0].loff, tb[1].cb, tb[1].groups, tb[1].list,
                     tb[1].loff};
     if (bmask) (void)hipFree(bmask);
+    if (adj_tmask) (void)hipFree(adj_tmask);
     for (TileBlocks& t : tb)
       if (t.order) (void)hipFree(t.order);
     if (vbase) (void)hipFree(vbase);
@@ -2899,11 +2957,15 @@ int launch_density(dg_handle* h, const double* dP) {
   });
 }
 
+// tmask: the tiles that matter per 64-point block (k_tile_mask), or null for
+// all; only the ring kernel uses it, the others form every tile.
 template <int NT>
-void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat) {
+void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag, double* tmat,
+                    const unsigned* tmask) {
   // 8 to 25 tiles: the fragments streamed through a ring, two work-groups per CU
   if constexpr (NT >= 8 && NT <= 25) {
-    hipLaunchKernelGGL(k_tmat_ring<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
+    hipLaunchKernelGGL(k_tmat_ring<NT>, dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat,
+                       tmask);
     return;
   }
   // eight waves (two per SIMD) when the rows come in 128s and the amplitude
@@ -2917,13 +2979,14 @@ void launch_tmat_nt(hipStream_t st, long npts_pad, const double* phi, int ldphi,
   hipLaunchKernelGGL((k_tmat<NT, 4>), dim3(nblocks(npts_pad, 64)), dim3(256), 0, st, phi, ldphi, frag, tmat);
 }
 
-// T = Phi P on npts_pad (a multiple of 64) rows, from full P fragments of nt tiles.
+// T = Phi P on npts_pad (a multiple of 64) rows, from full P fragments of nt tiles;
+// with tmask, tiles outside a block's mask may be left unformed.
 void launch_tmat_rows(int nt, hipStream_t st, long npts_pad, const double* phi, int ldphi, const double* frag,
-                      double* tmat) {
+                      double* tmat, const unsigned* tmask = nullptr) {
   switch (nt) {
-#define DG_NT(N)                                                  \
-  case N:                                                         \
-    launch_tmat_nt<N>(st, npts_pad, phi, ldphi, frag, tmat); \
+#define DG_NT(N)                                                         \
+  case N:                                                                \
+    launch_tmat_nt<N>(st, npts_pad, phi, ldphi, frag, tmat, tmask); \
     break;
     DG_NT(1) DG_NT(2) DG_NT(3) DG_NT(4) DG_NT(5) DG_NT(6) DG_NT(7) DG_NT(8) DG_NT(9) DG_NT(10) DG_NT(11)
     DG_NT(12) DG_NT(13) DG_NT(14) DG_NT(15) DG_NT(16) DG_NT(17) DG_NT(18) DG_NT(19) DG_NT(20) DG_NT(21)
@@ -2936,14 +2999,27 @@ void launch_tmat_rows(int nt, hipStream_t st, long npts_pad, const double* phi, 
 }
 
 // T = Phi P for the force terms (P = the last density matrix, full fragments).
-int launch_tmat(dg_handle* h) {
+// T = Phi P for the last density matrix.  masked (the force terms): only the
+// tiles k_force_basis reads are formed (k_tile_mask, computed on first use).
+int launch_tmat(dg_handle* h, bool masked = false) {
+  if (masked && !h->adj_tmask && h->ntb <= 32) {
+    const long nblk = h->npts_pad / 64;
+    DG_TRY(dalloc(&h->adj_tmask, nblk));
+    DG_TRY(run(h, "adj_tile_mask", [&] {
+      hipLaunchKernelGGL(k_tile_mask, dim3(nblk), dim3(256), 0, h->stream, h->atoms, h->nshell, h->sh_atom, h->sh_L,
+                         h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->npts, h->px, h->py, h->pz,
+                         h->adj_tmask);
+    }));
+  }
   DG_TRY(run(h, "adj_pfrag", [&] {
     const long nf = 256L * h->ntb * h->ntb;
     hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(nf, 256)), dim3(256), 0, h->stream, h->last_P, h->nbf, h->ntb,
                        h->adj_frag);
   }));
-  return run(h, "adj_tmat",
-             [&] { launch_tmat_rows(h->ntb, h->stream, h->npts_pad, h->phi, h->ldphi, h->adj_frag, h->adj_T); });
+  const unsigned* tmask = masked ? h->adj_tmask : nullptr;
+  return run(h, "adj_tmat", [&] {
+    launch_tmat_rows(h->ntb, h->stream, h->npts_pad, h->phi, h->ldphi, h->adj_frag, h->adj_T, tmask);
+  });
 }
 
 int stage_density(dg_handle* h, const double* dP) {
@@ -4129,7 +4205,7 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
     DG_TRY(copy_out(h, dEdP, h->adj_dP, (size_t)h->nbf * h->nbf));
   }
   if (grad) {
-    DG_TRY(launch_tmat(h));
+    DG_TRY(launch_tmat(h, true));
     DG_TRY(run(h, "adj_force_basis", [&] {
       hipLaunchKernelGGL(k_force_basis, dim3(nblk), dim3(64), 0, h->stream, h->atoms, na, h->ash_start, h->ash_list,
                          h->sh_L, h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->pr_cn0, h->pr_cn1, h->npts,

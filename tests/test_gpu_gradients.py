@@ -82,3 +82,24 @@ def test_points_in_chunks_and_empty():
     assert rel_err(rho[sel], rho_o) < 1e-12 and rel_err(grad[sel], grad_o) < 1e-12
     r0, g0 = G.density_at_points(mol, np.zeros((0, 3)), P)
     assert r0.shape == (0,) and g0.shape == (0, 3)
+
+
+def test_density_gradients_after_forces():
+    """The force terms form T = Phi P only on the tiles they read (nine tiles
+    here, so the streamed T kernel with its per-block tile mask); a later
+    dg_get_density_gradients forms all of T again and matches a fresh handle."""
+    mol = M.alkane(10)
+    P = random_density(mol.nbf(), 5)
+    print(f"density gradients after forces on {mol.name}: {mol.nbf()} functions", flush=True)
+    ref = G.MolecularGrid(mol, 20, 26, 4)
+    ref.set_density(P)
+    ref.correct_densities()
+    want = ref.get_density_gradients()
+    g = G.MolecularGrid(mol, 20, 26, 4)
+    g.set_density(P)
+    g.correct_densities()
+    g.calculate_exchange_correlation()
+    _, grad = g.xc_adjoint(hold_c=False)
+    assert np.all(np.isfinite(grad))
+    got = g.get_density_gradients()
+    assert np.array_equal(got, want)
