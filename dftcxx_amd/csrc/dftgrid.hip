@@ -2786,6 +2786,7 @@ struct dg_handle {
   double *adj_part = nullptr, *adj_mot = nullptr, *adj_lam = nullptr, *adj_out = nullptr, *adj_dP = nullptr;
   int *ash_start = nullptr, *ash_list = nullptr;
   unsigned* adj_tmask = nullptr;  // per 64-point block, the function tiles the force terms read
+  int *adj_list = nullptr, *adj_loff = nullptr;  // dE/dP: point blocks per one-output tile block
   // state
   bool has_density = false, has_hartree = false, has_xc = false, corrected = false;
   const double* last_P = nullptr;  // device density matrix of the last density evaluation
@@ -2804,7 +2805,7 @@ struct dg_handle {
                       mol_z, mol_fac, mol_qraw, atom_exc, atom_ej, qpart, uG};
     for (double* b : bufs)
       if (b) (void)hipFree(b);
-    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, plen, ash_start, ash_list,
+    int* ibufs[] = {sh_atom, sh_L, sh_nprim, sh_off, sh_cgf, blocks, items, plen, ash_start, ash_list, adj_list, adj_loff,
                     atom_mol, mol_atom0, mol_shell0, mol_nbf, bchunks, mol_chunk0,
                     tb[0].cb, tb[0].groups, tb[0].list, tb[0].loff, tb[1].cb, tb[1].groups, tb[1].list,
                     tb[1].loff};
@@ -2999,18 +3000,22 @@ void launch_tmat_rows(int nt, hipStream_t st, long npts_pad, const double* phi, 
 }
 
 // T = Phi P for the force terms (P = the last density matrix, full fragments).
+// The function tiles within reach of each 64-point block (k_tile_mask), on
+// first use; none beyond 32 tiles (the masks are 32-bit).
+int ensure_tile_mask(dg_handle* h) {
+  if (h->adj_tmask || h->ntb > 32) return DG_OK;
+  const long nblk = h->npts_pad / 64;
+  DG_TRY(dalloc(&h->adj_tmask, nblk));
+  return run(h, "adj_tile_mask", [&] {
+    hipLaunchKernelGGL(k_tile_mask, dim3(nblk), dim3(256), 0, h->stream, h->atoms, h->nshell, h->sh_atom, h->sh_L,
+                       h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->npts, h->px, h->py, h->pz, h->adj_tmask);
+  });
+}
+
 // T = Phi P for the last density matrix.  masked (the force terms): only the
-// tiles k_force_basis reads are formed (k_tile_mask, computed on first use).
+// tiles k_force_basis reads are formed (k_tile_mask).
 int launch_tmat(dg_handle* h, bool masked = false) {
-  if (masked && !h->adj_tmask && h->ntb <= 32) {
-    const long nblk = h->npts_pad / 64;
-    DG_TRY(dalloc(&h->adj_tmask, nblk));
-    DG_TRY(run(h, "adj_tile_mask", [&] {
-      hipLaunchKernelGGL(k_tile_mask, dim3(nblk), dim3(256), 0, h->stream, h->atoms, h->nshell, h->sh_atom, h->sh_L,
-                         h->sh_nprim, h->sh_off, h->sh_cgf, h->pr_alpha, h->npts, h->px, h->py, h->pz,
-                         h->adj_tmask);
-    }));
-  }
+  if (masked) DG_TRY(ensure_tile_mask(h));
   DG_TRY(run(h, "adj_pfrag", [&] {
     const long nf = 256L * h->ntb * h->ntb;
     hipLaunchKernelGGL(k_pfrag_full, dim3(nblocks(nf, 256)), dim3(256), 0, h->stream, h->last_P, h->nbf, h->ntb,
@@ -3266,34 +3271,41 @@ void launch_contract_small(dg_handle* h, const double* u0, const double* u1, int
   }
 }
 
-// two outputs on 4 x 2 tile blocks, one output on 4 x 4 (the same 16 MFMAs per step)
+// two outputs on 4 x 2 tile blocks, one output on 4 x 4 (the same 16 MFMAs per step);
+// list/loff (optional): point-block lists per tile block to use instead of the
+// handle's (the screened kernel then runs whatever the handle's screening)
 template <typename T, int NOUT>
-void launch_contract2d(dg_handle* h, const T* phi, const double* u0, const double* u1) {
+void launch_contract2d(dg_handle* h, const T* phi, const double* u0, const double* u1, const int* list = nullptr,
+                       const int* loff = nullptr) {
   constexpr int NJ = NOUT == 2 ? 2 : 4;
   const TileBlocks& t = h->tb[NOUT == 2 ? 0 : 1];
+  const int* bl = list ? list : t.list;
+  const int* bo = list ? loff : t.loff;
+  const bool dense = h->screen == 0.0 && !list;
   const dim3 grid(round_up((long)t.ncb * t.nsplit, kXcds));
   if constexpr (sizeof(T) == 8) {
     constexpr int D = NJ == 2 ? 4 : 3;  // ring slots: 3 work-groups per CU within 160 KB of LDS
-    if (h->screen == 0.0)
+    if (dense)
       hipLaunchKernelGGL((k_contract2d_lds<NOUT, NJ, D, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0, u1,
-                         t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+                         t.cb, bl, bo, t.order, 4 * t.ncb, t.nsplit, h->cpart);
     else
       hipLaunchKernelGGL((k_contract2d_lds<NOUT, NJ, D, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi, u0,
-                         u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+                         u1, t.cb, bl, bo, t.order, 4 * t.ncb, t.nsplit, h->cpart);
   } else {
     constexpr int D = NJ == 2 ? 12 : 9;  // ring slots: 2 work-groups per CU within 160 KB of LDS
-    if (h->screen == 0.0)
+    if (dense)
       hipLaunchKernelGGL((k_contract2d_lds_f32<NOUT, NJ, D, true>), grid, dim3(256), 0, h->stream, phi, h->ldphi,
-                         u0, u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+                         u0, u1, t.cb, bl, bo, t.order, 4 * t.ncb, t.nsplit, h->cpart);
     else
       hipLaunchKernelGGL((k_contract2d_lds_f32<NOUT, NJ, D, false>), grid, dim3(256), 0, h->stream, phi, h->ldphi,
-                         u0, u1, t.cb, t.list, t.loff, t.order, 4 * t.ncb, t.nsplit, h->cpart);
+                         u0, u1, t.cb, bl, bo, t.order, 4 * t.ncb, t.nsplit, h->cpart);
   }
 }
 
 // sa (optional): the build's scalars, evaluated by one more work-group of the reduction
+// list/loff (optional, one output, 2-D path): point-block lists per tile block
 int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out0, double* out1,
-                   const ScalarArgs* sa = nullptr) {
+                   const ScalarArgs* sa = nullptr, const int* list = nullptr, const int* loff = nullptr) {
   const int nout = u1 ? 2 : 1;
   const bool mx = h->precision == DG_PREC_MIXED;
   // up to 80 functions the dense all-tiles kernel (FP64, or FP32 in mixed mode)
@@ -3312,7 +3324,7 @@ int stage_contract(dg_handle* h, const double* u0, const double* u1, double* out
       if (nout == 2)
         launch_contract2d<double, 2>(h, h->phi, u0, u1);
       else
-        launch_contract2d<double, 1>(h, h->phi, u0, u0);
+        launch_contract2d<double, 1>(h, h->phi, u0, u0, list, loff);
     }
   }));
   const bool two_d = !small;
@@ -4196,10 +4208,30 @@ int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad) {
                        h->adj_udp);
   }));
   if (dEdP) {
-    // dE/dP = Phi^T diag(udp) Phi with the FP64 contraction in either precision mode
+    // dE/dP = Phi^T diag(udp) Phi with the FP64 contraction in either precision mode.
+    // Unscreened handles on the 2-D path take, per tile block, only the point
+    // blocks where both its row and its column tiles are within reach
+    // (k_tile_mask; the terms left out are below e^-100 of an amplitude).
+    const bool lists = h->screen == 0.0 && h->ntb > 5 && h->ntb <= 32;
+    if (lists && !h->adj_list) {
+      DG_TRY(ensure_tile_mask(h));
+      const long nblk_m = h->npts_pad / 64;
+      std::vector<unsigned> mk(nblk_m);
+      std::vector<int> cb(4 * (size_t)h->tb[1].ncb), l2, o2;
+      HIP_TRY(hipMemcpyAsync(mk.data(), h->adj_tmask, nblk_m * 4, hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipMemcpyAsync(cb.data(), h->tb[1].cb, cb.size() * 4, hipMemcpyDeviceToHost, h->stream));
+      DG_TRY(sync(h));
+      tile_block_lists(mk, cb, l2, o2);
+      DG_TRY(dalloc(&h->adj_list, std::max<size_t>(l2.size(), 1)));
+      DG_TRY(dalloc(&h->adj_loff, o2.size()));
+      HIP_TRY(hipMemcpyAsync(h->adj_list, l2.data(), l2.size() * 4, hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->adj_loff, o2.data(), o2.size() * 4, hipMemcpyHostToDevice, h->stream));
+      DG_TRY(sync(h));
+    }
     const int saved = h->precision;
     h->precision = DG_PREC_FP64;
-    const int rc = stage_contract(h, h->adj_udp, nullptr, h->adj_dP, nullptr);
+    const int rc = stage_contract(h, h->adj_udp, nullptr, h->adj_dP, nullptr, nullptr, lists ? h->adj_list : nullptr,
+                                  lists ? h->adj_loff : nullptr);
     h->precision = saved;
     DG_TRY(rc);
     DG_TRY(copy_out(h, dEdP, h->adj_dP, (size_t)h->nbf * h->nbf));
