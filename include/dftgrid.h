@@ -164,7 +164,9 @@ int dg_density_at_points_noalloc(const dg_molecule* mol, const dg_basis* basis, 
  *   hold_c = 0: c varies with P and R (the derivative of the E_xc the SCF reports).
  * dEdP [nbf*nbf] column-major; grad [natoms*3] = dE_xc/dR_A (atom-major xyz;
  * the force is -grad), including the basis-centre, grid-motion and Becke-weight
- * terms.  Needs a corrected density (dg_correct_densities or a build) and the
+ * terms.  Shells out of reach of a point (exp(-alpha r^2) below e^-100 for
+ * their most diffuse primitive) are left out of both, which is below the
+ * rounding of the sums.  Needs a corrected density (dg_correct_densities or a build) and the
  * density matrix of that evaluation still in the device buffer it was read from.
  * Either output may be NULL.  Returns DG_ERR_STATE before dg_xc. */
 int dg_xc_adjoint(dg_handle* h, int hold_c, double* dEdP, double* grad);
