@@ -33,7 +33,7 @@ def _lib_adjoint(mol, nrad, nang, lmax, P, hold_c, precision=G.PREC_FP64):
     return g, XC, exc, dP, grad
 
 
-@pytest.mark.parametrize("name", ["h2o", "ch4_d", "benzene_sto3g"])
+@pytest.mark.parametrize("name", ["h2o", "ch4_d", "benzene_sto3g", "c10h22"])
 @pytest.mark.parametrize("hold_c", [False, True])
 def test_adjoint_matches_oracle(name, hold_c):
     from oracle import adjoint as AD
@@ -43,6 +43,11 @@ def test_adjoint_matches_oracle(name, hold_c):
     elif name == "ch4_d":
         mol, grid = with_d_shell(M.ch4("sto3g")), (20, 26, 4)
         P = _density(mol, 0.05)
+    elif name == "c10h22":
+        # 134 functions in 9 tiles: the streamed T kernel with its per-block tile
+        # mask, the listed dE/dP contraction and the shell reach cut all take part
+        mol, grid = M.alkane(10), (10, 26, 2)
+        P = _density(mol)
     else:
         mol, grid = M.benzene("sto3g"), (16, 26, 4)
         P = _density(mol)
