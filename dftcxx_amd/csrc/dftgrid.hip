@@ -2267,8 +2267,8 @@ __global__ __launch_bounds__(256) void k_density_grad(
 }
 
 // Becke-weight term: sum_p w0_p a_p d wb_p / d R_B (w0 = weight without the
-// Becke factor), one wave per 64 points with the atom distances and the cell
-// functions P_n in LDS.  The B != owner terms are added to part[B][d][blk]
+// Becke factor), one wave per 64 points with each point's active cell functions
+// (atom and P_n) in LDS.  The B != owner terms are added to part[B][d][blk]
 // (after k_force_basis wrote it), the owner's share to mot[d][p].
 // d wb/d R_B = (dP_a/dR_B)/Z - (wb/Z) sum_n dP_n/dR_B for B != a, and
 // d wb/d R_a = -sum_{B != a} d wb/d R_B (translating everything leaves wb fixed).
